@@ -1,0 +1,126 @@
+// Internal declarations shared by the host code and the HIP kernels.
+#ifndef CHARM_AMD_INTERNAL_H
+#define CHARM_AMD_INTERNAL_H
+
+#include <cstddef>
+#include <cstdint>
+
+#include "charm_amd.h"
+
+namespace charm_amd {
+
+bool nearly_equal(double a, double b, double eps);
+void err_set(charm_err *err, const char *file, size_t line, const char *func, int code,
+             const char *msg);
+void err_propagate(charm_err *err, const char *file, size_t line, const char *func);
+
+#define CHARM_AMD_ERR(err, code, msg) \
+    ::charm_amd::err_set((err), __FILE__, __LINE__, __func__, (code), (msg))
+
+// Number of (n, m) pairs with 0 <= m <= n <= nmax.
+inline size_t tri_len(unsigned long nmax)
+{
+    return ((size_t)(nmax + 1) * (size_t)(nmax + 2)) / 2;
+}
+
+// Offset of column m (entries n = m..nmax) in the m-major triangle of degree
+// nmax (src/shc/shc_init_chunk.c:139-150).
+inline size_t col_off(unsigned long nmax, unsigned long m)
+{
+    return (size_t)m * (nmax + 1) - ((size_t)m * (m - 1)) / 2;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel launch parameters
+// ---------------------------------------------------------------------------
+// Lumped-coefficient synthesis on rings of a grid.
+struct ShsGridArgs {
+    int nmax;          // degree of the synthesis
+    int nring;         // rings evaluated directly ("north" half when folded)
+    int nlat;          // rows of the output
+    int mcount;        // columns written per row (>= nmax+1; extra columns = 0)
+    int ntile;         // ring tiles
+    int unit;          // all radii equal to R: no (R/r)^k factors
+    const double *t;   // [nring] sin(lat)
+    const double *sx;  // sectorial seeds, [(m-1)*sld + i]
+    const int *six;
+    size_t sld;
+    const double *rp;  // unit == 0: (R/r)^k, [k*rld + row], k = 0..nmax+1
+    const double *ratio;  // unit == 0: R/r per row
+    size_t rld;
+    const unsigned char *mirror;  // [nring] 1 if row nlat-1-i is the mirror ring
+    const void *ab;    // (a_nm, b_nm) table, m-major, degree ab_nt
+    int ab_nt;
+    const void *cs;    // (C_nm, S_nm), m-major, degree nmax
+    void *out;         // [row * ldo + m] double2
+    size_t ldo;
+    double mur;        // mu / R
+    int fft_layout;    // 1: out = Hermitian half-spectrum for c2r; 0: raw (a, b)
+};
+
+struct ShaArgs {
+    int nmax;
+    int nring;         // ring pairs ("north" rings, incl. the equator)
+    int nlat;
+    int ntile;
+    const double *t;   // [nring]
+    const double *sx;
+    const int *six;
+    size_t sld;
+    const unsigned char *mirror;
+    const double *cw;  // [nlat] quadrature weight times the global constant
+    const void *spec;  // r2c spectrum of f, [row * lds + k] double2
+    size_t lds;
+    const void *ab;
+    int ab_nt;
+    void *part;        // [tile][tri_len(nmax)] double2 partial sums
+};
+
+struct SctrArgs {
+    int nmax;
+    size_t npoint;
+    int nchunk;
+    const int *mchunk;  // [nchunk+1] order ranges
+    int unit;
+    double R;
+    const double *lat_t;  // sin(lat) per point
+    const double *lat_u;  // cos(lat) per point
+    const double *lon;
+    const double *r;
+    const double *dm;     // [nmax+1]
+    const void *ab;
+    int ab_nt;
+    const void *cs;
+    double *part;         // [chunk * npoint + i]
+};
+
+// Kernel launchers (kernels.hip).  All return hipError_t as int.
+int launch_ab_table(int nt, const double *r_dev, const double *ri_dev, void *ab_dev,
+                    void *stream);
+int launch_seeds(int nring, const double *u_dev, const double *dm_dev, int nmax, size_t sld,
+                 double *sx_dev, int *six_dev, void *stream);
+int launch_rpows(int nrow, const double *ratio_dev, int kmax, size_t rld, double *rp_dev,
+                 void *stream);
+int launch_shs_grid(const ShsGridArgs &a, void *stream);
+int launch_lon_direct(int nlat, int nmax, size_t nlon, const double *lon_dev, const void *lc_dev,
+                      size_t ldl, double mur, double *f_dev, void *stream);
+int launch_sha(const ShaArgs &a, void *stream);
+int launch_sha_reduce(int nmax, int ntile, const void *part_dev, int nmax_out,
+                      const double *scale_dev, double *c_dev, double *s_dev, size_t stride,
+                      void *stream);
+int launch_sctr(const SctrArgs &a, void *stream);
+int launch_sctr_reduce(size_t npoint, int nchunk, const double *part_dev, double mur,
+                       double *f_dev, void *stream);
+int launch_pack_cs(const double *c_dev, const double *s_dev, size_t n, void *cs_dev,
+                   void *stream);
+int launch_sincos(size_t n, const double *lat_dev, double *t_dev, double *u_dev, void *stream);
+int launch_all_near(size_t n, const double *x_dev, double ref, double eps, int *result,
+                    void *stream);
+
+// Ring-tile width of the grid kernels (ring pairs per lane times 64).
+constexpr int SHS_K = 2;
+constexpr int SHA_K = 4;
+
+}  // namespace charm_amd
+
+#endif

@@ -1,0 +1,925 @@
+// HIP kernels for point synthesis (SHS) and analysis (SHA) on gfx950.
+//
+// Mapping.  Lanes run over latitude rings (or scattered points), work-groups
+// over orders m; the degree recurrence runs sequentially in each lane, with
+// the recurrence coefficients (a_nm, b_nm) and the model coefficients
+// (C_nm, S_nm) as wave-uniform scalar loads.  Equatorially symmetric rings are
+// folded: one Legendre evaluation per ring pair, with even and odd (n + m)
+// sums kept apart so that P_nm(-t) = (-1)^(n+m) P_nm(t) gives the mirror ring.
+//
+// X-numbers (extended exponent, radix 2^960) carry the sectorial seeds and the
+// first tesseral terms of each column exactly as in CHarm
+// (src/leg/leg_func_xnum.h:299-443); once both terms of a lane have a zero
+// exponent that lane continues in ordinary doubles.  A wave switches to the
+// tight all-double loop when every lane has switched.
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+
+namespace charm_amd {
+
+namespace {
+
+constexpr double BIG = 0x1p960;
+constexpr double BIGI = 0x1p-960;
+constexpr double BIGS = 0x1p480;
+constexpr double BIGSI = 0x1p-480;
+// src/prec.h:301 (ROOT3 in the double build)
+constexpr double ROOT3 = 1.73205080756887729353;
+
+__device__ __forceinline__ size_t dcol_off(long nmax, long m)
+{
+    return (size_t)m * (size_t)(nmax + 1) - ((size_t)m * (size_t)(m - 1)) / 2;
+}
+
+__device__ __forceinline__ size_t tri_len_dev(long nmax)
+{
+    return ((size_t)(nmax + 1) * (size_t)(nmax + 2)) / 2;
+}
+
+// X-number to double (src/leg/leg_func_xnum.h:304-314)
+__device__ __forceinline__ double x2f(double x, int ix)
+{
+    return (ix == 0) ? x : ((ix == -1) ? x * BIGI : ((ix < -1) ? 0.0 : x * BIG));
+}
+
+// Keep |x| in [BIGSI, BIGS) (src/leg/leg_func_xnum.h:318-330)
+__device__ __forceinline__ void xnorm(double &x, int &ix)
+{
+    double w = fabs(x);
+    if (w >= BIGS) {
+        x *= BIGI;
+        ix += 1;
+    } else if (w < BIGSI) {
+        x *= BIG;
+        ix -= 1;
+    }
+}
+
+__device__ __forceinline__ bool wave_any(bool p)
+{
+    return __ballot(p) != 0;
+}
+
+// One tesseral step P_n = (a_n t) P_{n-1} - b_n P_{n-2} for a lane that may
+// still be in X-number mode (src/leg/leg_func_xnum.h:366-443).  Lanes with
+// ds set run the ordinary recurrence.  Returns P_n as an ordinary double.
+//
+// Scaling by BIG^k is done with ldexp, which rounds exactly like the
+// reference's multiplication by a power of two (also on underflow), so
+// x * BIG^ix == ldexp(x, 960 * ix) for the X-to-double conversion.  The rare
+// exponent-alignment cases are taken only when a lane of the wave needs them.
+__device__ __forceinline__ double leg_step_x(double &x, double &y, int &ix, int &iy, bool &ds,
+                                             double at, double b)
+{
+    const int ixy = ix - iy;
+    double xx = x, yy = y;
+    int iz = ix;
+    if (wave_any(ixy != 0)) {
+        yy = (ixy == 1) ? ldexp(y, -960) : ((ixy > 1) ? 0.0 : y);
+        xx = (ixy == -1) ? ldexp(x, -960) : ((ixy < -1) ? 0.0 : x);
+        iz = (ixy < 0) ? iy : ix;
+    }
+    double z = fma(at, xx, -(b * yy));
+    const double w = fabs(z);
+    const bool up = !ds && (w >= BIGS);
+    const bool dn = !ds && (w < BIGSI);
+    if (wave_any(up || dn)) {
+        z = ldexp(z, up ? -960 : (dn ? 960 : 0));
+        iz += up ? 1 : (dn ? -1 : 0);
+    }
+    y = x;
+    iy = ix;
+    x = z;
+    ix = iz;
+    ds = ds || (ix == 0 && iy == 0);
+    return ldexp(z, 960 * iz);
+}
+
+__device__ __forceinline__ int uniform(int v)
+{
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// (a_nm, b_nm) table, m-major, degree nt (src/leg/leg_func_anm_bnm.c:14-65)
+// ---------------------------------------------------------------------------
+__global__ void ab_table_kernel(int nt, const double *__restrict__ r, const double *__restrict__ ri,
+                                double2 *__restrict__ ab)
+{
+    const int m = blockIdx.x;
+    double2 *col = ab + dcol_off(nt, m) - m;
+    for (int n = m + threadIdx.x; n <= nt; n += blockDim.x) {
+        double a = 0.0, b = 0.0;
+        if (n == m + 1) {
+            a = r[2 * m + 3];
+        } else if (n >= m + 2) {
+            double w = r[2 * n + 1] * ri[n - m] * ri[n + m];
+            a = r[2 * n - 1] * w;
+            b = r[n - m - 1] * r[n + m - 1] * ri[2 * n - 3] * w;
+        }
+        col[n] = make_double2(a, b);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sectorial seeds P_mm, m = 1..nmax, as X-numbers, one lane per ring
+// (src/leg/leg_func_prepare.c:18-69).  Row m-1 of the table holds P_mm.
+// ---------------------------------------------------------------------------
+__global__ void seed_kernel(int nring, const double *__restrict__ u, const double *__restrict__ dm,
+                            int nmax, size_t sld, double *__restrict__ sx, int *__restrict__ six)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nring) return;
+    const double uu = u[i];
+    double x = ROOT3 * uu;
+    int ix = 0;
+    sx[i] = x;
+    six[i] = 0;
+    for (int n = 1; n < nmax; n++) {
+        x = (dm[n] * uu) * x;
+        xnorm(x, ix);
+        sx[(size_t)n * sld + i] = x;
+        six[(size_t)n * sld + i] = ix;
+    }
+}
+
+// (R/r)^k, k = 0..kmax, by repeated products (src/shs/shs_rpows.c:16-37)
+__global__ void rpows_kernel(int nrow, const double *__restrict__ ratio, int kmax, size_t rld,
+                             double *__restrict__ rp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrow) return;
+    const double q = ratio[i];
+    double v = 1.0;
+    rp[i] = v;
+    for (int k = 1; k <= kmax; k++) {
+        v = v * q;
+        rp[(size_t)k * rld + i] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SHS: lumped coefficients on grid rings
+// (src/shs/shs_point_kernel.c:1014-1623 for DR = DLAT = DLON = 0)
+//
+// Block = 4 waves = 4 consecutive orders m0..m0+3 for one tile of 64*K ring
+// pairs.  Results are staged in LDS and written as 64-byte runs per row.
+// Blocks are numbered so that one XCD (blockIdx mod 8) owns every 8th order
+// quad: all tiles of a column share that XCD's L2.
+// ---------------------------------------------------------------------------
+template <int K, bool UNIT>
+__global__ __launch_bounds__(256) void shs_grid_kernel(ShsGridArgs A)
+{
+    __shared__ double2 stage[2][64 * K][4];
+    const int lane = threadIdx.x & 63;
+    const int wave = uniform(threadIdx.x >> 6);
+    const int b = blockIdx.x;
+    const int tile = (b >> 3) % A.ntile;
+    const int quad = ((b >> 3) / A.ntile) * 8 + (b & 7);
+    const int m0 = quad * 4;
+    if (m0 >= A.mcount) return;
+    const int m = m0 + wave;
+    const int nmax = A.nmax;
+
+    int ring[K];
+    bool valid[K];
+    double t[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        ring[k] = tile * 64 * K + k * 64 + lane;
+        valid[k] = ring[k] < A.nring;
+        t[k] = valid[k] ? A.t[ring[k]] : 0.0;
+    }
+
+    double aN[K], bN[K], aS[K], bS[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) aN[k] = bN[k] = aS[k] = bS[k] = 0.0;
+
+    if (m <= nmax) {
+        const double2 *__restrict__ abm = (const double2 *)A.ab + dcol_off(A.ab_nt, m) - m;
+        const double2 *__restrict__ csm = (const double2 *)A.cs + dcol_off(nmax, m) - m;
+        double x[K], y[K];
+        int ix[K], iy[K];
+        bool ds[K];
+        // even/odd (n + m) partial sums (UNIT) or north/south sums (!UNIT)
+        double aE[K], aO[K], bE[K], bO[K];
+        double rpN[K], rpS[K], qN[K], qS[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            aE[k] = aO[k] = bE[k] = bO[k] = 0.0;
+            x[k] = y[k] = 0.0;
+            ix[k] = iy[k] = 0;
+            ds[k] = true;
+            rpN[k] = rpS[k] = qN[k] = qS[k] = 1.0;
+            if (!UNIT && valid[k]) {
+                const int rs = A.mirror[ring[k]] ? A.nlat - 1 - ring[k] : ring[k];
+                qN[k] = A.ratio[ring[k]];
+                qS[k] = A.ratio[rs];
+                rpN[k] = A.rp[(size_t)(m + 1) * A.rld + ring[k]];
+                rpS[k] = A.rp[(size_t)(m + 1) * A.rld + rs];
+            }
+        }
+        // accumulate term n with Legendre value p; par = (n + m) & 1
+#define SHS_ACC(k, p, c, par)                                              \
+    do {                                                                   \
+        if (UNIT) {                                                        \
+            if ((par) == 0) {                                              \
+                aE[k] = fma((p), (c).x, aE[k]);                            \
+                bE[k] = fma((p), (c).y, bE[k]);                            \
+            } else {                                                       \
+                aO[k] = fma((p), (c).x, aO[k]);                            \
+                bO[k] = fma((p), (c).y, bO[k]);                            \
+            }                                                              \
+        } else {                                                           \
+            const double pn_ = rpN[k] * (p);                               \
+            const double ps_ = ((par) == 0) ? rpS[k] * (p) : -(rpS[k] * (p)); \
+            aE[k] = fma(pn_, (c).x, aE[k]);                                \
+            bE[k] = fma(pn_, (c).y, bE[k]);                                \
+            aO[k] = fma(ps_, (c).x, aO[k]);                                \
+            bO[k] = fma(ps_, (c).y, bO[k]);                                \
+            rpN[k] *= qN[k];                                               \
+            rpS[k] *= qS[k];                                               \
+        }                                                                  \
+    } while (0)
+
+        int n;
+        if (m == 0) {
+            // zonal: P00 = 1, P10 = sqrt(3) t, then the ordinary recurrence
+            const double2 c0 = csm[0];
+#pragma unroll
+            for (int k = 0; k < K; k++) SHS_ACC(k, 1.0, c0, 0);
+            if (nmax >= 1) {
+                const double2 c1 = csm[1];
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const double p1 = ROOT3 * t[k];
+                    SHS_ACC(k, p1, c1, 1);
+                    x[k] = p1;
+                    y[k] = 1.0;
+                }
+            }
+            n = 2;
+        } else {
+            const double2 cmm = csm[m];
+            double xs[K];
+            int ixs[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                xs[k] = valid[k] ? A.sx[(size_t)(m - 1) * A.sld + ring[k]] : 0.0;
+                ixs[k] = valid[k] ? A.six[(size_t)(m - 1) * A.sld + ring[k]] : 0;
+                SHS_ACC(k, x2f(xs[k], ixs[k]), cmm, 0);
+            }
+            if (m < nmax) {
+                const double2 ab1 = abm[m + 1];
+                const double2 c1 = csm[m + 1];
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    y[k] = xs[k];
+                    iy[k] = ixs[k];
+                    double xx = (ab1.x * t[k]) * xs[k];
+                    int ixx = ixs[k];
+                    xnorm(xx, ixx);
+                    x[k] = xx;
+                    ix[k] = ixx;
+                    ds[k] = !valid[k];
+                    SHS_ACC(k, x2f(xx, ixx), c1, 1);
+                }
+            }
+            n = m + 2;
+        }
+
+        // Tiles of 8 degrees; (n + m) is even at every tile start.  While any
+        // lane of the wave is still in X-number mode the tile takes the
+        // general step; once none is, the wave stays in the tight loop (a lane
+        // never returns to X-numbers).
+        for (;;) {
+            bool xmode = false;
+#pragma unroll
+            for (int k = 0; k < K; k++) xmode = xmode || !ds[k];
+            if (n + 7 > nmax || !wave_any(xmode)) break;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const double2 abj = abm[n + j];
+                const double2 cj = csm[n + j];
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const double p = leg_step_x(x[k], y[k], ix[k], iy[k], ds[k], abj.x * t[k], abj.y);
+                    SHS_ACC(k, p, cj, j & 1);
+                }
+            }
+            n += 8;
+        }
+        for (; n + 7 <= nmax; n += 8) {
+#pragma unroll
+            for (int j = 0; j < 8; j += 2) {
+                const double2 ab0 = abm[n + j], ab1 = abm[n + j + 1];
+                const double2 c0 = csm[n + j], c1 = csm[n + j + 1];
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const double z = fma(ab0.x * t[k], x[k], -(ab0.y * y[k]));
+                    SHS_ACC(k, z, c0, 0);
+                    const double w = fma(ab1.x * t[k], z, -(ab1.y * x[k]));
+                    SHS_ACC(k, w, c1, 1);
+                    y[k] = z;
+                    x[k] = w;
+                }
+            }
+        }
+        for (; n <= nmax; n++) {
+            const double2 abj = abm[n];
+            const double2 cj = csm[n];
+            const int par = (n + m) & 1;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const double p = leg_step_x(x[k], y[k], ix[k], iy[k], ds[k], abj.x * t[k], abj.y);
+                SHS_ACC(k, p, cj, par);
+            }
+        }
+#undef SHS_ACC
+        if (m == 0) {
+            // S_n0 does not exist; the zonal sums have no sine part
+#pragma unroll
+            for (int k = 0; k < K; k++) bE[k] = bO[k] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (UNIT) {
+                aN[k] = aE[k] + aO[k];
+                bN[k] = bE[k] + bO[k];
+                aS[k] = aE[k] - aO[k];
+                bS[k] = bE[k] - bO[k];
+            } else {
+                aN[k] = aE[k];
+                bN[k] = bE[k];
+                aS[k] = aO[k];
+                bS[k] = bO[k];
+            }
+        }
+    }
+
+    // stage: X_m = c (a_m - i b_m) mu/R with c = 1 (m = 0) or 1/2
+    // (src/shs/shs_grd_fft_lc.c:24-44,72; shs_grd_fft.c:110-112)
+    const double sc = A.fft_layout ? ((m == 0) ? A.mur : 0.5 * A.mur) : 1.0;
+    const double sb = A.fft_layout ? -sc : sc;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        stage[0][k * 64 + lane][wave] = make_double2(sc * aN[k], sb * bN[k]);
+        stage[1][k * 64 + lane][wave] = make_double2(sc * aS[k], sb * bS[k]);
+    }
+    __syncthreads();
+    double2 *out = (double2 *)A.out;
+    for (int e = threadIdx.x; e < 2 * 64 * K * 4; e += 256) {
+        const int col = e & 3, rowsel = e >> 2;
+        const int ns = rowsel / (64 * K), rl = rowsel % (64 * K);
+        const int i = tile * 64 * K + rl;
+        const int mm = m0 + col;
+        if (i >= A.nring || mm >= A.mcount) continue;
+        int row = i;
+        if (ns) {
+            if (!A.mirror[i]) continue;
+            row = A.nlat - 1 - i;
+        }
+        out[(size_t)row * A.ldo + mm] = stage[ns][rl][col];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Longitude sums for grids that do not admit an FFT (custom longitudes):
+// f_ij = mu/R sum_m (a_im cos m lon_j + b_im sin m lon_j)
+// (src/shs/shs_grd_lr.c; evaluated directly)
+// ---------------------------------------------------------------------------
+__global__ void lon_direct_kernel(int nlat, int nmax, size_t nlon, const double *__restrict__ lon,
+                                  const double2 *__restrict__ lc, size_t ldl, double mur,
+                                  double *__restrict__ f)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j >= nlon || i >= nlat) return;
+    const double l = lon[j];
+    double s = 0.0;
+    for (int m = 0; m <= nmax; m++) {
+        double sn, cn;
+        sincos((double)m * l, &sn, &cn);
+        const double2 ab = lc[(size_t)i * ldl + m];
+        s = s + (ab.x * cn + ab.y * sn);
+    }
+    f[(size_t)i * nlon + j] = mur * s;
+}
+
+// ---------------------------------------------------------------------------
+// SHA: C_nm = sum over ring pairs of P_nm(t_i) (A_i +- A_mirror)
+// (src/sha/sha_point.c:490-980)
+//
+// Block = 4 waves = orders m0..m0+3 for one tile of 64*K ring pairs.  Each lane
+// sums its K ring pairs in registers; every 8 degrees the 64 lanes' partial
+// sums are transposed through LDS and reduced in a fixed order.  One partial
+// per (tile, n, m) goes to global memory; sha_reduce_kernel adds the tiles.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int RED_LD = 65;  // padded row of the per-wave transpose buffer
+
+// Reduce v[0..15] (= 8 degrees x (C, S)) over the 64 lanes of a wave and
+// store the first 2*cnt sums at dst[0..2*cnt).
+__device__ __forceinline__ void wave_reduce_store(const double v[16], int cnt, double *red,
+                                                  double *dst, int lane)
+{
+#pragma unroll
+    for (int j = 0; j < 16; j++) red[j * RED_LD + lane] = v[j];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int j = lane >> 2, q = lane & 3;
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < 16; r++) s += red[j * RED_LD + q * 16 + r];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if (q == 0 && j < 2 * cnt) dst[j] = s;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+}  // namespace
+
+template <int K>
+__global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
+{
+    constexpr int STAGE = 2 * 64 * K * 4 * 2;   // doubles
+    constexpr int RED = 4 * 16 * RED_LD;        // doubles
+    __shared__ double lds[STAGE > RED ? STAGE : RED];
+    const int lane = threadIdx.x & 63;
+    const int wave = uniform(threadIdx.x >> 6);
+    const int b = blockIdx.x;
+    const int tile = (b >> 3) % A.ntile;
+    const int quad = ((b >> 3) / A.ntile) * 8 + (b & 7);
+    const int m0 = quad * 4;
+    if (m0 > A.nmax) return;
+    const int m = m0 + wave;
+    const int nmax = A.nmax;
+
+    // stage the weighted spectra A = cw Re F, B = -cw Im F of the tile's rows
+    // for orders m0..m0+3 (64-byte runs per row)
+    double2 *stage = (double2 *)lds;
+    const double2 *spec = (const double2 *)A.spec;
+    for (int e = threadIdx.x; e < 2 * 64 * K * 4; e += 256) {
+        const int col = e & 3, rowsel = e >> 2;
+        const int ns = rowsel / (64 * K), rl = rowsel % (64 * K);
+        const int i = tile * 64 * K + rl;
+        const int mm = m0 + col;
+        double2 v = make_double2(0.0, 0.0);
+        if (i < A.nring && mm <= nmax) {
+            int row = i;
+            if (ns) row = A.mirror[i] ? A.nlat - 1 - i : -1;
+            if (row >= 0) {
+                const double2 F = spec[(size_t)row * A.lds + mm];
+                const double cw = A.cw[row];
+                v = make_double2(cw * F.x, -cw * F.y);
+            }
+        }
+        stage[e] = v;
+    }
+    __syncthreads();
+
+    int ring[K];
+    double t[K], ampC[K], ammC[K], ampS[K], ammS[K];
+    bool valid[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        ring[k] = tile * 64 * K + k * 64 + lane;
+        valid[k] = ring[k] < A.nring;
+        t[k] = valid[k] ? A.t[ring[k]] : 0.0;
+        const double2 N = stage[((0 * 64 * K) + k * 64 + lane) * 4 + wave];
+        const double2 S = stage[((1 * 64 * K) + k * 64 + lane) * 4 + wave];
+        // fold (src/sha/sha_point.c:728-748); rows without a mirror hold S = 0
+        ampC[k] = N.x + S.x;
+        ammC[k] = N.x - S.x;
+        ampS[k] = N.y + S.y;
+        ammS[k] = N.y - S.y;
+    }
+    __syncthreads();
+    if (m > nmax) return;
+
+    double *red = lds + wave * 16 * RED_LD;
+    const double2 *__restrict__ abm = (const double2 *)A.ab + dcol_off(A.ab_nt, m) - m;
+    double *dst = (double *)((double2 *)A.part + (size_t)tile * tri_len_dev(nmax) + dcol_off(nmax, m) - m);
+
+    double x[K], y[K];
+    int ix[K], iy[K];
+    bool ds[K];
+    double v[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) v[j] = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        x[k] = y[k] = 0.0;
+        ix[k] = iy[k] = 0;
+        ds[k] = true;
+    }
+
+    // degrees m and m + 1
+    if (m == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            v[0] += 1.0 * ampC[k];
+            if (nmax >= 1) {
+                const double p1 = ROOT3 * t[k];
+                v[2] = fma(p1, ammC[k], v[2]);
+                x[k] = p1;
+                y[k] = 1.0;
+            }
+        }
+    } else {
+        const double2 ab1 = (m < nmax) ? abm[m + 1] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const double xs = valid[k] ? A.sx[(size_t)(m - 1) * A.sld + ring[k]] : 0.0;
+            const int ixs = valid[k] ? A.six[(size_t)(m - 1) * A.sld + ring[k]] : 0;
+            const double p0 = x2f(xs, ixs);
+            v[0] = fma(p0, ampC[k], v[0]);
+            v[1] = fma(p0, ampS[k], v[1]);
+            if (m < nmax) {
+                y[k] = xs;
+                iy[k] = ixs;
+                double xx = (ab1.x * t[k]) * xs;
+                int ixx = ixs;
+                xnorm(xx, ixx);
+                x[k] = xx;
+                ix[k] = ixx;
+                ds[k] = !valid[k];
+                const double p1 = x2f(xx, ixx);
+                v[2] = fma(p1, ammC[k], v[2]);
+                v[3] = fma(p1, ammS[k], v[3]);
+            }
+        }
+    }
+    wave_reduce_store(v, (m < nmax) ? 2 : 1, red, dst + 2 * m, lane);
+
+    int n = m + 2;
+    for (;;) {
+        bool xmode = false;
+#pragma unroll
+        for (int k = 0; k < K; k++) xmode = xmode || !ds[k];
+        if (n + 7 > nmax || !wave_any(xmode)) break;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const double2 abj = abm[n + j];
+            double c = 0.0, s = 0.0;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const double p = leg_step_x(x[k], y[k], ix[k], iy[k], ds[k], abj.x * t[k], abj.y);
+                c = fma(p, (j & 1) ? ammC[k] : ampC[k], c);
+                s = fma(p, (j & 1) ? ammS[k] : ampS[k], s);
+            }
+            v[2 * j] = c;
+            v[2 * j + 1] = s;
+        }
+        wave_reduce_store(v, 8, red, dst + 2 * n, lane);
+        n += 8;
+    }
+    for (; n + 7 <= nmax; n += 8) {
+        {
+#pragma unroll
+            for (int j = 0; j < 8; j += 2) {
+                const double2 ab0 = abm[n + j], ab1 = abm[n + j + 1];
+                double c0 = 0.0, s0 = 0.0, c1 = 0.0, s1 = 0.0;
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const double z = fma(ab0.x * t[k], x[k], -(ab0.y * y[k]));
+                    c0 = fma(z, ampC[k], c0);
+                    s0 = fma(z, ampS[k], s0);
+                    const double w = fma(ab1.x * t[k], z, -(ab1.y * x[k]));
+                    c1 = fma(w, ammC[k], c1);
+                    s1 = fma(w, ammS[k], s1);
+                    y[k] = z;
+                    x[k] = w;
+                }
+                v[2 * j] = c0;
+                v[2 * j + 1] = s0;
+                v[2 * j + 2] = c1;
+                v[2 * j + 3] = s1;
+            }
+        }
+        wave_reduce_store(v, 8, red, dst + 2 * n, lane);
+    }
+    if (n <= nmax) {
+        const int cnt = nmax - n + 1;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            double c = 0.0, s = 0.0;
+            if (j < cnt) {
+                const double2 abj = abm[n + j];
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const double p = leg_step_x(x[k], y[k], ix[k], iy[k], ds[k], abj.x * t[k], abj.y);
+                    c = fma(p, (j & 1) ? ammC[k] : ampC[k], c);
+                    s = fma(p, (j & 1) ? ammS[k] : ampS[k], s);
+                }
+            }
+            v[2 * j] = c;
+            v[2 * j + 1] = s;
+        }
+        wave_reduce_store(v, cnt, red, dst + 2 * n, lane);
+    }
+}
+
+// Sum the tile partials in tile order, rescale by scale[n] (if given) and
+// write the coefficients of a model of degree nmax_out (zeros above nmax)
+// (src/sha/sha_point.c:381,1088-1108; src/shc/shc_rescale.c:24-90).
+__global__ void sha_reduce_kernel(int nmax, int ntile, const double2 *__restrict__ part,
+                                  int nmax_out, const double *__restrict__ scale,
+                                  double *__restrict__ c, double *__restrict__ s, size_t stride)
+{
+    const int m = blockIdx.y;
+    const size_t P = tri_len_dev(nmax);
+    const size_t out0 = dcol_off(nmax_out, m) - m;
+    for (int n = m + blockIdx.x * blockDim.x + threadIdx.x; n <= nmax_out;
+         n += gridDim.x * blockDim.x) {
+        double vc = 0.0, vs = 0.0;
+        if (m <= nmax && n <= nmax) {
+            const size_t q = dcol_off(nmax, m) + (n - m);
+            for (int tl = 0; tl < ntile; tl++) {
+                const double2 p = part[(size_t)tl * P + q];
+                vc += p.x;
+                vs += p.y;
+            }
+            if (scale) {
+                vc *= scale[n];
+                vs *= scale[n];
+            }
+            if (m == 0) vs = 0.0;
+        }
+        c[(out0 + n) * stride] = vc;
+        s[(out0 + n) * stride] = vs;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SHS at scattered points (src/shs/shs_point_sctr.c:294-687,
+// shs_sctr_mulc.c:36-57).  One lane per point, sequential over the orders of
+// one chunk; the sectorial seed and (R/r)^(m+1) advance with m in the lane.
+// ---------------------------------------------------------------------------
+template <bool UNIT>
+__global__ __launch_bounds__(256) void sctr_kernel(SctrArgs A)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int mb = A.mchunk[chunk], me = A.mchunk[chunk + 1];
+    const bool valid = i < A.npoint;
+    const double t = valid ? A.lat_t[i] : 0.0;
+    const double u = valid ? A.lat_u[i] : 0.0;
+    const double lon = valid ? A.lon[i] : 0.0;
+    const double q = (!UNIT && valid) ? A.R / A.r[i] : 1.0;
+    const int nmax = A.nmax;
+
+    // P_{mb,mb} (for mb >= 1) and (R/r)^(mb+1)
+    double xs = ROOT3 * u;
+    int ixs = 0;
+    for (int mm = 1; mm < mb; mm++) {
+        xs = (A.dm[mm] * u) * xs;
+        xnorm(xs, ixs);
+    }
+    double rpc = 1.0;
+    if (!UNIT)
+        for (int k = 1; k <= mb + 1; k++) rpc = rpc * q;
+
+    double f = 0.0;
+    for (int m = mb; m < me; m++) {
+        const double2 *__restrict__ abm = (const double2 *)A.ab + dcol_off(A.ab_nt, m) - m;
+        const double2 *__restrict__ csm = (const double2 *)A.cs + dcol_off(nmax, m) - m;
+        double a = 0.0, b = 0.0, rp = rpc;
+        double x = 0.0, y = 0.0;
+        int ix = 0, iy = 0;
+        bool ds = true;
+#define SC_ACC(p, c)                       \
+    do {                                   \
+        const double pq_ = UNIT ? (p) : rp * (p); \
+        a = fma(pq_, (c).x, a);            \
+        b = fma(pq_, (c).y, b);            \
+        if (!UNIT) rp *= q;                \
+    } while (0)
+        int n;
+        if (m == 0) {
+            SC_ACC(1.0, csm[0]);
+            if (nmax >= 1) {
+                x = ROOT3 * t;
+                y = 1.0;
+                SC_ACC(x, csm[1]);
+            }
+            n = 2;
+        } else {
+            SC_ACC(x2f(xs, ixs), csm[m]);
+            if (m < nmax) {
+                y = xs;
+                iy = ixs;
+                x = (abm[m + 1].x * t) * xs;
+                ix = ixs;
+                xnorm(x, ix);
+                ds = !valid;
+                SC_ACC(x2f(x, ix), csm[m + 1]);
+            }
+            n = m + 2;
+        }
+        while (n + 7 <= nmax && wave_any(!ds)) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const double2 ab = abm[n + j];
+                const double p = leg_step_x(x, y, ix, iy, ds, ab.x * t, ab.y);
+                SC_ACC(p, csm[n + j]);
+            }
+            n += 8;
+        }
+        for (; n + 7 <= nmax; n += 8) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const double2 ab = abm[n + j];
+                const double z = fma(ab.x * t, x, -(ab.y * y));
+                SC_ACC(z, csm[n + j]);
+                y = x;
+                x = z;
+            }
+        }
+        for (; n <= nmax; n++) {
+            const double2 ab = abm[n];
+            const double p = leg_step_x(x, y, ix, iy, ds, ab.x * t, ab.y);
+            SC_ACC(p, csm[n]);
+        }
+#undef SC_ACC
+        if (m == 0) b = 0.0;
+        double sn, cn;
+        sincos((double)m * lon, &sn, &cn);
+        f = f + (a * cn + b * sn);
+        // advance the seed to P_{m+1,m+1} and the radius power
+        if (m >= 1) {
+            xs = (A.dm[m] * u) * xs;
+            xnorm(xs, ixs);
+        }
+        if (!UNIT) rpc = rpc * q;
+    }
+    if (valid) A.part[(size_t)chunk * A.npoint + i] = f;
+}
+
+__global__ void sctr_reduce_kernel(size_t npoint, int nchunk, const double *__restrict__ part,
+                                   double mur, double *__restrict__ f)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoint) return;
+    double s = part[i];
+    for (int c = 1; c < nchunk; c++) s += part[(size_t)c * npoint + i];
+    f[i] = mur * s;
+}
+
+__global__ void sincos_kernel(size_t n, const double *__restrict__ lat, double *__restrict__ t,
+                              double *__restrict__ u)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) sincos(lat[i], &t[i], &u[i]);
+}
+
+// misc_is_nearly_equal (src/misc/misc_is_nearly_equal.c:16-32) for every
+// x[i] against ref; clears *flag if any value differs
+__global__ void all_near_kernel(size_t n, const double *__restrict__ x, double ref, double eps,
+                                int *flag)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = x[i], aa = fabs(a), ba = fabs(ref), d = fabs(a - ref), s = aa + ba;
+    bool ok;
+    if (a == ref)
+        ok = true;
+    else if (a == 0.0 || ref == 0.0 || s == aa || s == ba)
+        ok = d <= eps;
+    else
+        ok = d / fmax(aa, ba) <= eps;
+    if (!ok) *flag = 0;
+}
+
+__global__ void pack_cs_kernel(const double *__restrict__ c, const double *__restrict__ s,
+                               size_t n, double2 *__restrict__ cs)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cs[i] = make_double2(c[i], s[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline hipStream_t S(void *s)
+{
+    return (hipStream_t)s;
+}
+
+int launch_ab_table(int nt, const double *r, const double *ri, void *ab, void *stream)
+{
+    hipLaunchKernelGGL(ab_table_kernel, dim3(nt + 1), dim3(256), 0, S(stream), nt, r, ri,
+                       (double2 *)ab);
+    return (int)hipGetLastError();
+}
+
+int launch_seeds(int nring, const double *u, const double *dm, int nmax, size_t sld, double *sx,
+                 int *six, void *stream)
+{
+    if (nring <= 0 || nmax <= 0) return 0;
+    hipLaunchKernelGGL(seed_kernel, dim3((nring + 63) / 64), dim3(64), 0, S(stream), nring, u, dm,
+                       nmax, sld, sx, six);
+    return (int)hipGetLastError();
+}
+
+int launch_rpows(int nrow, const double *ratio, int kmax, size_t rld, double *rp, void *stream)
+{
+    if (nrow <= 0) return 0;
+    hipLaunchKernelGGL(rpows_kernel, dim3((nrow + 63) / 64), dim3(64), 0, S(stream), nrow, ratio,
+                       kmax, rld, rp);
+    return (int)hipGetLastError();
+}
+
+int launch_shs_grid(const ShsGridArgs &a, void *stream)
+{
+    const int nquad = (a.mcount + 3) / 4;
+    const int nqg = (nquad + 7) / 8;
+    const unsigned nblk = 8u * (unsigned)nqg * (unsigned)a.ntile;
+    if (a.unit)
+        hipLaunchKernelGGL((shs_grid_kernel<SHS_K, true>), dim3(nblk), dim3(256), 0, S(stream), a);
+    else
+        hipLaunchKernelGGL((shs_grid_kernel<SHS_K, false>), dim3(nblk), dim3(256), 0, S(stream), a);
+    return (int)hipGetLastError();
+}
+
+int launch_lon_direct(int nlat, int nmax, size_t nlon, const double *lon, const void *lc,
+                      size_t ldl, double mur, double *f, void *stream)
+{
+    dim3 grid((unsigned)((nlon + 127) / 128), (unsigned)nlat);
+    hipLaunchKernelGGL(lon_direct_kernel, grid, dim3(128), 0, S(stream), nlat, nmax, nlon, lon,
+                       (const double2 *)lc, ldl, mur, f);
+    return (int)hipGetLastError();
+}
+
+int launch_sha(const ShaArgs &a, void *stream)
+{
+    const int nquad = (a.nmax + 1 + 3) / 4;
+    const int nqg = (nquad + 7) / 8;
+    const unsigned nblk = 8u * (unsigned)nqg * (unsigned)a.ntile;
+    hipLaunchKernelGGL((sha_kernel<SHA_K>), dim3(nblk), dim3(256), 0, S(stream), a);
+    return (int)hipGetLastError();
+}
+
+int launch_sha_reduce(int nmax, int ntile, const void *part, int nmax_out, const double *scale,
+                      double *c, double *s, size_t stride, void *stream)
+{
+    const unsigned gx = (unsigned)((nmax_out + 1 + 255) / 256);
+    hipLaunchKernelGGL(sha_reduce_kernel, dim3(gx, nmax_out + 1), dim3(256), 0, S(stream), nmax,
+                       ntile, (const double2 *)part, nmax_out, scale, c, s, stride);
+    return (int)hipGetLastError();
+}
+
+int launch_sctr(const SctrArgs &a, void *stream)
+{
+    dim3 grid((unsigned)((a.npoint + 255) / 256), (unsigned)a.nchunk);
+    if (a.unit)
+        hipLaunchKernelGGL((sctr_kernel<true>), grid, dim3(256), 0, S(stream), a);
+    else
+        hipLaunchKernelGGL((sctr_kernel<false>), grid, dim3(256), 0, S(stream), a);
+    return (int)hipGetLastError();
+}
+
+int launch_sctr_reduce(size_t npoint, int nchunk, const double *part, double mur, double *f,
+                       void *stream)
+{
+    hipLaunchKernelGGL(sctr_reduce_kernel, dim3((unsigned)((npoint + 255) / 256)), dim3(256), 0,
+                       S(stream), npoint, nchunk, part, mur, f);
+    return (int)hipGetLastError();
+}
+
+int launch_sincos(size_t n, const double *lat, double *t, double *u, void *stream)
+{
+    hipLaunchKernelGGL(sincos_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), n,
+                       lat, t, u);
+    return (int)hipGetLastError();
+}
+
+int launch_all_near(size_t n, const double *x, double ref, double eps, int *result, void *stream)
+{
+    int *flag = nullptr;
+    hipError_t e = hipMallocAsync((void **)&flag, sizeof(int), S(stream));
+    if (e != hipSuccess) return (int)e;
+    const int one = 1;
+    e = hipMemcpyAsync(flag, &one, sizeof(int), hipMemcpyHostToDevice, S(stream));
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(all_near_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           S(stream), n, x, ref, eps, flag);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(result, flag, sizeof(int), hipMemcpyDeviceToHost, S(stream));
+    if (e == hipSuccess) e = hipStreamSynchronize(S(stream));
+    hipFreeAsync(flag, S(stream));
+    return (int)e;
+}
+
+int launch_pack_cs(const double *c, const double *s, size_t n, void *cs, void *stream)
+{
+    hipLaunchKernelGGL(pack_cs_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream),
+                       c, s, n, (double2 *)cs);
+    return (int)hipGetLastError();
+}
+
+}  // namespace charm_amd

@@ -1,0 +1,990 @@
+// Host dispatch of point synthesis and analysis onto the GPU.
+//
+// charm_shs_point / charm_sha_point keep CHarm's host-pointer contract: the
+// inputs are copied to HBM, the transform runs there, and the results are
+// copied back.  The charm_amd_*_dev entry points take device pointers and
+// leave everything resident.  Each host thread owns one non-blocking HIP
+// stream and its own FFT plans, so concurrent calls from several threads do
+// not share mutable state; the recurrence-coefficient table is shared and
+// only ever grows.
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "internal.h"
+
+namespace charm_amd {
+
+namespace {
+
+const double PI = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+struct HipFail {
+    hipError_t code;
+    const char *what;
+    const char *file;
+    int line;
+};
+
+#define HIPCHK(expr)                                                  \
+    do {                                                              \
+        hipError_t e_ = (expr);                                       \
+        if (e_ != hipSuccess) throw HipFail{e_, #expr, __FILE__, __LINE__}; \
+    } while (0)
+
+#define FFTCHK(expr)                                                        \
+    do {                                                                    \
+        hipfftResult r_ = (expr);                                           \
+        if (r_ != HIPFFT_SUCCESS)                                           \
+            throw HipFail{(hipError_t)(1000 + (int)r_), #expr, __FILE__, __LINE__}; \
+    } while (0)
+
+void report(charm_err *err, const HipFail &f, const char *func)
+{
+    char msg[CHARM_ERR_MAX_MSG];
+    int code = CHARM_EFUNCARG;
+    if ((int)f.code >= 1000) {
+        std::snprintf(msg, sizeof msg, "hipFFT failure %d in \"%s\".", (int)f.code - 1000, f.what);
+        code = CHARM_EFFTWINIT;
+    } else {
+        std::snprintf(msg, sizeof msg, "HIP error \"%s\" in \"%s\".", hipGetErrorString(f.code),
+                      f.what);
+        if (f.code == hipErrorOutOfMemory || f.code == hipErrorMemoryAllocation) code = CHARM_EMEM;
+    }
+    err_set(err, f.file, (size_t)f.line, func, code, msg);
+}
+
+// ---------------------------------------------------------------------------
+// Per-thread stream, device memory and FFT plans
+// ---------------------------------------------------------------------------
+std::once_flag g_pool_once;
+
+void init_pool()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t thr = UINT64_MAX;
+        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+}
+
+struct ThreadState {
+    hipStream_t stream = nullptr;
+    std::map<std::tuple<int, size_t, size_t, size_t>, hipfftHandle> plans;
+    void *pinned = nullptr;
+    size_t pinned_size = 0;
+    ~ThreadState()
+    {
+        for (auto &kv : plans) hipfftDestroy(kv.second);
+        if (pinned) hipHostFree(pinned);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+ThreadState &tls()
+{
+    thread_local ThreadState st;
+    return st;
+}
+
+hipStream_t thread_stream()
+{
+    ThreadState &st = tls();
+    if (!st.stream) {
+        std::call_once(g_pool_once, init_pool);
+        HIPCHK(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking));
+    }
+    return st.stream;
+}
+
+void *pinned_buffer(size_t bytes)
+{
+    ThreadState &st = tls();
+    if (st.pinned_size < bytes) {
+        if (st.pinned) HIPCHK(hipHostFree(st.pinned));
+        st.pinned = nullptr;
+        st.pinned_size = 0;
+        HIPCHK(hipHostMalloc(&st.pinned, bytes, hipHostMallocDefault));
+        st.pinned_size = bytes;
+    }
+    return st.pinned;
+}
+
+// stream-ordered device buffer
+struct DevBuf {
+    void *p = nullptr;
+    hipStream_t s = nullptr;
+    DevBuf() = default;
+    DevBuf(size_t bytes, hipStream_t st) { alloc(bytes, st); }
+    void alloc(size_t bytes, hipStream_t st)
+    {
+        s = st;
+        if (bytes) HIPCHK(hipMallocAsync(&p, bytes, st));
+    }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf()
+    {
+        if (p) hipFreeAsync(p, s);
+    }
+    template <class T>
+    T *as() const
+    {
+        return (T *)p;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Optional per-kernel timing: HIP events recorded on the launching stream
+// around each launch; resolved when read.
+// ---------------------------------------------------------------------------
+struct ProfRec {
+    const char *name;
+    hipEvent_t a, b;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfRec> g_prof_pending;
+std::map<std::string, std::pair<double, long>> g_prof_acc;
+
+struct ProfScope {
+    ProfRec rec{nullptr, nullptr, nullptr};
+    hipStream_t s;
+    ProfScope(const char *name, hipStream_t st) : s(st)
+    {
+        bool on;
+        {
+            std::lock_guard<std::mutex> lk(g_prof_mu);
+            on = g_prof_on;
+        }
+        if (!on) return;
+        rec.name = name;
+        HIPCHK(hipEventCreate(&rec.a));
+        HIPCHK(hipEventCreate(&rec.b));
+        HIPCHK(hipEventRecord(rec.a, s));
+    }
+    ~ProfScope()
+    {
+        if (!rec.name) return;
+        if (hipEventRecord(rec.b, s) != hipSuccess) return;
+        std::lock_guard<std::mutex> lk(g_prof_mu);
+        g_prof_pending.push_back(rec);
+    }
+};
+
+// kind 0: c2r (Z2D), 1: r2c (D2Z); n = transform length; batch rows
+hipfftHandle fft_plan(int kind, size_t n, size_t batch, size_t ldc, hipStream_t stream)
+{
+    ThreadState &st = tls();
+    auto key = std::make_tuple(kind, n, batch, ldc);
+    auto it = st.plans.find(key);
+    hipfftHandle h;
+    if (it == st.plans.end()) {
+        FFTCHK(hipfftCreate(&h));
+        int nn = (int)n;
+        int cemb = (int)ldc, remb = (int)n;
+        if (kind == 0)
+            FFTCHK(hipfftPlanMany(&h, 1, &nn, &cemb, 1, (int)ldc, &remb, 1, (int)n, HIPFFT_Z2D,
+                                  (int)batch));
+        else
+            FFTCHK(hipfftPlanMany(&h, 1, &nn, &remb, 1, (int)n, &cemb, 1, (int)ldc, HIPFFT_D2Z,
+                                  (int)batch));
+        st.plans[key] = h;
+    } else {
+        h = it->second;
+    }
+    FFTCHK(hipfftSetStream(h, stream));
+    return h;
+}
+
+// ---------------------------------------------------------------------------
+// Shared tables: (a_nm, b_nm) for all m <= n <= nt, and dm[n], n <= nt
+// (src/leg/leg_func_r_ri.c, leg_func_dm.c, leg_func_anm_bnm.c)
+// ---------------------------------------------------------------------------
+struct Tables {
+    void *ab = nullptr;
+    double *dm = nullptr;
+    int nt = -1;
+};
+std::mutex g_tab_mu;
+Tables g_tab;
+std::vector<void *> g_retired;  // earlier (smaller) tables stay valid for in-flight work
+
+Tables tables(int nmax, hipStream_t stream)
+{
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    if (g_tab.nt >= nmax) return g_tab;
+    int nt = std::max(nmax, std::max(64, g_tab.nt + g_tab.nt / 4));
+    std::vector<double> r(2 * (size_t)nt + 4), ri(2 * (size_t)nt + 4), dm((size_t)nt + 1);
+    r[0] = ri[0] = 0.0;
+    for (size_t k = 1; k <= 2 * (size_t)nt + 3; k++) {
+        double w = std::sqrt((double)k);
+        r[k] = w;
+        ri[k] = 1.0 / w;
+    }
+    dm[0] = 0.0;
+    for (size_t n = 1; n <= (size_t)nt; n++) dm[n] = r[2 * n + 3] * ri[2 * n + 2];
+    void *ab = nullptr;
+    double *ddm = nullptr, *dr = nullptr, *dri = nullptr;
+    HIPCHK(hipMalloc(&ab, tri_len(nt) * 2 * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&ddm, dm.size() * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&dr, r.size() * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&dri, ri.size() * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(ddm, dm.data(), dm.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(dr, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(dri, ri.data(), ri.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHK((hipError_t)launch_ab_table(nt, dr, dri, ab, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipFree(dr));
+    HIPCHK(hipFree(dri));
+    if (g_tab.ab) {
+        g_retired.push_back(g_tab.ab);
+        g_retired.push_back(g_tab.dm);
+    }
+    g_tab.ab = ab;
+    g_tab.dm = ddm;
+    g_tab.nt = nt;
+    return g_tab;
+}
+
+template <class T>
+void h2d(T *dst, const T *src, size_t n, hipStream_t s)
+{
+    if (n) HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+// Grid geometry shared by the synthesis and analysis paths
+struct GridGeom {
+    size_t nlat = 0, nlon = 0, nring = 0;
+    bool symm = false;
+    std::vector<double> t, u;             // [nring]
+    std::vector<unsigned char> mirror;    // [nring]
+};
+
+// Symmetry of a grid's latitudes (src/shs/shs_point_grd.c:133-197,
+// src/crd/crd_grd_check_symm.c:21-72, crd_point_quad_equator.c:22-53,
+// src/misc/misc_arr_chck_symm.c).
+GridGeom grid_geometry(const charm_point *pnt)
+{
+    GridGeom g;
+    g.nlat = pnt->nlat;
+    g.nlon = pnt->nlon;
+    const size_t nlat = g.nlat;
+    const bool even = !(nlat % 2);
+    const bool gl = pnt->type == CHARM_CRD_POINT_GRID_GL;
+    if (nlat == 1) {
+        g.symm = false;
+    } else if (gl) {
+        g.symm = true;
+    } else {
+        bool ok = true;
+        for (size_t i = 0; i < nlat / 2 && ok; i++)
+            ok = nearly_equal(pnt->lat[i] - 0.0, 0.0 - pnt->lat[nlat - 1 - i], charm_glob_threshold2);
+        if (ok && (nlat % 2)) ok = nearly_equal(0.0, pnt->lat[nlat / 2], charm_glob_threshold2);
+        g.symm = ok;
+    }
+    g.nring = g.symm ? (nlat + 1 - (even ? 1 : 0)) / 2 : nlat;
+    const size_t nmax_grd = nlat - 1;
+    const size_t equator = (gl && (nmax_grd % 2)) ? SIZE_MAX : nlat / 2;
+    g.t.resize(g.nring);
+    g.u.resize(g.nring);
+    g.mirror.resize(g.nring);
+    for (size_t i = 0; i < g.nring; i++) {
+        g.t[i] = std::sin(pnt->lat[i]);
+        g.u[i] = std::cos(pnt->lat[i]);
+        unsigned char mr = 0;
+        if (g.symm) {
+            if (gl)
+                mr = (i < equator) ? 1 : 0;
+            else
+                mr = (!even && i == g.nring - 1) ? 0 : 1;
+        }
+        g.mirror[i] = mr;
+    }
+    return g;
+}
+
+// FFT along parallels possible?  (src/shs/shs_grd_point_fft_check.c:14-40)
+bool fft_applicable(const charm_point *pnt, unsigned long nmax)
+{
+    const size_t nlon = pnt->nlon;
+    if (nlon == 0) return false;
+    const double dl = (nlon > 1) ? pnt->lon[1] - pnt->lon[0] : 0.0;
+    return ((nlon - 1) / 2 >= nmax) && nearly_equal(pnt->lon[0], 0.0, charm_glob_threshold) &&
+           nearly_equal(pnt->lon[nlon - 1] + dl, 2.0 * PI, charm_glob_threshold);
+}
+
+// Pack C, S of degree shcs->nmax into (C, S) pairs of degree nmax; S_n0 = 0.
+void pack_coeffs(const charm_shc *shcs, unsigned long nmax, double *cs)
+{
+    size_t q = 0;
+    for (unsigned long m = 0; m <= nmax; m++) {
+        const double *c = shcs->c[m], *s = shcs->s[m];
+        for (unsigned long n = m; n <= nmax; n++, q++) {
+            cs[2 * q] = c[n - m];
+            cs[2 * q + 1] = (m == 0) ? 0.0 : s[n - m];
+        }
+    }
+}
+
+int ring_tiles(size_t nring, int K)
+{
+    return (int)((nring + 64 * (size_t)K - 1) / (64 * (size_t)K));
+}
+
+// Lumped sums + longitude transform for the rings of a grid whose geometry
+// is already on the device.
+struct DevGrid {
+    const double *t, *u;
+    const unsigned char *mirror;
+    const double *sx;
+    const int *six;
+    size_t sld;
+    const double *ratio;   // nullptr when all radii equal R
+    const double *rp;
+    size_t rld;
+};
+
+void run_shs_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev, bool use_fft,
+                  unsigned long nmax, double mur, const void *cs_dev, const Tables &tb,
+                  double *f_dev, hipStream_t stream)
+{
+    const size_t nlat = g.nlat, nlon = g.nlon;
+    ShsGridArgs a{};
+    a.nmax = (int)nmax;
+    a.nring = (int)g.nring;
+    a.nlat = (int)nlat;
+    a.ntile = ring_tiles(g.nring, SHS_K);
+    a.unit = d.ratio == nullptr;
+    a.t = d.t;
+    a.sx = d.sx;
+    a.six = d.six;
+    a.sld = d.sld;
+    a.rp = d.rp;
+    a.ratio = d.ratio;
+    a.rld = d.rld;
+    a.mirror = d.mirror;
+    a.ab = tb.ab;
+    a.ab_nt = tb.nt;
+    a.cs = cs_dev;
+    a.mur = mur;
+    if (use_fft) {
+        const size_t nfc = nlon / 2 + 1;
+        DevBuf spec(nlat * nfc * 2 * sizeof(double), stream);
+        a.out = spec.p;
+        a.ldo = nfc;
+        a.mcount = (int)nfc;
+        a.fft_layout = 1;
+        {
+            ProfScope ps("shs_grid_kernel", stream);
+            HIPCHK((hipError_t)launch_shs_grid(a, stream));
+        }
+        hipfftHandle h = fft_plan(0, nlon, nlat, nfc, stream);
+        {
+            ProfScope ps("fft_c2r", stream);
+            FFTCHK(hipfftExecZ2D(h, (hipfftDoubleComplex *)spec.p, f_dev));
+        }
+    } else {
+        const size_t ldl = nmax + 1;
+        DevBuf lc(nlat * ldl * 2 * sizeof(double), stream);
+        a.out = lc.p;
+        a.ldo = ldl;
+        a.mcount = (int)ldl;
+        a.fft_layout = 0;
+        HIPCHK((hipError_t)launch_shs_grid(a, stream));
+        HIPCHK((hipError_t)launch_lon_direct((int)nlat, (int)nmax, nlon, lon_dev, lc.p, ldl, mur,
+                                             f_dev, stream));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain per-call synthesis on a grid (host arrays)
+// ---------------------------------------------------------------------------
+void shs_grid_host(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax, double *f,
+                   charm_err *err)
+{
+    const size_t nlat = pnt->nlat, nlon = pnt->nlon;
+    if (pnt->type != CHARM_CRD_POINT_GRID && pnt->type != CHARM_CRD_POINT_GRID_GL) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG,
+                      "Driscoll--Healy grids are not supported by this implementation.");
+        return;
+    }
+    // custom grids: longitudes must be linearly increasing
+    // (src/shs/shs_point_grd.c:205-236)
+    if (pnt->type == CHARM_CRD_POINT_GRID && nlon > 1) {
+        const double d0 = pnt->lon[1] - pnt->lon[0];
+        for (size_t j = 1; j < nlon; j++)
+            if (!nearly_equal(pnt->lon[j] - pnt->lon[j - 1], d0, charm_glob_threshold2)) {
+                CHARM_AMD_ERR(err, CHARM_EFUNCARG,
+                              "\"pnt->lon\" is not a linearly increasing array within the "
+                              "\"threshold2\".");
+                return;
+            }
+    }
+    hipStream_t s = thread_stream();
+    GridGeom g = grid_geometry(pnt);
+    const bool use_fft = fft_applicable(pnt, nmax);
+    bool unit = true;
+    for (size_t i = 0; i < nlat && unit; i++)
+        unit = nearly_equal(pnt->r[i], shcs->r, charm_glob_threshold);
+    Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
+
+    const size_t P = tri_len(nmax);
+    double *hcs = (double *)pinned_buffer(P * 2 * sizeof(double));
+    pack_coeffs(shcs, nmax, hcs);
+    DevBuf cs(P * 2 * sizeof(double), s);
+    h2d(cs.as<double>(), hcs, 2 * P, s);
+
+    const size_t nr = g.nring;
+    DevBuf dt(nr * sizeof(double), s), du(nr * sizeof(double), s), dmir(nr, s);
+    h2d(dt.as<double>(), g.t.data(), nr, s);
+    h2d(du.as<double>(), g.u.data(), nr, s);
+    h2d(dmir.as<unsigned char>(), g.mirror.data(), nr, s);
+    const size_t sld = nr;
+    DevBuf sx(std::max<size_t>(nmax, 1) * sld * sizeof(double), s);
+    DevBuf six(std::max<size_t>(nmax, 1) * sld * sizeof(int), s);
+    HIPCHK((hipError_t)launch_seeds((int)nr, du.as<double>(), tb.dm, (int)nmax, sld,
+                                    sx.as<double>(), six.as<int>(), s));
+    DevBuf ratio, rp;
+    DevGrid d{dt.as<double>(), du.as<double>(), dmir.as<unsigned char>(), sx.as<double>(),
+              six.as<int>(), sld, nullptr, nullptr, 0};
+    if (!unit) {
+        std::vector<double> q(nlat);
+        for (size_t i = 0; i < nlat; i++) q[i] = shcs->r / pnt->r[i];
+        ratio.alloc(nlat * sizeof(double), s);
+        rp.alloc((nmax + 2) * nlat * sizeof(double), s);
+        h2d(ratio.as<double>(), q.data(), nlat, s);
+        HIPCHK((hipError_t)launch_rpows((int)nlat, ratio.as<double>(), (int)nmax + 1, nlat,
+                                        rp.as<double>(), s));
+        d.ratio = ratio.as<double>();
+        d.rp = rp.as<double>();
+        d.rld = nlat;
+    }
+    DevBuf lon;
+    if (!use_fft) {
+        lon.alloc(nlon * sizeof(double), s);
+        h2d(lon.as<double>(), pnt->lon, nlon, s);
+    }
+    DevBuf df(nlat * nlon * sizeof(double), s);
+    run_shs_grid(g, d, lon.as<double>(), use_fft, nmax, shcs->mu / shcs->r, cs.p, tb,
+                 df.as<double>(), s);
+    HIPCHK(hipMemcpyAsync(f, df.p, nlat * nlon * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// Scattered points
+// ---------------------------------------------------------------------------
+void run_sctr(size_t npoint, const double *t_dev, const double *u_dev, const double *lon_dev,
+              const double *r_dev, bool unit, unsigned long nmax, double mu, double R,
+              const void *cs_dev, const Tables &tb, double *f_dev, hipStream_t s)
+{
+    // split orders into chunks of equal work when there are few points
+    const size_t nwave = (npoint + 63) / 64;
+    int nchunk = (int)std::min<size_t>(64, std::max<size_t>(1, 4096 / std::max<size_t>(nwave, 1)));
+    nchunk = (int)std::min<unsigned long>((unsigned long)nchunk, nmax + 1);
+    std::vector<int> mc(nchunk + 1);
+    {
+        const double total = 0.5 * (double)(nmax + 1) * (double)(nmax + 2);
+        double acc = 0.0;
+        int c = 1;
+        mc[0] = 0;
+        for (unsigned long m = 0; m <= nmax && c < nchunk; m++) {
+            acc += (double)(nmax + 1 - m);
+            if (acc >= total * c / nchunk) mc[c++] = (int)m + 1;
+        }
+        while (c < nchunk) mc[c++] = (int)nmax + 1;
+        mc[nchunk] = (int)nmax + 1;
+    }
+    DevBuf dmc((nchunk + 1) * sizeof(int), s);
+    h2d(dmc.as<int>(), mc.data(), nchunk + 1, s);
+    DevBuf part((size_t)nchunk * npoint * sizeof(double), s);
+    SctrArgs a{};
+    a.nmax = (int)nmax;
+    a.npoint = npoint;
+    a.nchunk = nchunk;
+    a.mchunk = dmc.as<int>();
+    a.unit = unit;
+    a.R = R;
+    a.lat_t = t_dev;
+    a.lat_u = u_dev;
+    a.lon = lon_dev;
+    a.r = r_dev;
+    a.dm = tb.dm;
+    a.ab = tb.ab;
+    a.ab_nt = tb.nt;
+    a.cs = cs_dev;
+    a.part = part.as<double>();
+    {
+        ProfScope ps("sctr_kernel", s);
+        HIPCHK((hipError_t)launch_sctr(a, s));
+    }
+    HIPCHK((hipError_t)launch_sctr_reduce(npoint, nchunk, part.as<double>(), mu / R, f_dev, s));
+}
+
+void shs_sctr_host(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax, double *f,
+                   charm_err *err)
+{
+    if (pnt->nlat != pnt->nlon) {
+        char msg[CHARM_ERR_MAX_MSG];
+        std::snprintf(msg, sizeof msg,
+                      "The number of latitudes \"%zu\" do not match the number of longitudes "
+                      "\"%zu\".",
+                      pnt->nlat, pnt->nlon);
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG, msg);
+        return;
+    }
+    const size_t npoint = pnt->nlat;
+    hipStream_t s = thread_stream();
+    bool unit = true;
+    for (size_t i = 0; i < npoint && unit; i++)
+        unit = nearly_equal(pnt->r[i], shcs->r, charm_glob_threshold);
+    Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
+    const size_t P = tri_len(nmax);
+    double *hcs = (double *)pinned_buffer(P * 2 * sizeof(double));
+    pack_coeffs(shcs, nmax, hcs);
+    DevBuf cs(P * 2 * sizeof(double), s);
+    h2d(cs.as<double>(), hcs, 2 * P, s);
+    std::vector<double> t(npoint), u(npoint);
+    for (size_t i = 0; i < npoint; i++) {
+        t[i] = std::sin(pnt->lat[i]);
+        u[i] = std::cos(pnt->lat[i]);
+    }
+    DevBuf dt(npoint * sizeof(double), s), du(npoint * sizeof(double), s);
+    DevBuf dl(npoint * sizeof(double), s), dr(npoint * sizeof(double), s);
+    h2d(dt.as<double>(), t.data(), npoint, s);
+    h2d(du.as<double>(), u.data(), npoint, s);
+    h2d(dl.as<double>(), pnt->lon, npoint, s);
+    h2d(dr.as<double>(), pnt->r, npoint, s);
+    DevBuf df(npoint * sizeof(double), s);
+    run_sctr(npoint, dt.as<double>(), du.as<double>(), dl.as<double>(), dr.as<double>(), unit,
+             nmax, shcs->mu, shcs->r, cs.p, tb, df.as<double>(), s);
+    HIPCHK(hipMemcpyAsync(f, df.p, npoint * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// Analysis
+// ---------------------------------------------------------------------------
+// c = pi/(nmax_grd+1) * 1/(4 pi) * r0/mu (src/sha/sha_point.c:313-322)
+double sha_const(size_t nlat, double r0, double mu)
+{
+    double c = PI / (double)((nlat - 1) + 1);
+    c *= 1.0 / (4.0 * PI) * (r0 / mu);
+    return c;
+}
+
+// (r0/R)^n, n = 0..nmax_out, when r0 differs from R (src/shc/shc_rescale.c:30-46)
+bool rescale_table(double r0, double R, unsigned long nmax_out, std::vector<double> &sc)
+{
+    if (nearly_equal(R, r0, charm_glob_threshold)) return false;
+    const double mu_ratio = 1.0, r_ratio = r0 / R;
+    sc.resize(nmax_out + 1);
+    double tmp2 = mu_ratio * r_ratio;
+    sc[0] = mu_ratio;
+    for (unsigned long n = 1; n <= nmax_out; n++) {
+        sc[n] = tmp2;
+        tmp2 *= r_ratio;
+    }
+    return true;
+}
+
+void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const double *f_dev,
+             unsigned long nmax, const Tables &tb, const double *scale_dev, unsigned long nmax_out,
+             double *c_dev, double *s_dev, size_t stride, hipStream_t s)
+{
+    const size_t nlat = g.nlat, nlon = g.nlon, nfc = nlon / 2 + 1;
+    DevBuf spec(nlat * nfc * 2 * sizeof(double), s);
+    hipfftHandle h = fft_plan(1, nlon, nlat, nfc, s);
+    {
+        ProfScope ps("fft_r2c", s);
+        FFTCHK(hipfftExecD2Z(h, (hipfftDoubleReal *)f_dev, (hipfftDoubleComplex *)spec.p));
+    }
+    ShaArgs a{};
+    a.nmax = (int)nmax;
+    a.nring = (int)g.nring;
+    a.nlat = (int)nlat;
+    a.ntile = ring_tiles(g.nring, SHA_K);
+    a.t = d.t;
+    a.sx = d.sx;
+    a.six = d.six;
+    a.sld = d.sld;
+    a.mirror = d.mirror;
+    a.cw = cw_dev;
+    a.spec = spec.p;
+    a.lds = nfc;
+    a.ab = tb.ab;
+    a.ab_nt = tb.nt;
+    DevBuf part((size_t)a.ntile * tri_len(nmax) * 2 * sizeof(double), s);
+    a.part = part.p;
+    {
+        ProfScope ps("sha_kernel", s);
+        HIPCHK((hipError_t)launch_sha(a, s));
+    }
+    {
+        ProfScope ps("sha_reduce_kernel", s);
+        HIPCHK((hipError_t)launch_sha_reduce((int)nmax, a.ntile, part.p, (int)nmax_out, scale_dev,
+                                             c_dev, s_dev, stride, s));
+    }
+}
+
+void sha_host(const charm_point *pnt, const double *f, unsigned long nmax, charm_shc *shcs,
+              charm_err *err)
+{
+    const int type = pnt->type;
+    if (type != CHARM_CRD_POINT_GRID_GL && type != CHARM_CRD_POINT_GRID_DH1 &&
+        type != CHARM_CRD_POINT_GRID_DH2) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG,
+                      "Unsupported \"pnt->type\" for spherical harmonic analysis of point data "
+                      "values.");
+        return;
+    }
+    const size_t nlat = pnt->nlat, nlon = pnt->nlon;
+    const double r0 = pnt->r[0];
+    for (size_t i = 1; i < nlat; i++)
+        if (!nearly_equal(pnt->r[i], r0, charm_glob_threshold)) {
+            CHARM_AMD_ERR(err, CHARM_EFUNCARG, "All spherical radii in \"pnt->r\" must be equal.");
+            return;
+        }
+    const unsigned long nmax_grd =
+        (type == CHARM_CRD_POINT_GRID_GL) ? (unsigned long)nlat - 1 : ((unsigned long)nlat - 2) / 2;
+    if (nmax > nmax_grd) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG,
+                      "The input data grid \"pnt\" was created for a maximum degree that is not "
+                      "high enough to recover harmonics up to the specified maximum degree of "
+                      "the analysis.");
+        return;
+    }
+    if (type != CHARM_CRD_POINT_GRID_GL) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG,
+                      "Driscoll--Healy grids are not supported by this implementation.");
+        return;
+    }
+    hipStream_t s = thread_stream();
+    GridGeom g = grid_geometry(pnt);
+    Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
+    const size_t nr = g.nring;
+    const double c = sha_const(nlat, r0, shcs->mu);
+    std::vector<double> cw(nlat);
+    for (size_t i = 0; i < nlat; i++) cw[i] = c * pnt->w[i];
+    DevBuf dt(nr * sizeof(double), s), du(nr * sizeof(double), s), dmir(nr, s);
+    DevBuf dcw(nlat * sizeof(double), s);
+    h2d(dt.as<double>(), g.t.data(), nr, s);
+    h2d(du.as<double>(), g.u.data(), nr, s);
+    h2d(dmir.as<unsigned char>(), g.mirror.data(), nr, s);
+    h2d(dcw.as<double>(), cw.data(), nlat, s);
+    const size_t sld = nr;
+    DevBuf sx(std::max<size_t>(nmax, 1) * sld * sizeof(double), s);
+    DevBuf six(std::max<size_t>(nmax, 1) * sld * sizeof(int), s);
+    HIPCHK((hipError_t)launch_seeds((int)nr, du.as<double>(), tb.dm, (int)nmax, sld,
+                                    sx.as<double>(), six.as<int>(), s));
+    DevGrid d{dt.as<double>(), du.as<double>(), dmir.as<unsigned char>(), sx.as<double>(),
+              six.as<int>(), sld, nullptr, nullptr, 0};
+    DevBuf df(nlat * nlon * sizeof(double), s);
+    h2d(df.as<double>(), f, nlat * nlon, s);
+    std::vector<double> sc;
+    DevBuf dsc;
+    if (rescale_table(r0, shcs->r, shcs->nmax, sc)) {
+        dsc.alloc(sc.size() * sizeof(double), s);
+        h2d(dsc.as<double>(), sc.data(), sc.size(), s);
+    }
+    const size_t nc = tri_len(shcs->nmax);
+    DevBuf dc(nc * sizeof(double), s), ds(nc * sizeof(double), s);
+    run_sha(g, d, dcw.as<double>(), df.as<double>(), nmax, tb, dsc.as<double>(), shcs->nmax,
+            dc.as<double>(), ds.as<double>(), 1, s);
+    HIPCHK(hipMemcpyAsync(shcs->c[0], dc.p, nc * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(shcs->s[0], ds.p, nc * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+}  // namespace charm_amd
+
+// ---------------------------------------------------------------------------
+// Device-resident GL grid
+// ---------------------------------------------------------------------------
+struct charm_amd_grid {
+    charm_amd::GridGeom g;
+    int type = 0;
+    double r0 = 0.0;
+    unsigned long nmax_grd = 0;
+    double *t = nullptr, *u = nullptr, *w = nullptr, *lon = nullptr, *sx = nullptr;
+    int *six = nullptr;
+    unsigned char *mirror = nullptr;
+    size_t sld = 0;
+    std::vector<double> wh;
+};
+
+using namespace charm_amd;
+
+extern "C" {
+
+void charm_amd_prof_enable(int on)
+{
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_on = on != 0;
+}
+
+void charm_amd_prof_reset(void)
+{
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (auto &r : g_prof_pending) {
+        (void)hipEventSynchronize(r.b);
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    g_prof_pending.clear();
+    g_prof_acc.clear();
+}
+
+int charm_amd_prof_read(const char *name, double *total_ms, long *count)
+{
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (auto &r : g_prof_pending) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            auto &acc = g_prof_acc[r.name];
+            acc.first += ms;
+            acc.second += 1;
+        }
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    g_prof_pending.clear();
+    auto it = g_prof_acc.find(name);
+    if (it == g_prof_acc.end()) {
+        *total_ms = 0.0;
+        *count = 0;
+        return 0;
+    }
+    *total_ms = it->second.first;
+    *count = it->second.second;
+    return 1;
+}
+
+int charm_amd_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+// src/shs/shs_point.c:26-129
+void charm_shs_point(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax,
+                     double *f, charm_err *err)
+{
+    if (nmax > shcs->nmax) {
+        char msg[CHARM_ERR_MAX_MSG];
+        std::snprintf(msg, sizeof msg,
+                      "Maximum harmonic degree of the synthesis \"nmax = %lu\" cannot be larger "
+                      "than maximum harmonic degree of spherical harmonic coefficients "
+                      "\"shcs->nmax = %lu\".",
+                      nmax, shcs->nmax);
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG, msg);
+        return;
+    }
+    const int type = pnt->type;
+    const bool sctr = type == CHARM_CRD_POINT_SCATTERED;
+    const bool grid = type == CHARM_CRD_POINT_GRID || type == CHARM_CRD_POINT_GRID_GL ||
+                      type == CHARM_CRD_POINT_GRID_DH1 || type == CHARM_CRD_POINT_GRID_DH2;
+    if (!sctr && !grid) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG,
+                      "Unsupported \"pnt->type\" for spherical harmonic synthesis of point "
+                      "values.");
+        return;
+    }
+    if (pnt->npoint == 0) return;
+    try {
+        if (sctr)
+            shs_sctr_host(pnt, shcs, nmax, f, err);
+        else
+            shs_grid_host(pnt, shcs, nmax, f, err);
+    } catch (const HipFail &e) {
+        report(err, e, __func__);
+    }
+    if (!charm_err_isempty(err)) err_propagate(err, __FILE__, __LINE__, __func__);
+}
+
+// src/sha/sha_point.c:124-1125
+void charm_sha_point(const charm_point *pnt, const double *f, unsigned long nmax,
+                     charm_shc *shcs, charm_err *err)
+{
+    if (nmax > shcs->nmax) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG,
+                      "Maximum harmonic degree of the analysis (\"nmax\") cannot be larger than "
+                      "maximum harmonic degree of spherical harmonic coefficients "
+                      "(\"shcs->nmax\").");
+        return;
+    }
+    try {
+        sha_host(pnt, f, nmax, shcs, err);
+    } catch (const HipFail &e) {
+        report(err, e, __func__);
+    }
+    if (!charm_err_isempty(err)) err_propagate(err, __FILE__, __LINE__, __func__);
+}
+
+charm_amd_grid *charm_amd_grid_create(const charm_point *pnt, charm_err *err)
+{
+    if (pnt == nullptr || pnt->type != CHARM_CRD_POINT_GRID_GL || pnt->nlat == 0) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG, "A Gauss--Legendre grid is required.");
+        return nullptr;
+    }
+    const double r0 = pnt->r[0];
+    for (size_t i = 1; i < pnt->nlat; i++)
+        if (!nearly_equal(pnt->r[i], r0, charm_glob_threshold)) {
+            CHARM_AMD_ERR(err, CHARM_EFUNCARG, "All spherical radii in \"pnt->r\" must be equal.");
+            return nullptr;
+        }
+    charm_amd_grid *grd = new charm_amd_grid();
+    try {
+        grd->g = grid_geometry(pnt);
+        grd->type = pnt->type;
+        grd->r0 = r0;
+        grd->nmax_grd = pnt->nlat - 1;
+        grd->wh.assign(pnt->w, pnt->w + pnt->nlat);
+        hipStream_t s = thread_stream();
+        const size_t nr = grd->g.nring, nlat = pnt->nlat;
+        const unsigned long nm = std::max<unsigned long>(grd->nmax_grd, 1);
+        Tables tb = tables((int)nm, s);
+        HIPCHK(hipMalloc((void **)&grd->t, nr * sizeof(double)));
+        HIPCHK(hipMalloc((void **)&grd->u, nr * sizeof(double)));
+        HIPCHK(hipMalloc((void **)&grd->mirror, nr));
+        HIPCHK(hipMalloc((void **)&grd->w, nlat * sizeof(double)));
+        HIPCHK(hipMalloc((void **)&grd->lon, std::max<size_t>(pnt->nlon, 1) * sizeof(double)));
+        grd->sld = nr;
+        HIPCHK(hipMalloc((void **)&grd->sx, nm * nr * sizeof(double)));
+        HIPCHK(hipMalloc((void **)&grd->six, nm * nr * sizeof(int)));
+        h2d(grd->t, grd->g.t.data(), nr, s);
+        h2d(grd->u, grd->g.u.data(), nr, s);
+        h2d(grd->mirror, grd->g.mirror.data(), nr, s);
+        h2d(grd->w, pnt->w, nlat, s);
+        h2d(grd->lon, pnt->lon, pnt->nlon, s);
+        HIPCHK((hipError_t)launch_seeds((int)nr, grd->u, tb.dm, (int)grd->nmax_grd, grd->sld,
+                                        grd->sx, grd->six, s));
+        HIPCHK(hipStreamSynchronize(s));
+    } catch (const HipFail &e) {
+        report(err, e, __func__);
+        charm_amd_grid_free(grd);
+        return nullptr;
+    }
+    return grd;
+}
+
+void charm_amd_grid_free(charm_amd_grid *grd)
+{
+    if (!grd) return;
+    hipDeviceSynchronize();
+    hipFree(grd->t);
+    hipFree(grd->u);
+    hipFree(grd->mirror);
+    hipFree(grd->w);
+    hipFree(grd->lon);
+    hipFree(grd->sx);
+    hipFree(grd->six);
+    delete grd;
+}
+
+void charm_amd_shs_grid_dev(const charm_amd_grid *grd, unsigned long nmax, double mu, double R,
+                            const double *cs_dev, double *f_dev, void *stream, charm_err *err)
+{
+    if (!grd || nmax > grd->nmax_grd) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG, "\"nmax\" exceeds the degree of the grid.");
+        return;
+    }
+    try {
+        hipStream_t s = (hipStream_t)stream;
+        Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
+        DevGrid d{grd->t, grd->u, grd->mirror, grd->sx, grd->six, grd->sld, nullptr, nullptr, 0};
+        DevBuf ratio, rp;
+        if (!nearly_equal(grd->r0, R, charm_glob_threshold)) {
+            // one radius for all rows: a single column of powers (row stride 0)
+            const double q = R / grd->r0;
+            std::vector<double> pw(nmax + 2);
+            pw[0] = 1.0;
+            for (unsigned long k = 1; k <= nmax + 1; k++) pw[k] = pw[k - 1] * q;
+            std::vector<double> qq(grd->g.nlat, q);
+            ratio.alloc(qq.size() * sizeof(double), s);
+            rp.alloc(pw.size() * sizeof(double), s);
+            h2d(ratio.as<double>(), qq.data(), qq.size(), s);
+            h2d(rp.as<double>(), pw.data(), pw.size(), s);
+            d.ratio = ratio.as<double>();
+            d.rp = rp.as<double>();
+            d.rld = 0;
+        }
+        // grids made by charm_crd_point_gl always admit the FFT
+        run_shs_grid(grd->g, d, grd->lon, true, nmax, mu / R, cs_dev, tb, f_dev, s);
+        HIPCHK(hipGetLastError());
+    } catch (const HipFail &e) {
+        report(err, e, __func__);
+    }
+}
+
+void charm_amd_sha_grid_dev(const charm_amd_grid *grd, unsigned long nmax, double mu, double R,
+                            const double *f_dev, double *cs_dev, void *stream, charm_err *err)
+{
+    if (!grd || nmax > grd->nmax_grd) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG, "\"nmax\" exceeds the degree of the grid.");
+        return;
+    }
+    try {
+        hipStream_t s = (hipStream_t)stream;
+        Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
+        const size_t nlat = grd->g.nlat;
+        const double c = sha_const(nlat, grd->r0, mu);
+        std::vector<double> cw(nlat);
+        for (size_t i = 0; i < nlat; i++) cw[i] = c * grd->wh[i];
+        DevBuf dcw(nlat * sizeof(double), s);
+        h2d(dcw.as<double>(), cw.data(), nlat, s);
+        std::vector<double> sc;
+        DevBuf dsc;
+        if (rescale_table(grd->r0, R, nmax, sc)) {
+            dsc.alloc(sc.size() * sizeof(double), s);
+            h2d(dsc.as<double>(), sc.data(), sc.size(), s);
+        }
+        DevGrid d{grd->t, grd->u, grd->mirror, grd->sx, grd->six, grd->sld, nullptr, nullptr, 0};
+        run_sha(grd->g, d, dcw.as<double>(), f_dev, nmax, tb, dsc.as<double>(), nmax, cs_dev,
+                cs_dev + 1, 2, s);
+        HIPCHK(hipGetLastError());
+    } catch (const HipFail &e) {
+        report(err, e, __func__);
+    }
+}
+
+void charm_amd_shs_sctr_dev(size_t npoint, const double *lat_dev, const double *lon_dev,
+                            const double *r_dev, unsigned long nmax, double mu, double R,
+                            const double *cs_dev, double *f_dev, void *stream, charm_err *err)
+{
+    if (npoint == 0) return;
+    try {
+        hipStream_t s = (hipStream_t)stream;
+        Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
+        DevBuf dt(npoint * sizeof(double), s), du(npoint * sizeof(double), s);
+        HIPCHK((hipError_t)launch_sincos(npoint, lat_dev, dt.as<double>(), du.as<double>(), s));
+        // radii equal to R within glob_threshold (src/shs/shs_r_eq_rref.c:17-30)
+        int unit = 0;
+        HIPCHK((hipError_t)launch_all_near(npoint, r_dev, R, charm_glob_threshold, &unit, s));
+        run_sctr(npoint, dt.as<double>(), du.as<double>(), lon_dev, r_dev, unit != 0, nmax, mu, R,
+                 cs_dev, tb, f_dev, s);
+        HIPCHK(hipGetLastError());
+    } catch (const HipFail &e) {
+        report(err, e, __func__);
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,210 @@
+/*
+ * charm_amd.h -- C ABI of libcharm_amd.so, an MI355X (gfx950) implementation
+ * of CHarm's point spherical-harmonic synthesis and analysis in double
+ * precision.
+ *
+ * The structures and the charm_* functions below are drop-in replacements for
+ * the same names in CHarm 0.4.9 built in double precision without MPI: same
+ * signatures, same struct layouts (x86-64 LP64), same error codes and the same
+ * argument checks.  Each declaration cites the CHarm interface it replaces
+ * (paths relative to the CHarm tree).  Functions prefixed charm_amd_ are new:
+ * device-resident and batched entry points with no counterpart in CHarm.
+ *
+ * Floating point arrays passed to charm_* functions are host memory.  Arrays
+ * passed to charm_amd_*_dev functions are device memory (hipMalloc'ed), and
+ * the work is enqueued on the given HIP stream (0 = the null stream).
+ */
+#ifndef CHARM_AMD_H
+#define CHARM_AMD_H
+
+#include <stddef.h>
+#include <stdbool.h>
+
+#ifdef __cplusplus
+/* C's _Bool and C++'s bool have the same size and representation on the
+ * platforms this library supports. */
+#define CHARM_AMD_BOOL bool
+extern "C" {
+#else
+#define CHARM_AMD_BOOL _Bool
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Error handling: src/err/err.h.in:58-294                                  */
+/* ------------------------------------------------------------------------ */
+#define CHARM_ERR_MAX_FILE 4096U
+#define CHARM_ERR_MAX_FUNC 256U
+#define CHARM_ERR_MAX_MSG 4096U
+#define CHARM_ERR_MAX_LEVEL 10U
+
+/* src/err/err.h.in:89-124 */
+enum {
+    CHARM_FAILURE = -1,
+    CHARM_SUCCESS,
+    CHARM_EMEM,
+    CHARM_EFUNCARG,
+    CHARM_EFILEIO,
+    CHARM_EFFTWINIT,
+    CHARM_EMPIPROCESS
+};
+
+/* src/err/err.h.in:159-222 (non-MPI build: 56 bytes) */
+typedef struct {
+    size_t level;
+    char **file;
+    size_t *line;
+    char **func;
+    int code;
+    char *msg;
+    CHARM_AMD_BOOL saturated;
+    CHARM_AMD_BOOL distributed;
+} charm_err;
+
+/* src/err/err.h.in:236 */
+charm_err *charm_err_init(void);
+/* src/err/err.h.in:250 */
+void charm_err_free(charm_err *err);
+/* src/err/err.h.in:262 */
+void charm_err_reset(charm_err *err);
+/* src/err/err.h.in:280 */
+void charm_err_handler(charm_err *err, CHARM_AMD_BOOL terminate);
+/* src/err/err.h.in:294 */
+CHARM_AMD_BOOL charm_err_isempty(const charm_err *err);
+
+/* ------------------------------------------------------------------------ */
+/* Spherical harmonic coefficients: src/shc/shc.h.in:139-602                */
+/* ------------------------------------------------------------------------ */
+/* src/shc/shc.h.in:139-381 (non-MPI build: 64 bytes).  c[m] points at
+ * C_{m,m}; the rows for m = 0..nmax are contiguous from c[0]
+ * (src/shc/shc_init_chunk.c:139-150).  S_{n,0} are stored as zeros. */
+typedef struct {
+    unsigned long nmax;
+    double mu;
+    double r;
+    size_t nc;
+    size_t ns;
+    double **c;
+    double **s;
+    CHARM_AMD_BOOL owner;
+    CHARM_AMD_BOOL distributed;
+} charm_shc;
+
+/* src/shc/shc.h.in:422-426 */
+charm_shc *charm_shc_malloc(unsigned long nmax, double mu, double r);
+/* src/shc/shc.h.in:434-438 */
+charm_shc *charm_shc_calloc(unsigned long nmax, double mu, double r);
+/* src/shc/shc.h.in:545-550 */
+charm_shc *charm_shc_init(unsigned long nmax, double mu, double r, double *c, double *s);
+/* src/shc/shc.h.in:602 */
+void charm_shc_free(charm_shc *shcs);
+
+/* ------------------------------------------------------------------------ */
+/* Evaluation points: src/crd/crd.h.in:72-628                               */
+/* ------------------------------------------------------------------------ */
+/* src/crd/crd.h.in:72-102 */
+enum {
+    CHARM_CRD_CELL_GRID = -2,
+    CHARM_CRD_CELL_SCATTERED,
+    CHARM_CRD_POINT_SCATTERED = 1,
+    CHARM_CRD_POINT_GRID,
+    CHARM_CRD_POINT_GRID_GL,
+    CHARM_CRD_POINT_GRID_DH1,
+    CHARM_CRD_POINT_GRID_DH2
+};
+
+/* src/crd/crd.h.in:125-332 (non-MPI build: 72 bytes) */
+typedef struct {
+    int type;
+    size_t nlat;
+    size_t nlon;
+    size_t npoint;
+    double *lat;
+    double *lon;
+    double *r;
+    double *w;
+    CHARM_AMD_BOOL owner;
+    CHARM_AMD_BOOL distributed;
+} charm_point;
+
+/* src/crd/crd.h.in:477-480 */
+charm_point *charm_crd_point_malloc(int type, size_t nlat, size_t nlon);
+/* src/crd/crd.h.in:489-492 */
+charm_point *charm_crd_point_calloc(int type, size_t nlat, size_t nlon);
+/* src/crd/crd.h.in:542-548 */
+charm_point *charm_crd_point_init(int type, size_t nlat, size_t nlon, double *lat,
+                                  double *lon, double *r);
+/* src/crd/crd.h.in:564 */
+void charm_crd_point_free(charm_point *pnt);
+/* src/crd/crd.h.in:614-616 */
+charm_point *charm_crd_point_gl(unsigned long nmax, double r);
+/* src/crd/crd.h.in:626-628 */
+void charm_crd_point_gl_shape(unsigned long nmax, size_t *nlat, size_t *nlon);
+
+/* ------------------------------------------------------------------------ */
+/* Global tunables: src/glob/glob.h.in:66-149, src/glob/glob.c:16-23        */
+/* ------------------------------------------------------------------------ */
+extern double charm_glob_threshold;
+extern double charm_glob_threshold2;
+extern unsigned long charm_glob_polar_optimization_a1;
+extern double charm_glob_polar_optimization_a2;
+
+/* ------------------------------------------------------------------------ */
+/* Synthesis and analysis: src/shs/shs.h.in:168-173, src/sha/sha.h.in:164-169 */
+/* ------------------------------------------------------------------------ */
+/* src/shs/shs.h.in:168-173.  Point synthesis of shcs up to degree nmax at
+ * pnt (GL grids, custom grids and scattered points); f receives pnt->npoint
+ * values (row-major [nlat][nlon] for grids). */
+void charm_shs_point(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax,
+                     double *f, charm_err *err);
+/* src/sha/sha.h.in:164-169.  Point analysis of f sampled on the quadrature
+ * grid pnt up to degree nmax; overwrites all of shcs->c and shcs->s. */
+void charm_sha_point(const charm_point *pnt, const double *f, unsigned long nmax,
+                     charm_shc *shcs, charm_err *err);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident entry points (no counterpart in CHarm)                    */
+/* ------------------------------------------------------------------------ */
+/* A Gauss-Legendre grid whose latitude tables live in device memory, for
+ * repeated transforms without host round trips.  Created from a host
+ * charm_point of type CHARM_CRD_POINT_GRID_GL; all radii must be equal. */
+typedef struct charm_amd_grid charm_amd_grid;
+
+charm_amd_grid *charm_amd_grid_create(const charm_point *pnt, charm_err *err);
+void charm_amd_grid_free(charm_amd_grid *grd);
+
+/* SHS on a device-resident GL grid.  cs_dev: device array of 2*P doubles,
+ * P = (nmax+1)(nmax+2)/2, holding (C_nm, S_nm) pairs in the m-major order of
+ * charm_shc truncated to degree nmax.  f_dev: device array of nlat*nlon
+ * doubles.  mu, R as in charm_shc.  Work is enqueued on stream. */
+void charm_amd_shs_grid_dev(const charm_amd_grid *grd, unsigned long nmax, double mu,
+                            double R, const double *cs_dev, double *f_dev,
+                            void *stream, charm_err *err);
+/* SHA on a device-resident GL grid; writes cs_dev (2*P doubles, layout as
+ * above) for a model with constants mu, R (coefficients are rescaled from the
+ * grid radius to R). */
+void charm_amd_sha_grid_dev(const charm_amd_grid *grd, unsigned long nmax, double mu,
+                            double R, const double *f_dev, double *cs_dev,
+                            void *stream, charm_err *err);
+/* SHS at npoint scattered points given by device arrays lat, lon, r. */
+void charm_amd_shs_sctr_dev(size_t npoint, const double *lat_dev, const double *lon_dev,
+                            const double *r_dev, unsigned long nmax, double mu, double R,
+                            const double *cs_dev, double *f_dev, void *stream,
+                            charm_err *err);
+
+/* Library information: number of visible HIP devices (0 if none). */
+int charm_amd_device_count(void);
+
+/* Kernel timing: when enabled, HIP events on the launching stream bracket
+ * each launch of the named kernels ("shs_grid_kernel", "fft_c2r",
+ * "sha_kernel", "sha_reduce_kernel", "fft_r2c", "sctr_kernel").
+ * charm_amd_prof_read waits for the recorded events and returns the summed
+ * milliseconds and the launch count of one kernel. */
+void charm_amd_prof_enable(int on);
+void charm_amd_prof_reset(void);
+int charm_amd_prof_read(const char *name, double *total_ms, long *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHARM_AMD_H */

@@ -1,0 +1,261 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and with CHarm's
+own golden fixtures.
+
+Tolerances:
+* CHarm's fixtures: the reference's own test bound, misc_is_nearly_equal with
+  10*glob_threshold (2.2e-13) per value (10*glob_threshold2 for the dsun
+  files), plus the north-star bound max|f - f_ref| <= 1e-12 max|f_ref|.
+* Oracle comparisons at larger degrees: max|f - f_oracle| <= 1e-12 max|f|
+  (synthesis) and max|C - C_oracle| <= 1e-12 max|C| (analysis).
+"""
+import numpy as np
+import pytest
+
+import charm_amd
+from charm_amd import crd, sha, shc, shs
+from charm_amd.workloads import GM, R_EARTH, kaula_coeffs, scattered_points
+from charm_fixtures import (BREAK_SYMM, DELTAR, THRESH, THRESH2, egm96_test_model, nearly_equal,
+                            read_fixture)
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if charm_amd.lib().charm_amd_device_count() < 1:
+        pytest.fail("no HIP device visible: the GPU tests need an MI355X")
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle("d")
+
+
+def relmax(x, ref):
+    x, ref = np.asarray(x), np.asarray(ref)
+    return float(np.max(np.abs(x - ref)) / max(np.max(np.abs(ref)), 1e-300))
+
+
+def model(nmax, C, S, mu, R):
+    return shc.Shc.from_arrays(nmax, np.array(C, dtype=float), np.array(S, dtype=float), mu, R)
+
+
+# --------------------------------------------------------------------------
+# CHarm golden fixtures
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("nmax", range(0, 5))
+@pytest.mark.parametrize("deltar", (0, 1))
+def test_shs_gl_fixtures(nmax, deltar):
+    mu, R, C, S = egm96_test_model()
+    m = model(10, C, S, mu, R)
+    g = crd.PointGridGL(nmax, R + DELTAR * deltar)
+    f = shs.point(g, m, nmax).ravel()
+    ref = read_fixture("double", "shs_p_nx%d_dr%d_fft1_gl.txt" % (nmax, deltar))
+    assert nearly_equal(f, ref, 10 * THRESH["double"]).all(), relmax(f, ref)
+    assert relmax(f, ref) <= TOL
+    # error against CHarm built in quad precision
+    refq = read_fixture("quad", "shs_p_nx%d_dr%d_fft1_gl.txt" % (nmax, deltar))
+    assert relmax(f, refq) <= TOL
+
+
+@pytest.mark.parametrize("nmax", range(0, 5))
+def test_shs_sctr_fixtures(nmax):
+    from charm_fixtures import scattered_points as fixture_points
+
+    mu, R, C, S = egm96_test_model()
+    m = model(10, C, S, mu, R)
+    for i, npt in enumerate([1, 2, 3, 4, 5, 6, 7, 8, 9, 20, 30, 31]):
+        lat, lon = fixture_points(npt, "double")
+        for deltar in (0, 1):
+            p = crd.PointSctr(lat, lon, np.full(npt, R + DELTAR * deltar))
+            f = shs.point(p, m, nmax)
+            ref = read_fixture("double", "shs_p_nx%d_n%d_dr%d_sctr.txt" % (nmax, i, deltar))
+            assert nearly_equal(f, ref, 10 * THRESH["double"]).all(), (i, deltar, relmax(f, ref))
+
+
+@pytest.mark.parametrize("nmax", range(150, 155))
+@pytest.mark.parametrize("s", (0, 1))
+def test_shs_dsun_fixtures(nmax, s):
+    """GL latitudes, one longitude: the lumped sums themselves (non-FFT path)."""
+    ncs = (nmax + 1) * (nmax + 2) // 2
+    from charm_amd.workloads import degree_order
+
+    n_of, m_of = degree_order(nmax)
+    C = 1.0 / (n_of + 1.0)
+    S = np.where(m_of > 0, C, 0.0)
+    gl = crd.PointGridGL(nmax, 1.0)
+    lat = gl.lat.copy()
+    if s == 0:
+        lat[0] -= BREAK_SYMM
+    for deltar in (0, 1):
+        # a GL-typed point set with the shifted latitude and lon = [0]
+        pnt = crd.PointGridGL(nmax, 1.0 + DELTAR * deltar)
+        pnt.lat[:] = lat
+        pnt._Point.contents.nlon = 1
+        pnt._Point.contents.npoint = len(lat)
+        m = model(nmax, C, S, 1.0, 1.0)
+        f = shs.point(pnt, m, nmax).ravel()[: len(lat)]
+        ref = read_fixture("double", "shs_p_nx%d_n0_dr%d_fft0_s%d_dsun.txt" % (nmax, deltar, s))
+        assert nearly_equal(f, ref, 10 * THRESH2["double"]).all(), relmax(f, ref)
+        assert relmax(f, ref) <= TOL
+    assert ncs == C.size
+
+
+# --------------------------------------------------------------------------
+# Oracle comparisons (Kaula-like models)
+# --------------------------------------------------------------------------
+def test_shs_gl_360_all_rings(orc):
+    N = 360
+    C, S = kaula_coeffs(N, 1)
+    g = crd.PointGridGL(N, R_EARTH)
+    f = shs.point(g, model(N, C, S, GM, R_EARTH), N)
+    ref = orc.shs_grid(N, C, S, GM, R_EARTH, g.lat, g.r, g.lon, gl=True)
+    assert relmax(f, ref) <= TOL, relmax(f, ref)
+
+
+def test_shs_gl_2160_sampled_rings(orc):
+    """Full-size grid; the oracle checks polar rings (X-number region),
+    mid-latitude rings, the equator and southern rings."""
+    N = 2160
+    C, S = kaula_coeffs(N, 2)
+    g = crd.PointGridGL(N, R_EARTH)
+    f = shs.point(g, model(N, C, S, GM, R_EARTH), N)
+    rings = [0, 1, 2, 37, 300, 700, 1080, 1500, 2100, 2160]
+    ref = orc.shs_grid(N, C, S, GM, R_EARTH, g.lat, g.r, g.lon, gl=True, rings=rings)
+    assert relmax(f[rings], ref) <= TOL, relmax(f[rings], ref)
+
+
+def test_shs_gl_r_differs_from_R(orc):
+    N = 120
+    C, S = kaula_coeffs(N, 3)
+    g = crd.PointGridGL(N, R_EARTH + 5000.0)
+    f = shs.point(g, model(N, C, S, GM, R_EARTH), N)
+    ref = orc.shs_grid(N, C, S, GM, R_EARTH, g.lat, g.r, g.lon, gl=True)
+    assert relmax(f, ref) <= TOL
+
+
+@pytest.mark.parametrize("symmetric", (True, False))
+@pytest.mark.parametrize("nlon", (7, 400))
+def test_shs_custom_grid(orc, symmetric, nlon):
+    """Custom grids: folded when the latitudes are symmetric; FFT when the
+    longitudes allow it (nlon = 400), direct longitude sums otherwise."""
+    N = 90
+    C, S = kaula_coeffs(N, 4)
+    if symmetric:
+        h = np.linspace(1.5, 0.1, 20)
+        lat = np.concatenate([h, [0.0], -h[::-1]])
+    else:
+        lat = np.linspace(1.4, -0.9, 31)
+    lon = np.arange(nlon) * (2 * np.pi / nlon) if nlon == 400 else np.linspace(0.2, 1.0, nlon)
+    r = np.full(lat.size, R_EARTH + 100.0)
+    p = crd.PointGrid(lat, lon, r)
+    f = shs.point(p, model(N, C, S, GM, R_EARTH), N)
+    ref = orc.shs_grid(N, C, S, GM, R_EARTH, lat, r, lon, gl=False)
+    assert relmax(f, ref) <= TOL, relmax(f, ref)
+
+
+def test_shs_sctr_vs_oracle(orc):
+    N = 360
+    C, S = kaula_coeffs(N, 5)
+    lat, lon, r = scattered_points(3000, 6)
+    f = shs.point(crd.PointSctr(lat, lon, r), model(N, C, S, GM, R_EARTH), N)
+    ref = orc.shs_sctr(N, C, S, GM, R_EARTH, lat, lon, r)
+    assert relmax(f, ref) <= TOL, relmax(f, ref)
+
+
+def test_sha_gl_vs_oracle(orc):
+    N = 200
+    C, S = kaula_coeffs(N, 7)
+    g = crd.PointGridGL(N, R_EARTH)
+    f = shs.point(g, model(N, C, S, GM, R_EARTH), N)
+    out = sha.point(g, f, N, GM, R_EARTH)
+    Co, So = orc.sha_gl(N, f, g.lat, g.w, R_EARTH, GM, R_EARTH)
+    scale = max(np.abs(Co).max(), np.abs(So).max())
+    assert np.abs(out.c - Co).max() <= TOL * scale
+    assert np.abs(out.s - So).max() <= TOL * scale
+    # the analysis inverts the synthesis (exact quadrature)
+    assert np.abs(out.c - C).max() <= TOL * scale
+    assert np.abs(out.s - S).max() <= TOL * scale
+
+
+def test_sha_roundtrip_rescale_and_zero_fill(orc):
+    """Grid at R + 1000 m, coefficients returned for radius R; the output
+    model is of higher degree than the analysis, so the rest is zero."""
+    N = 150
+    C, S = kaula_coeffs(N, 8)
+    R1 = R_EARTH + 1000.0
+    g = crd.PointGridGL(N, R1)
+    f = shs.point(g, model(N, C, S, GM, R_EARTH), N)
+    import ctypes as ct
+
+    out = shc.Shc.from_garbage(N + 10, GM, R_EARTH)
+    out.c[:] = np.nan
+    fn = charm_amd.lib().charm_sha_point
+    fn.restype = None
+    fn.argtypes = [ct.POINTER(crd._Point), ct.POINTER(ct.c_double), ct.c_ulong,
+                   ct.POINTER(shc._Shc), ct.POINTER(charm_amd._err._Err)]
+    err = charm_amd._err.init()
+    fc = np.ascontiguousarray(f)
+    fn(g._Point, fc.ctypes.data_as(ct.POINTER(ct.c_double)), N, out._Shc, err)
+    charm_amd._err.handler(err)
+    charm_amd._err.free(err)
+    Co, So = orc.sha_gl(N, f, g.lat, g.w, R1, GM, R_EARTH, nmax_shc=N + 10)
+    scale = np.abs(Co).max()
+    assert np.isfinite(out.c).all()
+    assert np.abs(out.c - Co).max() <= TOL * scale
+    assert np.abs(out.s - So).max() <= TOL * scale
+    from charm_amd.shc import index
+
+    assert out.c[index(N + 10, N + 5, 3)] == 0.0
+    assert out.mu == GM and out.r == R_EARTH
+
+
+def test_sha_roundtrip_2160():
+    N = 2160
+    C, S = kaula_coeffs(N, 9)
+    g = crd.PointGridGL(N, R_EARTH)
+    f = shs.point(g, model(N, C, S, GM, R_EARTH), N)
+    out = sha.point(g, f, N, GM, R_EARTH)
+    assert np.abs(out.c - C).max() <= TOL * np.abs(C).max()
+    assert np.abs(out.s - S).max() <= TOL * np.abs(C).max()
+
+
+# --------------------------------------------------------------------------
+# Device-resident entry points
+# --------------------------------------------------------------------------
+def test_device_grid_matches_host_api():
+    import torch
+
+    N = 300
+    C, S = kaula_coeffs(N, 10)
+    g = crd.PointGridGL(N, R_EARTH)
+    f_host = shs.point(g, model(N, C, S, GM, R_EARTH), N)
+    dg = charm_amd.device.DeviceGrid(N, R_EARTH)
+    cs = torch.tensor(np.stack([C, S], axis=1), dtype=torch.float64, device="cuda")
+    f = torch.empty((dg.nlat, dg.nlon), dtype=torch.float64, device="cuda")
+    dg.shs(cs, N, GM, R_EARTH, f)
+    out = torch.empty_like(cs)
+    dg.sha(f, N, GM, R_EARTH, out)
+    torch.cuda.synchronize()
+    assert np.array_equal(f.cpu().numpy(), f_host)
+    cs_np = out.cpu().numpy()
+    assert np.abs(cs_np[:, 0] - C).max() <= TOL * np.abs(C).max()
+    assert np.abs(cs_np[:, 1] - S).max() <= TOL * np.abs(C).max()
+
+
+def test_device_sctr_matches_host_api():
+    import torch
+
+    N = 200
+    C, S = kaula_coeffs(N, 11)
+    lat, lon, r = scattered_points(1000, 12)
+    f_host = shs.point(crd.PointSctr(lat, lon, r), model(N, C, S, GM, R_EARTH), N)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")
+    cs = t(np.stack([C, S], axis=1))
+    out = torch.empty(lat.size, dtype=torch.float64, device="cuda")
+    charm_amd.device.shs_sctr(t(lat), t(lon), t(r), cs, N, GM, R_EARTH, out)
+    torch.cuda.synchronize()
+    assert relmax(out.cpu().numpy(), f_host) <= 1e-14

@@ -1,0 +1,131 @@
+"""CPU-side checks of the C ABI library: it loads, exports every symbol the
+header declares, has CHarm's struct layouts, and its host-only functions
+(errors, coefficients, points, Gauss-Legendre grids) behave like CHarm's."""
+import ctypes as ct
+import os
+import re
+
+import numpy as np
+import pytest
+
+import charm_amd
+from charm_amd import crd, shc
+from charm_fixtures import THRESH, nearly_equal, read_fixture
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "charm_amd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    funcs = re.findall(r"\b(charm_[a-z0-9_]+)\s*\(", txt)
+    data = re.findall(r"extern\s+[a-z ]+\s+(charm_[a-z0-9_]+)\s*;", txt)
+    return sorted(set(funcs)), sorted(set(data))
+
+
+def test_exports_every_declared_symbol():
+    lib = charm_amd.lib()
+    funcs, data = header_symbols()
+    assert "charm_shs_point" in funcs and "charm_sha_point" in funcs
+    missing = [s for s in funcs + data if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_struct_layouts_match_charm():
+    # non-MPI x86-64 layouts of src/{shc,crd,err}/*.h.in
+    assert ct.sizeof(shc._Shc) == 64
+    assert shc._Shc.c.offset == 40 and shc._Shc.owner.offset == 56
+    assert ct.sizeof(crd._Point) == 72
+    assert crd._Point.lat.offset == 32 and crd._Point.owner.offset == 64
+    assert ct.sizeof(charm_amd._err._Err) == 56
+    assert charm_amd._err._Err.msg.offset == 40
+
+
+def test_globals_default_values():
+    lib = charm_amd.lib()
+    eps = np.finfo(float).eps
+    assert ct.c_double.in_dll(lib, "charm_glob_threshold").value == 100 * eps
+    assert ct.c_double.in_dll(lib, "charm_glob_threshold2").value == 1e5 * eps
+    assert ct.c_ulong.in_dll(lib, "charm_glob_polar_optimization_a1").value == 100
+    assert ct.c_double.in_dll(lib, "charm_glob_polar_optimization_a2").value == -1.0
+
+
+@pytest.mark.parametrize("nmax", range(0, 11))
+def test_gl_grid_matches_fixtures(nmax):
+    eps = 10 * THRESH["double"]
+    for d in (0, 1):
+        g = crd.PointGridGL(nmax, 1.0 + 1000.0 * d)
+        assert nearly_equal(g.lat, read_fixture("double", "crd_nx%d_dr%d_gl_lat.txt" % (nmax, d)), eps).all()
+        assert nearly_equal(g.w, read_fixture("double", "crd_nx%d_dr%d_gl_w.txt" % (nmax, d)), eps).all()
+        assert nearly_equal(g.lon, read_fixture("double", "crd_nx%d_dr%d_gl_lon.txt" % (nmax, d)), eps).all()
+        assert (g.r == 1.0 + 1000.0 * d).all()
+
+
+def test_gl_grid_matches_oracle_large():
+    from oracle.oracle import Oracle
+
+    lat, w, lon = Oracle("d").gl_grid(400)
+    g = crd.PointGridGL(400, 2.0)
+    assert np.array_equal(g.lat, lat) and np.array_equal(g.w, w) and np.array_equal(g.lon, lon)
+    assert abs(g.w.sum() - 2.0) < 1e-13
+
+
+def test_shc_layout_and_views():
+    c = np.arange(21, dtype=float)
+    s = -np.arange(21, dtype=float)
+    x = shc.Shc.from_arrays(5, c, s, 2.0, 3.0)
+    assert x.nmax == 5 and x.mu == 2.0 and x.r == 3.0 and not x.owner
+    assert x.get_coeffs(3, 2) == (c[shc.index(5, 3, 2)], s[shc.index(5, 3, 2)])
+    z = shc.Shc.from_zeros(7)
+    assert z.owner and z.c.size == 36 and not z.c.any()
+    with pytest.raises(ValueError):
+        shc.Shc.from_zeros(3, 1.0, -1.0)
+
+
+def test_point_checks():
+    with pytest.raises(ValueError):
+        crd.PointSctr(np.zeros(3), np.zeros(4), np.ones(3))  # nlat != nlon
+    p = crd.PointSctr(np.zeros(3), np.zeros(3), np.ones(3))
+    assert p.npoint == 3
+    g = crd.PointGrid(np.zeros(3), np.zeros(5), np.ones(3))
+    assert g.npoint == 15
+
+
+def test_error_structure_roundtrip():
+    err = charm_amd._err.init()
+    assert err.contents.code == 0 and err.contents.level == 0
+    charm_amd._err.free(err)
+
+
+def test_nmax_check_reports_efuncarg():
+    # argument checks run before any device work (src/shs/shs_point.c:37-48)
+    s = shc.Shc.from_zeros(3)
+    g = crd.PointGridGL(5, 1.0)
+    with pytest.raises(ValueError):
+        charm_amd.shs.point(g, s, 4)
+    f = charm_amd.lib().charm_shs_point
+    f.restype = None
+    f.argtypes = [ct.POINTER(crd._Point), ct.POINTER(shc._Shc), ct.c_ulong,
+                  ct.POINTER(ct.c_double), ct.POINTER(charm_amd._err._Err)]
+    err = charm_amd._err.init()
+    out = np.zeros(6 * 12)
+    f(g._Point, s._Shc, 4, out.ctypes.data_as(ct.POINTER(ct.c_double)), err)
+    assert err.contents.code == charm_amd._err.CHARM_EFUNCARG
+    charm_amd._err.free(err)
+
+
+def test_sha_rejects_scattered_points():
+    s = shc.Shc.from_zeros(2)
+    p = crd.PointSctr(np.zeros(3), np.zeros(3), np.ones(3))
+    with pytest.raises(TypeError):
+        charm_amd.sha.point(p, np.zeros((3, 3)), 2)
+
+
+def test_no_device_fails_loudly():
+    """Without a HIP device the transforms raise; there is no CPU fallback."""
+    if charm_amd.lib().charm_amd_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    s = shc.Shc.from_zeros(4, 1.0, 1.0)
+    g = crd.PointGridGL(4, 1.0)
+    with pytest.raises(charm_amd.CHarmError):
+        charm_amd.shs.point(g, s, 4)
