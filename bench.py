@@ -62,10 +62,9 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
 
     import charm_amd
-    from charm_amd.device import DeviceGrid
+    from charm_amd.device import DevArray, DeviceGrid, Stream
     from charm_amd.workloads import GM, R_EARTH, kaula_coeffs
 
     lib = charm_amd.lib()
@@ -73,27 +72,26 @@ def main():
     C, S = kaula_coeffs(N, 2160)
     grid = DeviceGrid(N, R_EARTH)
     nlat, nlon = grid.nlat, grid.nlon
-    dev = torch.device("cuda:0")
-    cs = torch.tensor(np.stack([C, S], axis=1), dtype=torch.float64, device=dev)
-    f = torch.empty((nlat, nlon), dtype=torch.float64, device=dev)
-    out = torch.empty_like(cs)
-    stream = torch.cuda.current_stream()
+    cs = DevArray.from_numpy(np.stack([C, S], axis=1))
+    f = DevArray((nlat, nlon))
+    out = DevArray(cs.shape)
+    stream = Stream()
 
     for _ in range(args.warmup):
         grid.shs(cs, N, GM, R_EARTH, f, stream)
         grid.sha(f, N, GM, R_EARTH, out, stream)
-    torch.cuda.synchronize()
-    o = out.cpu().numpy()
+    stream.synchronize()
+    o = out.numpy()
     rt_err = float(max(np.abs(o[:, 0] - C).max(), np.abs(o[:, 1] - S).max()) / np.abs(C).max())
 
     lib.charm_amd_prof_reset()
     lib.charm_amd_prof_enable(1)
-    torch.cuda.synchronize()
+    stream.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         grid.shs(cs, N, GM, R_EARTH, f, stream)
         grid.sha(f, N, GM, R_EARTH, out, stream)
-    torch.cuda.synchronize()
+    stream.synchronize()
     t1 = time.perf_counter()
     lib.charm_amd_prof_enable(0)
 

@@ -1,31 +1,100 @@
 """Device-resident entry points (no counterpart in CHarm).
 
-Inputs and outputs are device tensors (``torch`` on a ROCm build supplies the
-HBM allocations and the stream); the library only sees raw pointers.
-Coefficients are interleaved (C_nm, S_nm) pairs in CHarm's m-major order,
-i.e. a float64 tensor of shape (P, 2), P = (nmax+1)(nmax+2)/2.
+Device memory comes from the same HIP runtime the library links
+(libamdhip64.so.7 from ROCm), through :class:`DevArray`; the library only sees
+raw pointers and a stream handle.  Coefficients are interleaved
+(C_nm, S_nm) pairs in CHarm's m-major order, i.e. float64 arrays of shape
+(P, 2), P = (nmax+1)(nmax+2)/2.
 """
 import ctypes as _ct
+
+import numpy as _np
 
 from . import _err
 from . import crd as _crd
 from ._lib import lib
 
 _vp = _ct.c_void_p
+_hip = None
 
 
-def _stream_ptr(stream):
-    if stream is None:
-        import torch
+def hip():
+    """The HIP runtime the library is linked against."""
+    global _hip
+    if _hip is None:
+        lib()  # loads libamdhip64.so.7 as a dependency
+        h = _ct.CDLL("libamdhip64.so.7")
+        h.hipMalloc.argtypes = [_ct.POINTER(_vp), _ct.c_size_t]
+        h.hipFree.argtypes = [_vp]
+        h.hipMemcpy.argtypes = [_vp, _vp, _ct.c_size_t, _ct.c_int]
+        h.hipMemsetAsync.argtypes = [_vp, _ct.c_int, _ct.c_size_t, _vp]
+        h.hipStreamCreate.argtypes = [_ct.POINTER(_vp)]
+        h.hipStreamSynchronize.argtypes = [_vp]
+        h.hipStreamDestroy.argtypes = [_vp]
+        h.hipGetErrorString.restype = _ct.c_char_p
+        _hip = h
+    return _hip
 
-        stream = torch.cuda.current_stream()
-    return _vp(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError("%s failed: %s" % (what, hip().hipGetErrorString(rc).decode()))
 
 
-def _ptr(t):
-    if not t.is_contiguous():
-        raise ValueError("device tensors must be contiguous")
-    return _vp(t.data_ptr())
+def synchronize():
+    _check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class Stream:
+    """A HIP stream of the library's runtime (default priority)."""
+
+    def __init__(self):
+        self.handle = _vp()
+        _check(hip().hipStreamCreate(_ct.byref(self.handle)), "hipStreamCreate")
+
+    def synchronize(self):
+        _check(hip().hipStreamSynchronize(self.handle), "hipStreamSynchronize")
+
+    def __del__(self):
+        if getattr(self, "handle", None) and self.handle.value:
+            hip().hipStreamDestroy(self.handle)
+            self.handle = None
+
+
+class DevArray:
+    """A float64 array in device memory."""
+
+    def __init__(self, shape, dtype=_np.float64):
+        self.shape = tuple(int(s) for s in (shape if hasattr(shape, "__len__") else (shape,)))
+        self.dtype = _np.dtype(dtype)
+        self.nbytes = int(_np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = _vp()
+        _check(hip().hipMalloc(_ct.byref(self.ptr), max(self.nbytes, 1)), "hipMalloc")
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = _np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        _check(hip().hipMemcpy(d.ptr, a.ctypes.data_as(_vp), d.nbytes, 1), "hipMemcpy H2D")
+        return d
+
+    def numpy(self):
+        a = _np.empty(self.shape, dtype=self.dtype)
+        _check(hip().hipMemcpy(a.ctypes.data_as(_vp), self.ptr, self.nbytes, 2), "hipMemcpy D2H")
+        return a
+
+    @property
+    def size(self):
+        return int(_np.prod(self.shape))
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            hip().hipFree(self.ptr)
+            self.ptr = None
+
+
+def _s(stream):
+    return stream.handle if isinstance(stream, Stream) else _vp(stream or 0)
 
 
 def _call(name, argtypes, *args):
@@ -62,14 +131,14 @@ class DeviceGrid:
         """Synthesis into ``out`` (nlat, nlon) from ``cs`` (P, 2)."""
         _call("charm_amd_shs_grid_dev",
               [_vp, _ct.c_ulong, _ct.c_double, _ct.c_double, _vp, _vp, _vp],
-              self._h, int(nmax), float(mu), float(R), _ptr(cs), _ptr(out), _stream_ptr(stream))
+              self._h, int(nmax), float(mu), float(R), cs.ptr, out.ptr, _s(stream))
         return out
 
     def sha(self, f, nmax, mu, R, out, stream=None):
         """Analysis of ``f`` (nlat, nlon) into ``out`` (P, 2)."""
         _call("charm_amd_sha_grid_dev",
               [_vp, _ct.c_ulong, _ct.c_double, _ct.c_double, _vp, _vp, _vp],
-              self._h, int(nmax), float(mu), float(R), _ptr(f), _ptr(out), _stream_ptr(stream))
+              self._h, int(nmax), float(mu), float(R), f.ptr, out.ptr, _s(stream))
         return out
 
     def __del__(self):
@@ -83,9 +152,9 @@ class DeviceGrid:
 
 
 def shs_sctr(lat, lon, r, cs, nmax, mu, R, out, stream=None):
-    """Synthesis at scattered points given as device tensors."""
+    """Synthesis at scattered points given as device arrays."""
     _call("charm_amd_shs_sctr_dev",
           [_ct.c_size_t, _vp, _vp, _vp, _ct.c_ulong, _ct.c_double, _ct.c_double, _vp, _vp, _vp],
-          lat.numel(), _ptr(lat), _ptr(lon), _ptr(r), int(nmax), float(mu), float(R), _ptr(cs),
-          _ptr(out), _stream_ptr(stream))
+          lat.size, lat.ptr, lon.ptr, r.ptr, int(nmax), float(mu), float(R), cs.ptr, out.ptr,
+          _s(stream))
     return out

@@ -227,35 +227,36 @@ def test_sha_roundtrip_2160():
 # Device-resident entry points
 # --------------------------------------------------------------------------
 def test_device_grid_matches_host_api():
-    import torch
+    from charm_amd.device import DevArray, DeviceGrid, Stream
 
     N = 300
     C, S = kaula_coeffs(N, 10)
     g = crd.PointGridGL(N, R_EARTH)
     f_host = shs.point(g, model(N, C, S, GM, R_EARTH), N)
-    dg = charm_amd.device.DeviceGrid(N, R_EARTH)
-    cs = torch.tensor(np.stack([C, S], axis=1), dtype=torch.float64, device="cuda")
-    f = torch.empty((dg.nlat, dg.nlon), dtype=torch.float64, device="cuda")
-    dg.shs(cs, N, GM, R_EARTH, f)
-    out = torch.empty_like(cs)
-    dg.sha(f, N, GM, R_EARTH, out)
-    torch.cuda.synchronize()
-    assert np.array_equal(f.cpu().numpy(), f_host)
-    cs_np = out.cpu().numpy()
+    dg = DeviceGrid(N, R_EARTH)
+    st = Stream()
+    cs = DevArray.from_numpy(np.stack([C, S], axis=1))
+    f = DevArray((dg.nlat, dg.nlon))
+    dg.shs(cs, N, GM, R_EARTH, f, st)
+    out = DevArray(cs.shape)
+    dg.sha(f, N, GM, R_EARTH, out, st)
+    st.synchronize()
+    assert np.array_equal(f.numpy(), f_host)
+    cs_np = out.numpy()
     assert np.abs(cs_np[:, 0] - C).max() <= TOL * np.abs(C).max()
     assert np.abs(cs_np[:, 1] - S).max() <= TOL * np.abs(C).max()
 
 
 def test_device_sctr_matches_host_api():
-    import torch
+    from charm_amd.device import DevArray, shs_sctr, synchronize
 
     N = 200
     C, S = kaula_coeffs(N, 11)
     lat, lon, r = scattered_points(1000, 12)
     f_host = shs.point(crd.PointSctr(lat, lon, r), model(N, C, S, GM, R_EARTH), N)
-    t = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")
-    cs = t(np.stack([C, S], axis=1))
-    out = torch.empty(lat.size, dtype=torch.float64, device="cuda")
-    charm_amd.device.shs_sctr(t(lat), t(lon), t(r), cs, N, GM, R_EARTH, out)
-    torch.cuda.synchronize()
-    assert relmax(out.cpu().numpy(), f_host) <= 1e-14
+    cs = DevArray.from_numpy(np.stack([C, S], axis=1))
+    out = DevArray(lat.size)
+    shs_sctr(DevArray.from_numpy(lat), DevArray.from_numpy(lon), DevArray.from_numpy(r), cs, N,
+             GM, R_EARTH, out)
+    synchronize()
+    assert relmax(out.numpy(), f_host) <= 1e-14
