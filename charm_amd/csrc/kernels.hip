@@ -37,12 +37,6 @@ __device__ __forceinline__ size_t tri_len_dev(long nmax)
     return ((size_t)(nmax + 1) * (size_t)(nmax + 2)) / 2;
 }
 
-// X-number to double (src/leg/leg_func_xnum.h:304-314)
-__device__ __forceinline__ double x2f(double x, int ix)
-{
-    return (ix == 0) ? x : ((ix == -1) ? x * BIGI : ((ix < -1) ? 0.0 : x * BIG));
-}
-
 // Keep |x| in [BIGSI, BIGS) (src/leg/leg_func_xnum.h:318-330)
 __device__ __forceinline__ void xnorm(double &x, int &ix)
 {
@@ -61,39 +55,48 @@ __device__ __forceinline__ bool wave_any(bool p)
     return __ballot(p) != 0;
 }
 
-// One tesseral step P_n = (a_n t) P_{n-1} - b_n P_{n-2} for a lane that may
-// still be in X-number mode (src/leg/leg_func_xnum.h:366-443).  Lanes with
-// ds set run the ordinary recurrence.  Returns P_n as an ordinary double.
-//
-// Scaling by BIG^k is done with ldexp, which rounds exactly like the
-// reference's multiplication by a power of two (also on underflow), so
-// x * BIG^ix == ldexp(x, 960 * ix) for the X-to-double conversion.  The rare
-// exponent-alignment cases are taken only when a lane of the wave needs them.
-__device__ __forceinline__ double leg_step_x(double &x, double &y, int &ix, int &iy, bool &ds,
-                                             double at, double b)
+__device__ __forceinline__ bool wave_all(bool p)
 {
-    const int ixy = ix - iy;
-    double xx = x, yy = y;
-    int iz = ix;
-    if (wave_any(ixy != 0)) {
-        yy = (ixy == 1) ? ldexp(y, -960) : ((ixy > 1) ? 0.0 : y);
-        xx = (ixy == -1) ? ldexp(x, -960) : ((ixy < -1) ? 0.0 : x);
-        iz = (ixy < 0) ? iy : ix;
+    return __ballot(!p) == 0;
+}
+
+// Column state of one lane: P_{n-1} = x 2^e and P_{n-2} = y 2^e.
+//
+// CHarm carries the first terms of a column as X-numbers (mantissa in
+// [2^-480, 2^480) and exponent in units of 2^960, src/leg/leg_func_xnum.h)
+// and switches to plain doubles once both terms have a zero exponent, i.e.
+// once both true values lie in [2^-480, 2^480).  Scaling by a power of two is
+// exact, so the same true values come out of the plain recurrence on scaled
+// mantissas with a separate binary exponent e that is refreshed only every
+// 8 degrees; a lane drops e (e = 0) at the same point CHarm leaves
+// X-numbers.  P_n itself is ldexp(z, e), which rounds (also into the
+// subnormal range, and to 0 far below it) exactly as CHarm's x * BIG^ix.
+__device__ __forceinline__ void lane_renorm(double &x, double &y, int &e)
+{
+    if (e != 0) {
+        int ex, ey;
+        (void)frexp(x, &ex);
+        (void)frexp(y, &ey);
+        if (e + ex > -480 && e + ey > -480) {
+            x = ldexp(x, e);
+            y = ldexp(y, e);
+            e = 0;
+        } else if (ex > 256 || ex < -256) {
+            x = ldexp(x, -ex);
+            y = ldexp(y, -ex);
+            e += ex;
+        }
     }
-    double z = fma(at, xx, -(b * yy));
-    const double w = fabs(z);
-    const bool up = !ds && (w >= BIGS);
-    const bool dn = !ds && (w < BIGSI);
-    if (wave_any(up || dn)) {
-        z = ldexp(z, up ? -960 : (dn ? 960 : 0));
-        iz += up ? 1 : (dn ? -1 : 0);
-    }
-    y = x;
-    iy = ix;
-    x = z;
-    ix = iz;
-    ds = ds || (ix == 0 && iy == 0);
-    return ldexp(z, 960 * iz);
+}
+
+// True |P| stays below 2^-1100 for the next 8 degrees (|a_nm t| < 2^7), so
+// these terms are exactly zero in double precision.
+__device__ __forceinline__ bool lane_deep(double x, double y, int e)
+{
+    int ex, ey;
+    (void)frexp(x, &ex);
+    (void)frexp(y, &ey);
+    return e != 0 && e + max(ex, ey) < -1160;
 }
 
 __device__ __forceinline__ int uniform(int v)
@@ -202,8 +205,7 @@ __global__ __launch_bounds__(256) void shs_grid_kernel(ShsGridArgs A)
         const double2 *__restrict__ abm = (const double2 *)A.ab + dcol_off(A.ab_nt, m) - m;
         const double2 *__restrict__ csm = (const double2 *)A.cs + dcol_off(nmax, m) - m;
         double x[K], y[K];
-        int ix[K], iy[K];
-        bool ds[K];
+        int e[K];
         // even/odd (n + m) partial sums (UNIT) or north/south sums (!UNIT)
         double aE[K], aO[K], bE[K], bO[K];
         double rpN[K], rpS[K], qN[K], qS[K];
@@ -211,8 +213,7 @@ __global__ __launch_bounds__(256) void shs_grid_kernel(ShsGridArgs A)
         for (int k = 0; k < K; k++) {
             aE[k] = aO[k] = bE[k] = bO[k] = 0.0;
             x[k] = y[k] = 0.0;
-            ix[k] = iy[k] = 0;
-            ds[k] = true;
+            e[k] = 0;
             rpN[k] = rpS[k] = qN[k] = qS[k] = 1.0;
             if (!UNIT && valid[k]) {
                 const int rs = A.mirror[ring[k]] ? A.nlat - 1 - ring[k] : ring[k];
@@ -244,6 +245,14 @@ __global__ __launch_bounds__(256) void shs_grid_kernel(ShsGridArgs A)
             rpS[k] *= qS[k];                                               \
         }                                                                  \
     } while (0)
+        // advance the radius powers over terms that are exactly zero
+#define SHS_SKIP(k)                                                        \
+    do {                                                                   \
+        if (!UNIT) {                                                       \
+            rpN[k] *= qN[k];                                               \
+            rpS[k] *= qS[k];                                               \
+        }                                                                  \
+    } while (0)
 
         int n;
         if (m == 0) {
@@ -263,53 +272,67 @@ __global__ __launch_bounds__(256) void shs_grid_kernel(ShsGridArgs A)
             }
             n = 2;
         } else {
+            // sectorial seed P_mm (X-number from the seed table) and
+            // semi-sectorial P_{m+1,m} = (a t) P_mm
             const double2 cmm = csm[m];
-            double xs[K];
-            int ixs[K];
+            const double2 ab1 = (m < nmax) ? abm[m + 1] : make_double2(0.0, 0.0);
+            const double2 c1 = (m < nmax) ? csm[m + 1] : make_double2(0.0, 0.0);
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                xs[k] = valid[k] ? A.sx[(size_t)(m - 1) * A.sld + ring[k]] : 0.0;
-                ixs[k] = valid[k] ? A.six[(size_t)(m - 1) * A.sld + ring[k]] : 0;
-                SHS_ACC(k, x2f(xs[k], ixs[k]), cmm, 0);
-            }
-            if (m < nmax) {
-                const double2 ab1 = abm[m + 1];
-                const double2 c1 = csm[m + 1];
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    y[k] = xs[k];
-                    iy[k] = ixs[k];
-                    double xx = (ab1.x * t[k]) * xs[k];
-                    int ixx = ixs[k];
-                    xnorm(xx, ixx);
-                    x[k] = xx;
-                    ix[k] = ixx;
-                    ds[k] = !valid[k];
-                    SHS_ACC(k, x2f(xx, ixx), c1, 1);
+                const double xs = valid[k] ? A.sx[(size_t)(m - 1) * A.sld + ring[k]] : 0.0;
+                e[k] = valid[k] ? 960 * A.six[(size_t)(m - 1) * A.sld + ring[k]] : 0;
+                SHS_ACC(k, ldexp(xs, e[k]), cmm, 0);
+                if (m < nmax) {
+                    y[k] = xs;
+                    x[k] = (ab1.x * t[k]) * xs;
+                    SHS_ACC(k, ldexp(x[k], e[k]), c1, 1);
+                    lane_renorm(x[k], y[k], e[k]);
                 }
             }
             n = m + 2;
         }
 
-        // Tiles of 8 degrees; (n + m) is even at every tile start.  While any
-        // lane of the wave is still in X-number mode the tile takes the
-        // general step; once none is, the wave stays in the tight loop (a lane
-        // never returns to X-numbers).
+        // Tiles of 8 degrees; (n + m) is even at every tile start.  While a
+        // lane of the wave still carries an exponent, tiles convert each term
+        // with ldexp (or, when every lane is far below the double range, run
+        // the bare recurrence); afterwards the wave stays in the tight loop.
         for (;;) {
-            bool xmode = false;
+            if (n + 7 > nmax) break;
+            bool anyx = false, deep = true;
 #pragma unroll
-            for (int k = 0; k < K; k++) xmode = xmode || !ds[k];
-            if (n + 7 > nmax || !wave_any(xmode)) break;
+            for (int k = 0; k < K; k++) {
+                anyx = anyx || (e[k] != 0);
+                deep = deep && (!valid[k] || lane_deep(x[k], y[k], e[k]));
+            }
+            if (!wave_any(anyx)) break;
+            if (wave_all(deep)) {
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const double2 abj = abm[n + j];
-                const double2 cj = csm[n + j];
+                for (int j = 0; j < 8; j++) {
+                    const double2 abj = abm[n + j];
 #pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const double p = leg_step_x(x[k], y[k], ix[k], iy[k], ds[k], abj.x * t[k], abj.y);
-                    SHS_ACC(k, p, cj, j & 1);
+                    for (int k = 0; k < K; k++) {
+                        const double z = fma(abj.x * t[k], x[k], -(abj.y * y[k]));
+                        y[k] = x[k];
+                        x[k] = z;
+                        SHS_SKIP(k);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const double2 abj = abm[n + j];
+                    const double2 cj = csm[n + j];
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        const double z = fma(abj.x * t[k], x[k], -(abj.y * y[k]));
+                        y[k] = x[k];
+                        x[k] = z;
+                        SHS_ACC(k, ldexp(z, e[k]), cj, j & 1);
+                    }
                 }
             }
+#pragma unroll
+            for (int k = 0; k < K; k++) lane_renorm(x[k], y[k], e[k]);
             n += 8;
         }
         for (; n + 7 <= nmax; n += 8) {
@@ -334,11 +357,14 @@ __global__ __launch_bounds__(256) void shs_grid_kernel(ShsGridArgs A)
             const int par = (n + m) & 1;
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                const double p = leg_step_x(x[k], y[k], ix[k], iy[k], ds[k], abj.x * t[k], abj.y);
-                SHS_ACC(k, p, cj, par);
+                const double z = fma(abj.x * t[k], x[k], -(abj.y * y[k]));
+                y[k] = x[k];
+                x[k] = z;
+                SHS_ACC(k, ldexp(z, e[k]), cj, par);
             }
         }
 #undef SHS_ACC
+#undef SHS_SKIP
         if (m == 0) {
             // S_n0 does not exist; the zonal sums have no sine part
 #pragma unroll
@@ -505,23 +531,21 @@ __global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
     double *dst = (double *)((double2 *)A.part + (size_t)tile * tri_len_dev(nmax) + dcol_off(nmax, m) - m);
 
     double x[K], y[K];
-    int ix[K], iy[K];
-    bool ds[K];
+    int e[K];
     double v[16];
 #pragma unroll
     for (int j = 0; j < 16; j++) v[j] = 0.0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         x[k] = y[k] = 0.0;
-        ix[k] = iy[k] = 0;
-        ds[k] = true;
+        e[k] = 0;
     }
 
     // degrees m and m + 1
     if (m == 0) {
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            v[0] += 1.0 * ampC[k];
+            v[0] += ampC[k];
             if (nmax >= 1) {
                 const double p1 = ROOT3 * t[k];
                 v[2] = fma(p1, ammC[k], v[2]);
@@ -534,22 +558,17 @@ __global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const double xs = valid[k] ? A.sx[(size_t)(m - 1) * A.sld + ring[k]] : 0.0;
-            const int ixs = valid[k] ? A.six[(size_t)(m - 1) * A.sld + ring[k]] : 0;
-            const double p0 = x2f(xs, ixs);
+            e[k] = valid[k] ? 960 * A.six[(size_t)(m - 1) * A.sld + ring[k]] : 0;
+            const double p0 = ldexp(xs, e[k]);
             v[0] = fma(p0, ampC[k], v[0]);
             v[1] = fma(p0, ampS[k], v[1]);
             if (m < nmax) {
                 y[k] = xs;
-                iy[k] = ixs;
-                double xx = (ab1.x * t[k]) * xs;
-                int ixx = ixs;
-                xnorm(xx, ixx);
-                x[k] = xx;
-                ix[k] = ixx;
-                ds[k] = !valid[k];
-                const double p1 = x2f(xx, ixx);
+                x[k] = (ab1.x * t[k]) * xs;
+                const double p1 = ldexp(x[k], e[k]);
                 v[2] = fma(p1, ammC[k], v[2]);
                 v[3] = fma(p1, ammS[k], v[3]);
+                lane_renorm(x[k], y[k], e[k]);
             }
         }
     }
@@ -557,48 +576,70 @@ __global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
 
     int n = m + 2;
     for (;;) {
-        bool xmode = false;
+        if (n + 7 > nmax) break;
+        bool anyx = false, deep = true;
 #pragma unroll
-        for (int k = 0; k < K; k++) xmode = xmode || !ds[k];
-        if (n + 7 > nmax || !wave_any(xmode)) break;
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const double2 abj = abm[n + j];
-            double c = 0.0, s = 0.0;
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                const double p = leg_step_x(x[k], y[k], ix[k], iy[k], ds[k], abj.x * t[k], abj.y);
-                c = fma(p, (j & 1) ? ammC[k] : ampC[k], c);
-                s = fma(p, (j & 1) ? ammS[k] : ampS[k], s);
-            }
-            v[2 * j] = c;
-            v[2 * j + 1] = s;
+        for (int k = 0; k < K; k++) {
+            anyx = anyx || (e[k] != 0);
+            deep = deep && (!valid[k] || lane_deep(x[k], y[k], e[k]));
         }
-        wave_reduce_store(v, 8, red, dst + 2 * n, lane);
+        if (!wave_any(anyx)) break;
+        if (wave_all(deep)) {
+            // every term of the tile is exactly zero
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const double2 abj = abm[n + j];
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const double z = fma(abj.x * t[k], x[k], -(abj.y * y[k]));
+                    y[k] = x[k];
+                    x[k] = z;
+                }
+            }
+            if (lane < 16) dst[2 * n + lane] = 0.0;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const double2 abj = abm[n + j];
+                double c = 0.0, sv = 0.0;
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const double z = fma(abj.x * t[k], x[k], -(abj.y * y[k]));
+                    y[k] = x[k];
+                    x[k] = z;
+                    const double p = ldexp(z, e[k]);
+                    c = fma(p, (j & 1) ? ammC[k] : ampC[k], c);
+                    sv = fma(p, (j & 1) ? ammS[k] : ampS[k], sv);
+                }
+                v[2 * j] = c;
+                v[2 * j + 1] = sv;
+            }
+            wave_reduce_store(v, 8, red, dst + 2 * n, lane);
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) lane_renorm(x[k], y[k], e[k]);
         n += 8;
     }
     for (; n + 7 <= nmax; n += 8) {
-        {
 #pragma unroll
-            for (int j = 0; j < 8; j += 2) {
-                const double2 ab0 = abm[n + j], ab1 = abm[n + j + 1];
-                double c0 = 0.0, s0 = 0.0, c1 = 0.0, s1 = 0.0;
+        for (int j = 0; j < 8; j += 2) {
+            const double2 ab0 = abm[n + j], ab1 = abm[n + j + 1];
+            double c0 = 0.0, s0 = 0.0, c1 = 0.0, s1 = 0.0;
 #pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const double z = fma(ab0.x * t[k], x[k], -(ab0.y * y[k]));
-                    c0 = fma(z, ampC[k], c0);
-                    s0 = fma(z, ampS[k], s0);
-                    const double w = fma(ab1.x * t[k], z, -(ab1.y * x[k]));
-                    c1 = fma(w, ammC[k], c1);
-                    s1 = fma(w, ammS[k], s1);
-                    y[k] = z;
-                    x[k] = w;
-                }
-                v[2 * j] = c0;
-                v[2 * j + 1] = s0;
-                v[2 * j + 2] = c1;
-                v[2 * j + 3] = s1;
+            for (int k = 0; k < K; k++) {
+                const double z = fma(ab0.x * t[k], x[k], -(ab0.y * y[k]));
+                c0 = fma(z, ampC[k], c0);
+                s0 = fma(z, ampS[k], s0);
+                const double w = fma(ab1.x * t[k], z, -(ab1.y * x[k]));
+                c1 = fma(w, ammC[k], c1);
+                s1 = fma(w, ammS[k], s1);
+                y[k] = z;
+                x[k] = w;
             }
+            v[2 * j] = c0;
+            v[2 * j + 1] = s0;
+            v[2 * j + 2] = c1;
+            v[2 * j + 3] = s1;
         }
         wave_reduce_store(v, 8, red, dst + 2 * n, lane);
     }
@@ -606,18 +647,21 @@ __global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
         const int cnt = nmax - n + 1;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            double c = 0.0, s = 0.0;
+            double c = 0.0, sv = 0.0;
             if (j < cnt) {
                 const double2 abj = abm[n + j];
 #pragma unroll
                 for (int k = 0; k < K; k++) {
-                    const double p = leg_step_x(x[k], y[k], ix[k], iy[k], ds[k], abj.x * t[k], abj.y);
+                    const double z = fma(abj.x * t[k], x[k], -(abj.y * y[k]));
+                    y[k] = x[k];
+                    x[k] = z;
+                    const double p = ldexp(z, e[k]);
                     c = fma(p, (j & 1) ? ammC[k] : ampC[k], c);
-                    s = fma(p, (j & 1) ? ammS[k] : ampS[k], s);
+                    sv = fma(p, (j & 1) ? ammS[k] : ampS[k], sv);
                 }
             }
             v[2 * j] = c;
-            v[2 * j + 1] = s;
+            v[2 * j + 1] = sv;
         }
         wave_reduce_store(v, cnt, red, dst + 2 * n, lane);
     }
@@ -672,7 +716,7 @@ __global__ __launch_bounds__(256) void sctr_kernel(SctrArgs A)
     const double q = (!UNIT && valid) ? A.R / A.r[i] : 1.0;
     const int nmax = A.nmax;
 
-    // P_{mb,mb} (for mb >= 1) and (R/r)^(mb+1)
+    // P_{mb,mb} (for mb >= 1) as an X-number, and (R/r)^(mb+1)
     double xs = ROOT3 * u;
     int ixs = 0;
     for (int mm = 1; mm < mb; mm++) {
@@ -689,14 +733,13 @@ __global__ __launch_bounds__(256) void sctr_kernel(SctrArgs A)
         const double2 *__restrict__ csm = (const double2 *)A.cs + dcol_off(nmax, m) - m;
         double a = 0.0, b = 0.0, rp = rpc;
         double x = 0.0, y = 0.0;
-        int ix = 0, iy = 0;
-        bool ds = true;
-#define SC_ACC(p, c)                       \
-    do {                                   \
+        int e = 0;
+#define SC_ACC(p, c)                              \
+    do {                                          \
         const double pq_ = UNIT ? (p) : rp * (p); \
-        a = fma(pq_, (c).x, a);            \
-        b = fma(pq_, (c).y, b);            \
-        if (!UNIT) rp *= q;                \
+        a = fma(pq_, (c).x, a);                   \
+        b = fma(pq_, (c).y, b);                   \
+        if (!UNIT) rp *= q;                       \
     } while (0)
         int n;
         if (m == 0) {
@@ -708,25 +751,38 @@ __global__ __launch_bounds__(256) void sctr_kernel(SctrArgs A)
             }
             n = 2;
         } else {
-            SC_ACC(x2f(xs, ixs), csm[m]);
+            e = valid ? 960 * ixs : 0;
+            SC_ACC(ldexp(xs, e), csm[m]);
             if (m < nmax) {
                 y = xs;
-                iy = ixs;
                 x = (abm[m + 1].x * t) * xs;
-                ix = ixs;
-                xnorm(x, ix);
-                ds = !valid;
-                SC_ACC(x2f(x, ix), csm[m + 1]);
+                SC_ACC(ldexp(x, e), csm[m + 1]);
+                lane_renorm(x, y, e);
             }
             n = m + 2;
         }
-        while (n + 7 <= nmax && wave_any(!ds)) {
+        for (;;) {
+            if (n + 7 > nmax || !wave_any(e != 0)) break;
+            if (wave_all(!valid || lane_deep(x, y, e))) {
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const double2 ab = abm[n + j];
-                const double p = leg_step_x(x, y, ix, iy, ds, ab.x * t, ab.y);
-                SC_ACC(p, csm[n + j]);
+                for (int j = 0; j < 8; j++) {
+                    const double2 ab = abm[n + j];
+                    const double z = fma(ab.x * t, x, -(ab.y * y));
+                    y = x;
+                    x = z;
+                    if (!UNIT) rp *= q;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const double2 ab = abm[n + j];
+                    const double z = fma(ab.x * t, x, -(ab.y * y));
+                    y = x;
+                    x = z;
+                    SC_ACC(ldexp(z, e), csm[n + j]);
+                }
             }
+            lane_renorm(x, y, e);
             n += 8;
         }
         for (; n + 7 <= nmax; n += 8) {
@@ -741,8 +797,10 @@ __global__ __launch_bounds__(256) void sctr_kernel(SctrArgs A)
         }
         for (; n <= nmax; n++) {
             const double2 ab = abm[n];
-            const double p = leg_step_x(x, y, ix, iy, ds, ab.x * t, ab.y);
-            SC_ACC(p, csm[n]);
+            const double z = fma(ab.x * t, x, -(ab.y * y));
+            y = x;
+            x = z;
+            SC_ACC(ldexp(z, e), csm[n]);
         }
 #undef SC_ACC
         if (m == 0) b = 0.0;
