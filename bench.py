@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--nmax", type=int, default=2160)
     ap.add_argument("--cpu-rows", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="before timing, copy this many MiB device-to-device once "
+                         "(a reference kernel of known traffic for counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); "
@@ -76,6 +79,15 @@ def main():
     f = DevArray((nlat, nlon))
     out = DevArray(cs.shape)
     stream = Stream()
+
+    if args.copy_mb > 0:
+        from charm_amd.device import hip, synchronize
+
+        nb = args.copy_mb << 20
+        src, dst = DevArray(nb // 8), DevArray(nb // 8)
+        hip().hipMemcpy(dst.ptr, src.ptr, nb, 3)
+        synchronize()
+        del src, dst
 
     for _ in range(args.warmup):
         grid.shs(cs, N, GM, R_EARTH, f, stream)
