@@ -5,6 +5,6 @@ Modules mirror PyHarm: :mod:`shc` (coefficients), :mod:`crd` (points),
 :mod:`shs` (synthesis), :mod:`sha` (analysis); :mod:`device` holds the
 device-resident entry points.
 """
-from . import _err, crd, device, sha, shc, shs, workloads  # noqa: F401
+from . import _err, crd, device, serve, sha, shc, shs, workloads  # noqa: F401
 from ._err import CHarmError  # noqa: F401
 from ._lib import LIB_PATH, build, lib  # noqa: F401

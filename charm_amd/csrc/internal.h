@@ -56,6 +56,7 @@ struct ShsGridArgs {
     size_t ldo;
     double mur;        // mu / R
     int fft_layout;    // 1: out = Hermitian half-spectrum for c2r; 0: raw (a, b)
+    int k;             // ring pairs per lane (1..3)
 };
 
 struct ShaArgs {
@@ -74,6 +75,7 @@ struct ShaArgs {
     const void *ab;
     int ab_nt;
     void *part;        // [tile][tri_len(nmax)] double2 partial sums
+    int k;             // ring pairs per lane (2..4)
 };
 
 struct SctrArgs {
@@ -114,12 +116,22 @@ int launch_sctr_reduce(size_t npoint, int nchunk, const double *part_dev, double
 int launch_pack_cs(const double *c_dev, const double *s_dev, size_t n, void *cs_dev,
                    void *stream);
 int launch_sincos(size_t n, const double *lat_dev, double *t_dev, double *u_dev, void *stream);
+int launch_zero_ints(int *p, int n, void *stream);
+int launch_cs_truncate(const double *src_dev, unsigned long nmax_src, double *dst_dev,
+                       unsigned long nmax, void *stream);
+// Persistent batch kernels: kind 0 = synthesis, 1 = analysis.
+int batch_ring_k(int kind);
+int batch_grid_size(int kind);
+int launch_shs_batch(const void *reqs_dev, const void *items_dev, int nitems, int *counter,
+                     int grid, void *stream);
+int launch_sha_batch(const void *reqs_dev, const void *items_dev, int nitems, int *counter,
+                     int grid, void *stream);
 int launch_all_near(size_t n, const double *x_dev, double ref, double eps, int *result,
                     void *stream);
 
-// Ring-tile width of the grid kernels (ring pairs per lane times 64).
-constexpr int SHS_K = 2;
-constexpr int SHA_K = 4;
+// Ring pairs per lane for nring rings, kmin <= k <= kmax: the fewest idle
+// lanes in the last ring tile, then the most rings per lane.
+int ring_k(int nring, int kmin, int kmax);
 
 }  // namespace charm_amd
 

@@ -174,14 +174,11 @@ __global__ void rpows_kernel(int nrow, const double *__restrict__ ratio, int kma
 // quad: all tiles of a column share that XCD's L2.
 // ---------------------------------------------------------------------------
 template <int K, bool UNIT>
-__global__ __launch_bounds__(256) void shs_grid_kernel(ShsGridArgs A)
+__device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, int quad,
+                                              double2 (*stage)[64 * K][4])
 {
-    __shared__ double2 stage[2][64 * K][4];
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
-    const int b = blockIdx.x;
-    const int tile = (b >> 3) % A.ntile;
-    const int quad = ((b >> 3) / A.ntile) * 8 + (b & 7);
     const int m0 = quad * 4;
     if (m0 >= A.mcount) return;
     const int m = m0 + wave;
@@ -412,6 +409,38 @@ __global__ __launch_bounds__(256) void shs_grid_kernel(ShsGridArgs A)
     }
 }
 
+template <int K, bool UNIT>
+__global__ __launch_bounds__(256) void shs_grid_kernel(ShsGridArgs A)
+{
+    __shared__ double2 stage[2][64 * K][4];
+    const int b = blockIdx.x;
+    shs_grid_item<K, UNIT>(A, (b >> 3) % A.ntile, ((b >> 3) / A.ntile) * 8 + (b & 7), stage);
+}
+
+// Persistent form: blocks pull (request, ring tile, order quad) items from a
+// list sorted by decreasing cost until the list is exhausted.
+template <int K>
+__global__ __launch_bounds__(256) void shs_batch_kernel(const ShsGridArgs *__restrict__ reqs,
+                                                        const int2 *__restrict__ items, int nitems,
+                                                        int *counter)
+{
+    __shared__ double2 stage[2][64 * K][4];
+    __shared__ int cur;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) cur = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = cur;
+        if (it >= nitems) break;
+        const int2 w = items[it];
+        const ShsGridArgs A = reqs[w.x];
+        if (A.unit)
+            shs_grid_item<K, true>(A, w.y >> 16, w.y & 0xFFFF, stage);
+        else
+            shs_grid_item<K, false>(A, w.y >> 16, w.y & 0xFFFF, stage);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Longitude sums for grids that do not admit an FFT (custom longitudes):
 // f_ij = mu/R sum_m (a_im cos m lon_j + b_im sin m lon_j)
@@ -445,40 +474,85 @@ __global__ void lon_direct_kernel(int nlat, int nmax, size_t nlon, const double 
 // per (tile, n, m) goes to global memory; sha_reduce_kernel adds the tiles.
 // ---------------------------------------------------------------------------
 namespace {
-constexpr int RED_LD = 65;  // padded row of the per-wave transpose buffer
+constexpr int RED_LD = 66;  // padded row (16-byte aligned) of the per-wave transpose buffer
 
-// Reduce v[0..15] (= 8 degrees x (C, S)) over the 64 lanes of a wave and
-// store the first 2*cnt sums at dst[0..2*cnt).
-__device__ __forceinline__ void wave_reduce_store(const double v[16], int cnt, double *red,
-                                                  double *dst, int lane)
+__device__ __forceinline__ double dpp_quad_xor1(double v)
 {
-#pragma unroll
-    for (int j = 0; j < 16; j++) red[j * RED_LD + lane] = v[j];
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    const int j = lane >> 2, q = lane & 3;
-    double s = 0.0;
-#pragma unroll
-    for (int r = 0; r < 16; r++) s += red[j * RED_LD + q * 16 + r];
-    s += __shfl_xor(s, 1);
-    s += __shfl_xor(s, 2);
-    if (q == 0 && j < 2 * cnt) dst[j] = s;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    int2 w = __builtin_bit_cast(int2, v);
+    w.x = __builtin_amdgcn_mov_dpp(w.x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    w.y = __builtin_amdgcn_mov_dpp(w.y, 0xB1, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, w);
 }
+
+__device__ __forceinline__ double dpp_quad_xor2(double v)
+{
+    int2 w = __builtin_bit_cast(int2, v);
+    w.x = __builtin_amdgcn_mov_dpp(w.x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    w.y = __builtin_amdgcn_mov_dpp(w.y, 0x4E, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, w);
+}
+
+// Cross-lane reduction of 16 values per lane (8 degrees x (C, S)) over the 64
+// lanes of a wave, software-pipelined across tiles: a tile's values are
+// written to LDS after the tile is computed, read back at the start of the
+// next tile and summed after that tile's arithmetic, so the LDS latency hides
+// behind the recurrence.  Lane l sums value l/4 over 16 lanes; the 4 lanes of
+// a quad then combine through DPP.  The order of additions is fixed.
+struct WaveReducer {
+    double *red;      // [16][RED_LD] doubles, this wave's slice of LDS
+    double *dst;      // destination of the pending tile (nullptr: none)
+    int cnt;          // degrees in the pending tile
+    double r[16];     // pending values read back from LDS
+    int lane;
+
+    __device__ __forceinline__ void begin()
+    {
+        if (dst) {
+            const int j = lane >> 2, q = lane & 3;
+            const double2 *row = (const double2 *)(red + j * RED_LD + q * 16);
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const double2 w = row[i];
+                r[2 * i] = w.x;
+                r[2 * i + 1] = w.y;
+            }
+        }
+    }
+    __device__ __forceinline__ void finish()
+    {
+        if (dst) {
+            double s = r[0];
+#pragma unroll
+            for (int i = 1; i < 16; i++) s += r[i];
+            s += dpp_quad_xor1(s);
+            s += dpp_quad_xor2(s);
+            const int j = lane >> 2;
+            if ((lane & 3) == 0 && j < 2 * cnt) dst[j] = s;
+            dst = nullptr;
+        }
+    }
+    __device__ __forceinline__ void post(const double v[16], int c, double *d)
+    {
+#pragma unroll
+        for (int j = 0; j < 16; j++) red[j * RED_LD + lane] = v[j];
+        dst = d;
+        cnt = c;
+    }
+};
 }  // namespace
 
 template <int K>
-__global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
+struct ShaLds {
+    static constexpr int STAGE = 2 * 64 * K * 4 * 2;  // doubles
+    static constexpr int RED = 4 * 16 * RED_LD;       // doubles
+    static constexpr int SIZE = STAGE > RED ? STAGE : RED;
+};
+
+template <int K>
+__device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, double *lds)
 {
-    constexpr int STAGE = 2 * 64 * K * 4 * 2;   // doubles
-    constexpr int RED = 4 * 16 * RED_LD;        // doubles
-    __shared__ double lds[STAGE > RED ? STAGE : RED];
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
-    const int b = blockIdx.x;
-    const int tile = (b >> 3) % A.ntile;
-    const int quad = ((b >> 3) / A.ntile) * 8 + (b & 7);
     const int m0 = quad * 4;
     if (m0 > A.nmax) return;
     const int m = m0 + wave;
@@ -526,9 +600,13 @@ __global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
     __syncthreads();
     if (m > nmax) return;
 
-    double *red = lds + wave * 16 * RED_LD;
     const double2 *__restrict__ abm = (const double2 *)A.ab + dcol_off(A.ab_nt, m) - m;
     double *dst = (double *)((double2 *)A.part + (size_t)tile * tri_len_dev(nmax) + dcol_off(nmax, m) - m);
+    WaveReducer rd;
+    rd.red = lds + wave * 16 * RED_LD;
+    rd.dst = nullptr;
+    rd.cnt = 0;
+    rd.lane = lane;
 
     double x[K], y[K];
     int e[K];
@@ -572,7 +650,7 @@ __global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
             }
         }
     }
-    wave_reduce_store(v, (m < nmax) ? 2 : 1, red, dst + 2 * m, lane);
+    rd.post(v, (m < nmax) ? 2 : 1, dst + 2 * m);
 
     int n = m + 2;
     for (;;) {
@@ -584,6 +662,7 @@ __global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
             deep = deep && (!valid[k] || lane_deep(x[k], y[k], e[k]));
         }
         if (!wave_any(anyx)) break;
+        rd.begin();
         if (wave_all(deep)) {
             // every term of the tile is exactly zero
 #pragma unroll
@@ -596,6 +675,7 @@ __global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
                     x[k] = z;
                 }
             }
+            rd.finish();
             if (lane < 16) dst[2 * n + lane] = 0.0;
         } else {
 #pragma unroll
@@ -614,13 +694,15 @@ __global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
                 v[2 * j] = c;
                 v[2 * j + 1] = sv;
             }
-            wave_reduce_store(v, 8, red, dst + 2 * n, lane);
+            rd.finish();
+            rd.post(v, 8, dst + 2 * n);
         }
 #pragma unroll
         for (int k = 0; k < K; k++) lane_renorm(x[k], y[k], e[k]);
         n += 8;
     }
     for (; n + 7 <= nmax; n += 8) {
+        rd.begin();
 #pragma unroll
         for (int j = 0; j < 8; j += 2) {
             const double2 ab0 = abm[n + j], ab1 = abm[n + j + 1];
@@ -641,10 +723,12 @@ __global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
             v[2 * j + 2] = c1;
             v[2 * j + 3] = s1;
         }
-        wave_reduce_store(v, 8, red, dst + 2 * n, lane);
+        rd.finish();
+        rd.post(v, 8, dst + 2 * n);
     }
     if (n <= nmax) {
         const int cnt = nmax - n + 1;
+        rd.begin();
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             double c = 0.0, sv = 0.0;
@@ -663,7 +747,37 @@ __global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
             v[2 * j] = c;
             v[2 * j + 1] = sv;
         }
-        wave_reduce_store(v, cnt, red, dst + 2 * n, lane);
+        rd.finish();
+        rd.post(v, cnt, dst + 2 * n);
+    }
+    rd.begin();
+    rd.finish();
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
+{
+    __shared__ __align__(16) double lds[ShaLds<K>::SIZE];
+    const int b = blockIdx.x;
+    sha_item<K>(A, (b >> 3) % A.ntile, ((b >> 3) / A.ntile) * 8 + (b & 7), lds);
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void sha_batch_kernel(const ShaArgs *__restrict__ reqs,
+                                                        const int2 *__restrict__ items, int nitems,
+                                                        int *counter)
+{
+    __shared__ __align__(16) double lds[ShaLds<K>::SIZE];
+    __shared__ int cur;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) cur = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = cur;
+        if (it >= nitems) break;
+        const int2 w = items[it];
+        const ShaArgs A = reqs[w.x];
+        sha_item<K>(A, w.y >> 16, w.y & 0xFFFF, lds);
     }
 }
 
@@ -891,15 +1005,42 @@ int launch_rpows(int nrow, const double *ratio, int kmax, size_t rld, double *rp
     return (int)hipGetLastError();
 }
 
+int ring_k(int nring, int kmin, int kmax)
+{
+    // rings per lane: the fewest idle lanes in the last tile, then the most
+    // rings per lane
+    int best = kmax;
+    long best_cap = -1;
+    for (int k = kmax; k >= kmin; k--) {
+        const long tiles = (nring + 64L * k - 1) / (64L * k);
+        const long cap = tiles * 64L * k;
+        if (best_cap < 0 || cap < best_cap) {
+            best_cap = cap;
+            best = k;
+        }
+    }
+    return best;
+}
+
+template <int K>
+static void shs_launch_k(const ShsGridArgs &a, unsigned nblk, hipStream_t s)
+{
+    if (a.unit)
+        hipLaunchKernelGGL((shs_grid_kernel<K, true>), dim3(nblk), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((shs_grid_kernel<K, false>), dim3(nblk), dim3(256), 0, s, a);
+}
+
 int launch_shs_grid(const ShsGridArgs &a, void *stream)
 {
     const int nquad = (a.mcount + 3) / 4;
     const int nqg = (nquad + 7) / 8;
     const unsigned nblk = 8u * (unsigned)nqg * (unsigned)a.ntile;
-    if (a.unit)
-        hipLaunchKernelGGL((shs_grid_kernel<SHS_K, true>), dim3(nblk), dim3(256), 0, S(stream), a);
-    else
-        hipLaunchKernelGGL((shs_grid_kernel<SHS_K, false>), dim3(nblk), dim3(256), 0, S(stream), a);
+    switch (a.k) {
+    case 1: shs_launch_k<1>(a, nblk, S(stream)); break;
+    case 3: shs_launch_k<3>(a, nblk, S(stream)); break;
+    default: shs_launch_k<2>(a, nblk, S(stream)); break;
+    }
     return (int)hipGetLastError();
 }
 
@@ -917,7 +1058,11 @@ int launch_sha(const ShaArgs &a, void *stream)
     const int nquad = (a.nmax + 1 + 3) / 4;
     const int nqg = (nquad + 7) / 8;
     const unsigned nblk = 8u * (unsigned)nqg * (unsigned)a.ntile;
-    hipLaunchKernelGGL((sha_kernel<SHA_K>), dim3(nblk), dim3(256), 0, S(stream), a);
+    switch (a.k) {
+    case 2: hipLaunchKernelGGL((sha_kernel<2>), dim3(nblk), dim3(256), 0, S(stream), a); break;
+    case 3: hipLaunchKernelGGL((sha_kernel<3>), dim3(nblk), dim3(256), 0, S(stream), a); break;
+    default: hipLaunchKernelGGL((sha_kernel<4>), dim3(nblk), dim3(256), 0, S(stream), a); break;
+    }
     return (int)hipGetLastError();
 }
 
@@ -971,6 +1116,76 @@ int launch_all_near(size_t n, const double *x, double ref, double eps, int *resu
     if (e == hipSuccess) e = hipStreamSynchronize(S(stream));
     hipFreeAsync(flag, S(stream));
     return (int)e;
+}
+
+// (C, S) pairs of a model of degree ns, cut to degree nmax (m-major order)
+__global__ void cs_truncate_kernel(const double2 *__restrict__ src, int ns, double2 *__restrict__ dst,
+                                   int nmax)
+{
+    const int m = blockIdx.y;
+    const double2 *sc = src + dcol_off(ns, m) - m;
+    double2 *dc = dst + dcol_off(nmax, m) - m;
+    for (int n = m + blockIdx.x * blockDim.x + threadIdx.x; n <= nmax; n += gridDim.x * blockDim.x)
+        dc[n] = sc[n];
+}
+
+int launch_cs_truncate(const double *src, unsigned long ns, double *dst, unsigned long nmax,
+                       void *stream)
+{
+    const unsigned gx = (unsigned)((nmax + 1 + 255) / 256);
+    hipLaunchKernelGGL(cs_truncate_kernel, dim3(gx, (unsigned)nmax + 1), dim3(256), 0, S(stream),
+                       (const double2 *)src, (int)ns, (double2 *)dst, (int)nmax);
+    return (int)hipGetLastError();
+}
+
+__global__ void zero_ints_kernel(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+int launch_zero_ints(int *p, int n, void *stream)
+{
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 255) / 256), dim3(256), 0, S(stream), p, n);
+    return (int)hipGetLastError();
+}
+
+constexpr int BATCH_SHS_K = 2;
+constexpr int BATCH_SHA_K = 3;
+
+int batch_ring_k(int kind)
+{
+    return kind == 0 ? BATCH_SHS_K : BATCH_SHA_K;
+}
+
+int batch_grid_size(int kind)
+{
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    hipError_t e = (kind == 0)
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, shs_batch_kernel<BATCH_SHS_K>, 256, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, sha_batch_kernel<BATCH_SHA_K>, 256, 0);
+    if (e != hipSuccess || per < 1) per = 1;
+    return ncu * per;
+}
+
+int launch_shs_batch(const void *reqs, const void *items, int nitems, int *counter, int grid,
+                     void *stream)
+{
+    if (nitems <= 0) return 0;
+    hipLaunchKernelGGL((shs_batch_kernel<BATCH_SHS_K>), dim3(grid), dim3(256), 0, S(stream),
+                       (const ShsGridArgs *)reqs, (const int2 *)items, nitems, counter);
+    return (int)hipGetLastError();
+}
+
+int launch_sha_batch(const void *reqs, const void *items, int nitems, int *counter, int grid,
+                     void *stream)
+{
+    if (nitems <= 0) return 0;
+    hipLaunchKernelGGL((sha_batch_kernel<BATCH_SHA_K>), dim3(grid), dim3(256), 0, S(stream),
+                       (const ShaArgs *)reqs, (const int2 *)items, nitems, counter);
+    return (int)hipGetLastError();
 }
 
 int launch_pack_cs(const double *c, const double *s, size_t n, void *cs, void *stream)

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -84,7 +85,7 @@ void init_pool()
 
 struct ThreadState {
     hipStream_t stream = nullptr;
-    std::map<std::tuple<int, size_t, size_t, size_t>, hipfftHandle> plans;
+    std::map<std::tuple<int, size_t, size_t, size_t, hipStream_t>, hipfftHandle> plans;
     void *pinned = nullptr;
     size_t pinned_size = 0;
     ~ThreadState()
@@ -186,11 +187,13 @@ struct ProfScope {
     }
 };
 
-// kind 0: c2r (Z2D), 1: r2c (D2Z); n = transform length; batch rows
+// kind 0: c2r (Z2D), 1: r2c (D2Z); n = transform length; batch rows.  Plans are
+// kept per host thread and stream: a plan's work area must not be shared by
+// transforms running concurrently on different streams.
 hipfftHandle fft_plan(int kind, size_t n, size_t batch, size_t ldc, hipStream_t stream)
 {
     ThreadState &st = tls();
-    auto key = std::make_tuple(kind, n, batch, ldc);
+    auto key = std::make_tuple(kind, n, batch, ldc, stream);
     auto it = st.plans.find(key);
     hipfftHandle h;
     if (it == st.plans.end()) {
@@ -359,6 +362,68 @@ struct DevGrid {
     size_t rld;
 };
 
+// ---------------------------------------------------------------------------
+// Ring-tile autotuner: on the first large transform of a given kind, ring
+// count and degree, every candidate number of ring pairs per lane is timed
+// with HIP events on the caller's stream and the fastest is kept for the
+// process.  Small transforms and streams under graph capture use the
+// heuristic of ring_k.  CHARM_AMD_SHS_K / CHARM_AMD_SHA_K force a value.
+// ---------------------------------------------------------------------------
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int>, int> g_tuned;
+
+int forced_k(const char *env, int kmin, int kmax)
+{
+    const char *v = std::getenv(env);
+    if (!v) return 0;
+    int k = std::atoi(v);
+    return (k >= kmin && k <= kmax) ? k : 0;
+}
+
+template <class Launch>
+int tuned_k(int kind, int nring, unsigned long nmax, int kmin, int kmax, hipStream_t s,
+            Launch launch)
+{
+    if (int k = forced_k(kind == 0 ? "CHARM_AMD_SHS_K" : "CHARM_AMD_SHA_K", kmin, kmax)) return k;
+    const int heuristic = (kind == 0) ? 2 : ring_k(nring, kmin, kmax);
+    const double work = (double)nring * (double)(nmax + 1) * (double)(nmax + 1);
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (work < 2e8 || hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
+        return heuristic;
+    const auto key = std::make_tuple(kind, nring, (int)nmax);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) return it->second;
+    }
+    int best = heuristic;
+    float best_ms = 0.0f;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    for (int k = kmin; k <= kmax; k++) {
+        launch(k);  // warm-up (code load, caches)
+        HIPCHK(hipEventRecord(a, s));
+        launch(k);
+        HIPCHK(hipEventRecord(b, s));
+        HIPCHK(hipEventSynchronize(b));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, a, b));
+        if (k == kmin || ms < best_ms) {
+            best_ms = ms;
+            best = k;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (std::getenv("CHARM_AMD_TUNE_LOG"))
+        std::fprintf(stderr, "charm_amd tune: kind %d nring %d nmax %lu -> K=%d (%.3f ms)\n", kind,
+                     nring, nmax, best, (double)best_ms);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[key] = best;
+    return best;
+}
+
 void run_shs_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev, bool use_fft,
                   unsigned long nmax, double mur, const void *cs_dev, const Tables &tb,
                   double *f_dev, hipStream_t stream)
@@ -368,7 +433,6 @@ void run_shs_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev, bo
     a.nmax = (int)nmax;
     a.nring = (int)g.nring;
     a.nlat = (int)nlat;
-    a.ntile = ring_tiles(g.nring, SHS_K);
     a.unit = d.ratio == nullptr;
     a.t = d.t;
     a.sx = d.sx;
@@ -382,6 +446,15 @@ void run_shs_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev, bo
     a.ab_nt = tb.nt;
     a.cs = cs_dev;
     a.mur = mur;
+    auto pick_k = [&](ShsGridArgs &x) {
+        x.k = tuned_k(0, (int)g.nring, nmax, 1, 3, stream, [&](int k) {
+            ShsGridArgs y = x;
+            y.k = k;
+            y.ntile = ring_tiles(g.nring, k);
+            HIPCHK((hipError_t)launch_shs_grid(y, stream));
+        });
+        x.ntile = ring_tiles(g.nring, x.k);
+    };
     if (use_fft) {
         const size_t nfc = nlon / 2 + 1;
         DevBuf spec(nlat * nfc * 2 * sizeof(double), stream);
@@ -389,6 +462,7 @@ void run_shs_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev, bo
         a.ldo = nfc;
         a.mcount = (int)nfc;
         a.fft_layout = 1;
+        pick_k(a);
         {
             ProfScope ps("shs_grid_kernel", stream);
             HIPCHK((hipError_t)launch_shs_grid(a, stream));
@@ -405,6 +479,7 @@ void run_shs_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev, bo
         a.ldo = ldl;
         a.mcount = (int)ldl;
         a.fft_layout = 0;
+        pick_k(a);
         HIPCHK((hipError_t)launch_shs_grid(a, stream));
         HIPCHK((hipError_t)launch_lon_direct((int)nlat, (int)nmax, nlon, lon_dev, lc.p, ldl, mur,
                                              f_dev, stream));
@@ -618,7 +693,6 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
     a.nmax = (int)nmax;
     a.nring = (int)g.nring;
     a.nlat = (int)nlat;
-    a.ntile = ring_tiles(g.nring, SHA_K);
     a.t = d.t;
     a.sx = d.sx;
     a.six = d.six;
@@ -629,8 +703,16 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
     a.lds = nfc;
     a.ab = tb.ab;
     a.ab_nt = tb.nt;
-    DevBuf part((size_t)a.ntile * tri_len(nmax) * 2 * sizeof(double), s);
+    // partial sums sized for the smallest tile width (most tiles)
+    DevBuf part((size_t)ring_tiles(g.nring, 2) * tri_len(nmax) * 2 * sizeof(double), s);
     a.part = part.p;
+    a.k = tuned_k(1, (int)g.nring, nmax, 2, 4, s, [&](int k) {
+        ShaArgs y = a;
+        y.k = k;
+        y.ntile = ring_tiles(g.nring, k);
+        HIPCHK((hipError_t)launch_sha(y, s));
+    });
+    a.ntile = ring_tiles(g.nring, a.k);
     {
         ProfScope ps("sha_kernel", s);
         HIPCHK((hipError_t)launch_sha(a, s));
@@ -966,6 +1048,20 @@ void charm_amd_sha_grid_dev(const charm_amd_grid *grd, unsigned long nmax, doubl
     }
 }
 
+void charm_amd_cs_truncate(const double *src_dev, unsigned long nmax_src, double *dst_dev,
+                           unsigned long nmax, void *stream, charm_err *err)
+{
+    if (nmax > nmax_src) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG, "\"nmax\" exceeds the degree of the source model.");
+        return;
+    }
+    try {
+        HIPCHK((hipError_t)launch_cs_truncate(src_dev, nmax_src, dst_dev, nmax, stream));
+    } catch (const HipFail &e) {
+        report(err, e, __func__);
+    }
+}
+
 void charm_amd_shs_sctr_dev(size_t npoint, const double *lat_dev, const double *lon_dev,
                             const double *r_dev, unsigned long nmax, double mu, double R,
                             const double *cs_dev, double *f_dev, void *stream, charm_err *err)
@@ -985,6 +1081,280 @@ void charm_amd_shs_sctr_dev(size_t npoint, const double *lat_dev, const double *
     } catch (const HipFail &e) {
         report(err, e, __func__);
     }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched execution
+// ---------------------------------------------------------------------------
+struct charm_amd_batch {
+    std::vector<charm_amd_request> req;
+    std::vector<void *> owned;
+    ShsGridArgs *d_shs = nullptr;
+    ShaArgs *d_sha = nullptr;
+    int2 *d_shs_items = nullptr, *d_sha_items = nullptr;
+    int n_shs_items = 0, n_sha_items = 0;
+    int *d_counter = nullptr;
+    struct Job {
+        size_t r;
+        void *spec;
+        void *part;
+        int ntile;
+        const double *scale;
+        size_t nlat, nlon, nfc;
+    };
+    std::vector<Job> shs, sha;
+    int grid_shs = 0, grid_sha = 0;
+    bool use_graph = false;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap = nullptr;
+};
+
+namespace charm_amd {
+namespace {
+
+void *dmalloc(charm_amd_batch *b, size_t bytes)
+{
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+    b->owned.push_back(p);
+    return p;
+}
+
+template <class T>
+T *dupload(charm_amd_batch *b, const std::vector<T> &v, hipStream_t s)
+{
+    T *p = (T *)dmalloc(b, v.size() * sizeof(T));
+    h2d(p, v.data(), v.size(), s);
+    return p;
+}
+
+void batch_enqueue(charm_amd_batch *b, hipStream_t s)
+{
+    HIPCHK((hipError_t)launch_zero_ints(b->d_counter, 2, s));
+    for (const auto &j : b->sha) {
+        const charm_amd_request &q = b->req[j.r];
+        hipfftHandle h = fft_plan(1, j.nlon, j.nlat, j.nfc, s);
+        FFTCHK(hipfftExecD2Z(h, (hipfftDoubleReal *)q.f, (hipfftDoubleComplex *)j.spec));
+    }
+    HIPCHK((hipError_t)launch_sha_batch(b->d_sha, b->d_sha_items, b->n_sha_items,
+                                        b->d_counter + 1, b->grid_sha, s));
+    for (const auto &j : b->sha) {
+        const charm_amd_request &q = b->req[j.r];
+        HIPCHK((hipError_t)launch_sha_reduce((int)q.nmax, j.ntile, j.part, (int)q.nmax, j.scale,
+                                             q.cs, q.cs + 1, 2, s));
+    }
+    HIPCHK((hipError_t)launch_shs_batch(b->d_shs, b->d_shs_items, b->n_shs_items, b->d_counter,
+                                        b->grid_shs, s));
+    for (const auto &j : b->shs) {
+        const charm_amd_request &q = b->req[j.r];
+        hipfftHandle h = fft_plan(0, j.nlon, j.nlat, j.nfc, s);
+        FFTCHK(hipfftExecZ2D(h, (hipfftDoubleComplex *)j.spec, q.f));
+    }
+}
+
+}  // namespace
+}  // namespace charm_amd
+
+extern "C" {
+
+charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nreq, int use_graph,
+                                        charm_err *err)
+{
+    for (size_t i = 0; i < nreq; i++) {
+        const charm_amd_request &q = reqs[i];
+        if (!q.grid || (q.kind != CHARM_AMD_SHS_GRID && q.kind != CHARM_AMD_SHA_GRID) ||
+            q.nmax > q.grid->nmax_grd || !q.cs || !q.f) {
+            char msg[CHARM_ERR_MAX_MSG];
+            std::snprintf(msg, sizeof msg, "Invalid batch request %zu.", i);
+            CHARM_AMD_ERR(err, CHARM_EFUNCARG, msg);
+            return nullptr;
+        }
+    }
+    charm_amd_batch *b = new charm_amd_batch();
+    b->req.assign(reqs, reqs + nreq);
+    b->use_graph = use_graph != 0;
+    try {
+        hipStream_t s = thread_stream();
+        unsigned long nmax_all = 1;
+        for (auto &q : b->req) nmax_all = std::max(nmax_all, q.nmax);
+        Tables tb = tables((int)nmax_all, s);
+        std::vector<ShsGridArgs> shs_args;
+        std::vector<ShaArgs> sha_args;
+        struct Item {
+            long cost;
+            int2 w;
+        };
+        std::vector<Item> shs_items, sha_items;
+        for (size_t i = 0; i < nreq; i++) {
+            const charm_amd_request &q = b->req[i];
+            const charm_amd_grid *g = q.grid;
+            const size_t nlat = g->g.nlat, nlon = g->g.nlon, nfc = nlon / 2 + 1;
+            const int nmax = (int)q.nmax;
+            if (q.kind == CHARM_AMD_SHS_GRID) {
+                const int K = batch_ring_k(0);
+                ShsGridArgs a{};
+                a.nmax = nmax;
+                a.nring = (int)g->g.nring;
+                a.nlat = (int)nlat;
+                a.k = K;
+                a.ntile = ring_tiles(g->g.nring, K);
+                a.t = g->t;
+                a.sx = g->sx;
+                a.six = g->six;
+                a.sld = g->sld;
+                a.mirror = g->mirror;
+                a.ab = tb.ab;
+                a.ab_nt = tb.nt;
+                a.cs = q.cs;
+                a.mur = q.mu / q.R;
+                a.fft_layout = 1;
+                a.mcount = (int)nfc;
+                a.ldo = nfc;
+                a.unit = 1;
+                if (!nearly_equal(g->r0, q.R, charm_glob_threshold)) {
+                    const double qq = q.R / g->r0;
+                    std::vector<double> pw(q.nmax + 2), rat(nlat, qq);
+                    pw[0] = 1.0;
+                    for (unsigned long k = 1; k <= q.nmax + 1; k++) pw[k] = pw[k - 1] * qq;
+                    a.ratio = dupload(b, rat, s);
+                    a.rp = dupload(b, pw, s);
+                    a.rld = 0;
+                    a.unit = 0;
+                }
+                void *spec = dmalloc(b, nlat * nfc * 2 * sizeof(double));
+                a.out = spec;
+                const int idx = (int)shs_args.size();
+                shs_args.push_back(a);
+                b->shs.push_back({i, spec, nullptr, a.ntile, nullptr, nlat, nlon, nfc});
+                const int nquad = (a.mcount + 3) / 4;
+                for (int qd = 0; qd < nquad; qd++)
+                    for (int tl = 0; tl < a.ntile; tl++) {
+                        const long m0 = 4L * qd;
+                        const long cost = (m0 <= nmax ? (nmax + 1 - m0) : 0) + 1;
+                        shs_items.push_back({cost, make_int2(idx, (tl << 16) | qd)});
+                    }
+            } else {
+                const int K = batch_ring_k(1);
+                ShaArgs a{};
+                a.nmax = nmax;
+                a.nring = (int)g->g.nring;
+                a.nlat = (int)nlat;
+                a.k = K;
+                a.ntile = ring_tiles(g->g.nring, K);
+                a.t = g->t;
+                a.sx = g->sx;
+                a.six = g->six;
+                a.sld = g->sld;
+                a.mirror = g->mirror;
+                const double c = sha_const(nlat, g->r0, q.mu);
+                std::vector<double> cw(nlat);
+                for (size_t r = 0; r < nlat; r++) cw[r] = c * g->wh[r];
+                a.cw = dupload(b, cw, s);
+                void *spec = dmalloc(b, nlat * nfc * 2 * sizeof(double));
+                a.spec = spec;
+                a.lds = nfc;
+                a.ab = tb.ab;
+                a.ab_nt = tb.nt;
+                void *part = dmalloc(b, (size_t)a.ntile * tri_len(q.nmax) * 2 * sizeof(double));
+                a.part = part;
+                std::vector<double> sc;
+                const double *scale = nullptr;
+                if (rescale_table(g->r0, q.R, q.nmax, sc)) scale = dupload(b, sc, s);
+                const int idx = (int)sha_args.size();
+                sha_args.push_back(a);
+                b->sha.push_back({i, spec, part, a.ntile, scale, nlat, nlon, nfc});
+                const int nquad = (nmax + 1 + 3) / 4;
+                for (int qd = 0; qd < nquad; qd++)
+                    for (int tl = 0; tl < a.ntile; tl++)
+                        sha_items.push_back({(long)(nmax + 1 - 4 * qd), make_int2(idx, (tl << 16) | qd)});
+            }
+        }
+        auto sort_items = [](std::vector<Item> &v) {
+            std::stable_sort(v.begin(), v.end(),
+                             [](const Item &x, const Item &y) { return x.cost > y.cost; });
+            std::vector<int2> w(v.size());
+            for (size_t i = 0; i < v.size(); i++) w[i] = v[i].w;
+            return w;
+        };
+        std::vector<int2> wi = sort_items(shs_items), wa = sort_items(sha_items);
+        b->n_shs_items = (int)wi.size();
+        b->n_sha_items = (int)wa.size();
+        if (!shs_args.empty()) b->d_shs = dupload(b, shs_args, s);
+        if (!sha_args.empty()) b->d_sha = dupload(b, sha_args, s);
+        if (!wi.empty()) b->d_shs_items = dupload(b, wi, s);
+        if (!wa.empty()) b->d_sha_items = dupload(b, wa, s);
+        b->d_counter = (int *)dmalloc(b, 2 * sizeof(int));
+        b->grid_shs = batch_grid_size(0);
+        b->grid_sha = batch_grid_size(1);
+        HIPCHK(hipStreamSynchronize(s));
+    } catch (const HipFail &e) {
+        report(err, e, __func__);
+        charm_amd_batch_free(b);
+        return nullptr;
+    }
+    return b;
+}
+
+void charm_amd_batch_run(charm_amd_batch *b, void *stream, charm_err *err)
+{
+    if (!b) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG, "NULL batch.");
+        return;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    try {
+        if (b->exec) {
+            HIPCHK(hipGraphLaunch(b->exec, s));
+            return;
+        }
+        batch_enqueue(b, s);
+        if (b->use_graph) {
+            // record the same work once for later replays
+            HIPCHK(hipStreamSynchronize(s));
+            if (!b->cap) HIPCHK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+            // FFT plans for the capture stream are made (and warmed) before the
+            // capture: plan creation allocates and synchronises
+            for (const auto &j : b->sha) (void)fft_plan(1, j.nlon, j.nlat, j.nfc, b->cap);
+            for (const auto &j : b->shs) (void)fft_plan(0, j.nlon, j.nlat, j.nfc, b->cap);
+            batch_enqueue(b, b->cap);
+            HIPCHK(hipStreamSynchronize(b->cap));
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+            try {
+                batch_enqueue(b, b->cap);
+            } catch (...) {
+                (void)hipStreamEndCapture(b->cap, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(b->cap, &graph));
+            hipError_t e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                b->exec = nullptr;
+                b->use_graph = false;
+            }
+        }
+    } catch (const HipFail &e) {
+        report(err, e, __func__);
+    }
+}
+
+int charm_amd_batch_uses_graph(const charm_amd_batch *b)
+{
+    return (b && b->exec) ? 1 : 0;
+}
+
+void charm_amd_batch_free(charm_amd_batch *b)
+{
+    if (!b) return;
+    (void)hipDeviceSynchronize();
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    for (void *p : b->owned) (void)hipFree(p);
+    delete b;
 }
 
 }  // extern "C"

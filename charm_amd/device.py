@@ -158,3 +158,60 @@ def shs_sctr(lat, lon, r, cs, nmax, mu, R, out, stream=None):
           lat.size, lat.ptr, lon.ptr, r.ptr, int(nmax), float(mu), float(R), cs.ptr, out.ptr,
           _s(stream))
     return out
+
+
+SHS_GRID = 0
+SHA_GRID = 1
+
+
+class _Request(_ct.Structure):
+    _fields_ = [("kind", _ct.c_int),
+                ("nmax", _ct.c_ulong),
+                ("mu", _ct.c_double),
+                ("R", _ct.c_double),
+                ("grid", _vp),
+                ("cs", _vp),
+                ("f", _vp)]
+
+
+class Batch:
+    """Many independent grid requests executed together (one persistent
+    launch per transform kind), optionally replayed as a HIP graph.
+
+    ``requests``: sequence of (kind, nmax, mu, R, DeviceGrid, cs DevArray,
+    f DevArray) with kind SHS_GRID (cs -> f) or SHA_GRID (f -> cs)."""
+
+    def __init__(self, requests, use_graph=True):
+        self._keep = list(requests)
+        arr = (_Request * len(self._keep))()
+        for i, (kind, nmax, mu, R, grid, cs, f) in enumerate(self._keep):
+            arr[i] = _Request(int(kind), int(nmax), float(mu), float(R), grid._h, cs.ptr, f.ptr)
+        fn = lib().charm_amd_batch_create
+        fn.restype = _vp
+        fn.argtypes = [_ct.POINTER(_Request), _ct.c_size_t, _ct.c_int, _ct.POINTER(_err._Err)]
+        err = _err.init()
+        try:
+            h = fn(arr, len(self._keep), 1 if use_graph else 0, err)
+            _err.handler(err, True)
+        finally:
+            _err.free(err)
+        self._h = _vp(h)
+
+    def run(self, stream=None):
+        _call("charm_amd_batch_run", [_vp, _vp], self._h, _s(stream))
+
+    @property
+    def uses_graph(self):
+        f = lib().charm_amd_batch_uses_graph
+        f.restype = _ct.c_int
+        f.argtypes = [_vp]
+        return bool(f(self._h))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            f = lib().charm_amd_batch_free
+            f.restype = None
+            f.argtypes = [_vp]
+            f(h)
+            self._h = None

@@ -191,6 +191,76 @@ void charm_amd_shs_sctr_dev(size_t npoint, const double *lat_dev, const double *
                             const double *cs_dev, double *f_dev, void *stream,
                             charm_err *err);
 
+/* Copy the (C, S) pairs of a device model of degree nmax_src, cut to degree
+ * nmax <= nmax_src, into dst_dev (the layout the *_dev functions take). */
+void charm_amd_cs_truncate(const double *src_dev, unsigned long nmax_src, double *dst_dev,
+                           unsigned long nmax, void *stream, charm_err *err);
+
+/* Batched execution: many independent requests in one persistent launch per
+ * transform kind, optionally replayed as a HIP graph.  Synthesis requests
+ * (kind CHARM_AMD_SHS_GRID) read cs and write f; analysis requests
+ * (CHARM_AMD_SHA_GRID) read f and write cs; layouts as for
+ * charm_amd_shs_grid_dev / charm_amd_sha_grid_dev.  All pointers are device
+ * memory and must stay valid for the life of the batch; their contents may
+ * change between runs. */
+enum { CHARM_AMD_SHS_GRID = 0, CHARM_AMD_SHA_GRID = 1 };
+
+typedef struct {
+    int kind;
+    unsigned long nmax;
+    double mu;
+    double R;
+    const charm_amd_grid *grid;
+    double *cs;
+    double *f;
+} charm_amd_request;
+
+typedef struct charm_amd_batch charm_amd_batch;
+
+charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nreq, int use_graph,
+                                        charm_err *err);
+/* Enqueue one execution of every request of the batch on stream.  With
+ * use_graph, the first run executes eagerly and records a HIP graph that
+ * later runs replay. */
+void charm_amd_batch_run(charm_amd_batch *b, void *stream, charm_err *err);
+int charm_amd_batch_uses_graph(const charm_amd_batch *b);
+void charm_amd_batch_free(charm_amd_batch *b);
+
+/* Serving: a worker thread admits requests in arrival order under an HBM
+ * byte budget and a limit of requests in flight, runs each on its own stream
+ * (so requests overlap on the device), and completes them asynchronously.
+ * Models and Gauss-Legendre grids (with their seed tables) stay resident.
+ * Host arrays passed in a request must stay valid until charm_amd_server_wait
+ * returns for it. */
+enum { CHARM_AMD_SRV_SHS_SCTR = 0, CHARM_AMD_SRV_SHS_GRID = 1, CHARM_AMD_SRV_SHA_GRID = 2 };
+
+typedef struct {
+    int kind;
+    int model;                /* SHS kinds: id from charm_amd_server_add_model */
+    unsigned long nmax;       /* degree of the transform */
+    size_t npoint;            /* SHS_SCTR: points given by lat, lon, r */
+    const double *lat, *lon, *r;
+    unsigned long grid_nmax;  /* grid kinds: the GL grid of degree grid_nmax >= nmax */
+    double r_grid;            /* grid kinds: radius of the grid */
+    const double *f_in;       /* SHA_GRID: grid values, (grid_nmax+1) x 2(grid_nmax+1) */
+    double mu, R;             /* SHA_GRID: constants of the output coefficients */
+    double *out;              /* SHS: values (npoint or grid); SHA: (C, S) pairs */
+} charm_amd_srv_request;
+
+typedef struct charm_amd_server charm_amd_server;
+
+charm_amd_server *charm_amd_server_create(size_t hbm_budget_bytes, int max_inflight, charm_err *err);
+/* Copies shcs up to degree nmax into HBM; returns the model id (-1 on error). */
+int charm_amd_server_add_model(charm_amd_server *s, const charm_shc *shcs, unsigned long nmax,
+                               charm_err *err);
+/* Queues a request; returns a ticket (> 0), or -1 on error. */
+long charm_amd_server_submit(charm_amd_server *s, const charm_amd_srv_request *r, charm_err *err);
+/* Waits for a ticket; *latency_ms = completion time minus submission time. */
+int charm_amd_server_wait(charm_amd_server *s, long ticket, double *latency_ms, charm_err *err);
+/* HBM bytes currently accounted to models, grids and requests in flight. */
+size_t charm_amd_server_in_use(charm_amd_server *s);
+void charm_amd_server_free(charm_amd_server *s);
+
 /* Library information: number of visible HIP devices (0 if none). */
 int charm_amd_device_count(void);
 
