@@ -1,13 +1,23 @@
-"""Benchmark: FP64 point synthesis + analysis round trip on a Gauss-Legendre
-grid at nmax = 2160 (2161 x 4322 points), Kaula-like seeded coefficients,
-GM = 3.986004415e14, R = 6378136.3 m, r = R, inputs resident in HBM.
+"""Benchmarks of CHarm-style point synthesis (SHS) and analysis (SHA) in FP64
+on one MI355X, on seeded synthetic inputs (GM = 3.986004415e14,
+R = 6378136.3 m, Kaula-like coefficients: C_00 = 1, degree 1 zero,
+C_nm, S_nm ~ N(0, (1e-5/n^2)^2)).
 
-One step = one SHS (coefficients -> grid) plus one SHA (grid ->
-coefficients) through the device-resident C ABI.  The metric counts
-(nmax+1)^2 * nlat units per transform, i.e. 2 (nmax+1)^2 nlat per step.
+Workloads (--workload):
+  gl2160  (default) SHS + SHA round trip on the Gauss-Legendre grid of
+          nmax = 2160 (2161 x 4322 points), r = R, inputs resident in HBM.  One
+          step = one SHS + one SHA; value = 2 (nmax+1)^2 nlat / step time.
+          Also times SHS at 1e6 scattered points once (lat = asin U(-1,1),
+          lon = U(0, 2 pi), r = R + U(0, 9000 m)) and the CPU baseline.
+  gl10800 the same round trip at nmax = 10800 (10801 x 21602 points).
+  batch   64 requests, nmax ~ U{90..2700}, each with its own coefficients and
+          GL grid, half SHS and half SHA: per-call vs one batched launch.
+  serve   Poisson arrivals against a resident nmax = 2190 model (60% scattered
+          SHS of 1e3..1e6 points, 30% GL SHS on grids nmax ~ U{180..5400}
+          with the model truncated to the grid, 10% GL SHA nmax ~ U{180..2160});
+          p50/p99 latency.
 
-Prints one JSON line (see the keys below).  --gpus N > 1 is not supported:
-this build runs on one GPU.
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
 import argparse
 import ctypes as ct
@@ -20,22 +30,30 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X vector FP64 (also its FP64 matrix peak)
-PEAK_HBM_GBS = 8000.0
+UNIT = "(nmax+1)^2*nlat/s"
 
 
 def kernel_ms(lib, name):
+    """(total ms, launches) recorded by the library's event timers."""
     tot, cnt = ct.c_double(), ct.c_long()
     lib.charm_amd_prof_read.argtypes = [ct.c_char_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_long)]
     lib.charm_amd_prof_read(name.encode(), ct.byref(tot), ct.byref(cnt))
     return tot.value, cnt.value
 
 
-def cpu_baseline(N, C, S, lat, w, lon, R, GM, rows, threads):
+def base_line(metric, value, steps, warmup, ms, config):
+    return {"metric": metric, "value": value, "unit": UNIT, "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded Kaula-like coefficients)", "config": config}
+
+
+def cpu_baseline(N, C, S, lat, lon, R, GM, rows, threads):
     """The oracle (scalar C port of CHarm's algorithm) on a bounded sample of
     the same workload: SHS of `rows` GL rows at degree N on the host."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    from oracle.oracle import Oracle
     import numpy as np
+    from oracle.oracle import Oracle
 
     orc = Oracle("d")
     ring_idx = np.linspace(0, len(lat) - 1, rows).round().astype(np.int64)
@@ -46,24 +64,7 @@ def cpu_baseline(N, C, S, lat, w, lon, R, GM, rows, threads):
     return rows * (N + 1) ** 2 / dt, dt
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nmax", type=int, default=2160)
-    ap.add_argument("--cpu-rows", type=int, default=256)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-mb", type=int, default=0,
-                    help="before timing, copy this many MiB device-to-device once "
-                         "(a reference kernel of known traffic for counter scaling)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (--gpus %d requested); "
-              "multi-GPU sharding is described in DESIGN.md, not implemented." % args.gpus,
-              file=sys.stderr)
-        sys.exit(2)
-
+def gl_roundtrip(args, N, with_sctr, with_cpu):
     import numpy as np
 
     import charm_amd
@@ -71,8 +72,7 @@ def main():
     from charm_amd.workloads import GM, R_EARTH, kaula_coeffs
 
     lib = charm_amd.lib()
-    N = args.nmax
-    C, S = kaula_coeffs(N, 2160)
+    C, S = kaula_coeffs(N, N)
     grid = DeviceGrid(N, R_EARTH)
     nlat, nlon = grid.nlat, grid.nlon
     cs = DevArray.from_numpy(np.stack([C, S], axis=1))
@@ -94,7 +94,10 @@ def main():
         grid.sha(f, N, GM, R_EARTH, out, stream)
     stream.synchronize()
     o = out.numpy()
-    rt_err = float(max(np.abs(o[:, 0] - C).max(), np.abs(o[:, 1] - S).max()) / np.abs(C).max())
+    scale = np.abs(C).max()
+    rt_err = float(max(np.abs(o[:, 0] - C).max(), np.abs(o[:, 1] - S).max()) / scale)
+    print("# warm-up done (%d steps), round-trip rel err %.2e" % (args.warmup, rt_err),
+          file=sys.stderr, flush=True)
 
     lib.charm_amd_prof_reset()
     lib.charm_amd_prof_enable(1)
@@ -108,55 +111,277 @@ def main():
     lib.charm_amd_prof_enable(0)
 
     elapsed = t1 - t0
-    units_per_transform = float(N + 1) ** 2 * nlat
-    value = 2.0 * units_per_transform * args.steps / elapsed
+    units = float(N + 1) ** 2 * nlat
     kt = {k: kernel_ms(lib, k) for k in ("shs_grid_kernel", "fft_c2r", "sha_kernel",
                                           "sha_reduce_kernel", "fft_r2c")}
     avg = {k: (v[0] / v[1] if v[1] else float("nan")) for k, v in kt.items()}
     shs_ms = avg["shs_grid_kernel"] + avg["fft_c2r"]
     sha_ms = avg["fft_r2c"] + avg["sha_kernel"] + avg["sha_reduce_kernel"]
-
-    # dominant kernel: FP64 recurrence; 8 flops per (n, m, ring pair)
+    # dominant kernel: FP64 recurrence, 8 flops per (n, m, ring pair)
     P = (N + 1) * (N + 2) // 2
-    ring_pairs = (nlat + 1) // 2
-    flops = 8.0 * P * ring_pairs
+    flops = 8.0 * P * ((nlat + 1) // 2)
     dom = max(("shs_grid_kernel", "sha_kernel"), key=lambda k: avg[k])
     achieved = flops / (avg[dom] * 1e-3) / 1e12
 
-    line = {
-        "metric": "(nmax+1)^2*nlat/s FP64 SHS+SHA round trip (GL grid, device-resident)",
-        "value": value,
-        "unit": "(nmax+1)^2*nlat/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded Kaula-like coefficients)",
-        "config": {"workload": "GL SHS+SHA round trip nmax=%d, grid %dx%d, r=R" % (N, nlat, nlon),
-                   "nmax": N, "nlat": nlat, "nlon": nlon, "parallelism": "single GPU"},
-        "shs_units_per_s": units_per_transform / (shs_ms * 1e-3),
-        "sha_units_per_s": units_per_transform / (sha_ms * 1e-3),
+    line = base_line("(nmax+1)^2*nlat/s FP64 SHS+SHA round trip (GL grid, device-resident)",
+                     2.0 * units * args.steps / elapsed, args.steps, args.warmup,
+                     1e3 * elapsed / args.steps,
+                     {"workload": "GL SHS+SHA round trip nmax=%d, grid %dx%d, r=R" % (N, nlat, nlon),
+                      "nmax": N, "nlat": nlat, "nlon": nlon, "parallelism": "single GPU"})
+    line.update({
+        "shs_units_per_s": units / (shs_ms * 1e-3),
+        "sha_units_per_s": units / (sha_ms * 1e-3),
         "kernel_avg_ms": avg,
         "roundtrip_rel_err": rt_err,
         "roofline": {"bound": "mfma", "compute": "vector FP64 (VALU)", "kernel": dom,
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "flops_per_launch": flops},
-    }
-    if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        from charm_amd.crd import PointGridGL
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": args.traffic,
+                     "flops_per_launch": flops,
+                     "frac_by_kernel": {k: flops / (avg[k] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
+                                        for k in ("shs_grid_kernel", "sha_kernel")}},
+    })
+    if args.check_rings:
+        # oracle on sampled rings: polar (X-number region), mid-latitude, equator
+        from oracle.oracle import Oracle
 
-        g = PointGridGL(N, R_EARTH)
-        v, dt = cpu_baseline(N, C, S, g.lat, g.w, g.lon, R_EARTH, GM, args.cpu_rows, threads)
+        grid.shs(cs, N, GM, R_EARTH, f, stream)
+        stream.synchronize()
+        f_np = f.numpy()
+        rings = np.array([0, 1, nlat // 7, nlat // 2, nlat - 1], dtype=np.int64)
+        pnt = grid.pnt
+        t0 = time.perf_counter()
+        ref = Oracle("d").shs_grid(N, C, S, GM, R_EARTH, pnt.lat, pnt.r, pnt.lon, gl=True,
+                                   rings=rings)
+        line["shs_rel_err_vs_oracle_sampled_rings"] = float(
+            np.abs(f_np[rings] - ref).max() / np.abs(ref).max())
+        line["oracle_check_s"] = time.perf_counter() - t0
+    del f, out
+    if with_sctr:
+        line.update(sctr_measure(N, cs, args.sctr_points, stream))
+    if with_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        pnt = grid.pnt
+        v, dt = cpu_baseline(N, C, S, pnt.lat, pnt.lon, R_EARTH, GM, args.cpu_rows, threads)
         line["cpu_baseline"] = {
-            "value": v, "unit": "(nmax+1)^2*nlat/s", "cores": threads, "kind": "port",
-            "sample": "SHS of %d of the %d GL rows at nmax=%d (Legendre sums + direct DFT per "
-                      "row), oracle in double, %.1f s wall" % (args.cpu_rows, nlat, N, dt)}
+            "value": v, "unit": UNIT, "cores": threads, "kind": "port",
+            "sample": "SHS of %d of the %d GL rows at nmax=%d (Legendre sums without the "
+                      "symmetry fold + direct DFT per row), oracle in double, %.1f s wall"
+                      % (args.cpu_rows, nlat, N, dt)}
+    return line
+
+
+def sctr_measure(N, cs, npoint, stream):
+    """SHS at `npoint` scattered points (device arrays), timed once after a
+    warm-up call."""
+    import charm_amd
+    from charm_amd.device import DevArray, shs_sctr
+    from charm_amd.workloads import GM, R_EARTH, scattered_points
+
+    lib = charm_amd.lib()
+    lat, lon, r = scattered_points(npoint, 12345)
+    d = [DevArray.from_numpy(a) for a in (lat, lon, r)]
+    out = DevArray(npoint)
+    shs_sctr(*d, cs, N, GM, R_EARTH, out, stream)
+    stream.synchronize()
+    lib.charm_amd_prof_reset()
+    lib.charm_amd_prof_enable(1)
+    t0 = time.perf_counter()
+    shs_sctr(*d, cs, N, GM, R_EARTH, out, stream)
+    stream.synchronize()
+    dt = time.perf_counter() - t0
+    lib.charm_amd_prof_enable(0)
+    kms, _ = kernel_ms(lib, "sctr_kernel")
+    P = (N + 1) * (N + 2) // 2
+    # per (n, m, point): 3 flops recurrence + 4 flops C/S accumulation
+    return {"sctr_npoint": npoint, "sctr_nmax": N, "sctr_s": dt,
+            "sctr_units_per_s": (N + 1) ** 2 * npoint / dt,
+            "sctr_kernel_ms": kms,
+            "sctr_kernel_frac_fp64_peak": 7.0 * P * npoint / (kms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
+
+
+def batch_bench(args):
+    import numpy as np
+
+    import charm_amd  # noqa: F401
+    from charm_amd import crd, sha, shc, shs
+    from charm_amd.device import SHA_GRID, SHS_GRID, Batch, DevArray, DeviceGrid, Stream
+    from charm_amd.workloads import GM, R_EARTH, kaula_coeffs
+
+    rng = np.random.default_rng(64)
+    degs = rng.integers(90, 2701, 64)
+    st = Stream()
+    reqs, host = [], []
+    units = 0.0
+    for i, N in enumerate(degs):
+        N = int(N)
+        kind = SHS_GRID if i % 2 == 0 else SHA_GRID
+        C, S = kaula_coeffs(N, 1000 + i)
+        g = DeviceGrid(N, R_EARTH)
+        cs = DevArray.from_numpy(np.stack([C, S], axis=1))
+        f = DevArray((g.nlat, g.nlon))
+        if kind == SHA_GRID:
+            g.shs(cs, N, GM, R_EARTH, f, st)
+            cs = DevArray(cs.shape)
+        reqs.append((kind, N, GM, R_EARTH, g, cs, f))
+        host.append((kind, N, C, S))
+        units += float(N + 1) ** 2 * g.nlat
+    st.synchronize()
+    print("# batch: %d requests set up" % len(reqs), file=sys.stderr, flush=True)
+
+    def per_call():
+        for kind, N, mu, R, g, cs, f in reqs:
+            if kind == SHS_GRID:
+                g.shs(cs, N, mu, R, f, st)
+            else:
+                g.sha(f, N, mu, R, cs, st)
+        st.synchronize()
+
+    def timed(fn, reps):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) / reps
+
+    def outputs():
+        return [(r[6].numpy() if r[0] == SHS_GRID else r[5].numpy()) for r in reqs]
+
+    t_call = timed(per_call, args.steps)
+    ref = outputs()
+    b_eager = Batch(reqs, use_graph=False)
+    t_batch = timed(lambda: (b_eager.run(st), st.synchronize()), args.steps)
+    b_graph = Batch(reqs, use_graph=True)
+    t_graph = timed(lambda: (b_graph.run(st), st.synchronize()), args.steps)
+    err = 0.0
+    for x, y in zip(ref, outputs()):
+        err = max(err, float(np.abs(y - x).max() / np.abs(x).max()))
+    print("# batch: timed per-call and batched", file=sys.stderr, flush=True)
+    # the plain drop-in API on host arrays (grid set-up and PCIe included)
+    t0 = time.perf_counter()
+    for kind, N, C, S in host:
+        g = crd.PointGridGL(N, R_EARTH)
+        m = shc.Shc.from_arrays(N, C, S, GM, R_EARTH)
+        fh = shs.point(g, m, N)
+        if kind == SHA_GRID:
+            sha.point(g, fh, N, GM, R_EARTH)
+    t_host = time.perf_counter() - t0
+    best = min(t_batch, t_graph)
+    line = base_line("(nmax+1)^2*nlat/s FP64 batched SHS/SHA, 64 requests nmax~U{90..2700}",
+                     units / best, args.steps, 1, 1e3 * best,
+                     {"workload": "64 GL requests (32 SHS, 32 SHA), nmax~U{90..2700} seed 64, "
+                                  "own coefficients and grid each", "requests": 64,
+                      "parallelism": "single GPU"})
+    line.update({"per_call_units_per_s": units / t_call, "batch_units_per_s": units / t_batch,
+                 "batch_graph_units_per_s": units / t_graph, "graph_used": b_graph.uses_graph,
+                 "speedup_vs_per_call_dev": t_call / best,
+                 "host_api_per_call_s": t_host,
+                 "speedup_vs_host_api_per_call": t_host / best,
+                 "batch_vs_alone_max_rel_diff": err})
+    return line
+
+
+def serve_bench(args):
+    import numpy as np
+
+    from charm_amd import shc
+    from charm_amd.serve import Server
+    from charm_amd.workloads import GM, R_EARTH, kaula_coeffs, scattered_points
+
+    NM = 2190
+    C, S = kaula_coeffs(NM, 2190)
+    srv = Server(200 * 2**30, args.inflight)
+    mid = srv.add_model(shc.Shc.from_arrays(NM, C, S, GM, R_EARTH))
+    rng = np.random.default_rng(2190)
+    nreq = args.requests
+    reqs, units = [], 0.0
+    for i in range(nreq):
+        u = rng.random()
+        if u < 0.6:
+            npt = int(round(10 ** rng.uniform(3, 6)))
+            reqs.append(("sctr", NM, scattered_points(npt, 5000 + i)))
+            units += (NM + 1) ** 2 * npt
+        elif u < 0.9:
+            Ng = int(rng.integers(180, 5401))
+            reqs.append(("shs", min(Ng, NM), Ng))
+            units += (min(Ng, NM) + 1) ** 2 * (Ng + 1)
+        else:
+            N = int(rng.integers(180, 2161))
+            reqs.append(("sha", N, N))
+            units += (N + 1) ** 2 * (N + 1)
+    sha_in = {i: srv.shs_grid(mid, n, a, R_EARTH).wait().copy()
+              for i, (kind, n, a) in enumerate(reqs) if kind == "sha"}
+    srv.shs_sctr(mid, NM, *scattered_points(1000, 1)).wait()
+    srv.shs_grid(mid, 180, 180, R_EARTH).wait()
+    print("# serve: %d requests prepared" % nreq, file=sys.stderr, flush=True)
+    gaps = rng.exponential(1.0 / args.rate, nreq)
+    tickets = []
+    t0 = time.perf_counter()
+    t_next = t0
+    for i, (kind, n, a) in enumerate(reqs):
+        t_next += gaps[i]
+        while time.perf_counter() < t_next:
+            time.sleep(0.0002)
+        if kind == "sctr":
+            tickets.append(srv.shs_sctr(mid, n, *a))
+        elif kind == "shs":
+            tickets.append(srv.shs_grid(mid, n, a, R_EARTH))
+        else:
+            tickets.append(srv.sha_grid(sha_in[i], n, R_EARTH, GM, R_EARTH))
+    for t in tickets:
+        t.wait()
+    wall = time.perf_counter() - t0
+    lat = np.array([t.latency_ms for t in tickets])
+    srv.close()
+    line = base_line("(nmax+1)^2*nlat/s FP64 served SHS/SHA mix, p50/p99 latency", units / wall,
+                     nreq, 2, 1e3 * wall / nreq,
+                     {"workload": "Poisson arrivals (rate %.1f/s) on a resident nmax=%d model: 60%% "
+                                  "scattered SHS 1e3..1e6 points (log-uniform), 30%% GL SHS grid "
+                                  "nmax~U{180..5400} (model truncated to min(grid, %d)), 10%% GL "
+                                  "SHA nmax~U{180..2160}" % (args.rate, NM, NM),
+                      "requests": nreq, "hbm_budget_GiB": 200, "max_inflight": args.inflight,
+                      "parallelism": "single GPU"})
+    line.update({"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                 "mean_ms": float(lat.mean()), "wall_s": wall,
+                 "note": "units of scattered requests count (nmax+1)^2 per point; results "
+                         "copied back to host memory inside the latency"})
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="gl2160",
+                    choices=("gl2160", "gl10800", "batch", "serve"))
+    ap.add_argument("--cpu-rows", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sctr", action="store_true")
+    ap.add_argument("--sctr-points", type=int, default=1000000)
+    ap.add_argument("--check-rings", action="store_true")
+    ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--inflight", type=int, default=4)
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="HBM bytes per launch of the dominant kernel (rocprofv3 FETCH_SIZE and "
+                         "WRITE_SIZE, scaled by a copy kernel) to report in the roofline object")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="before timing, copy this many MiB device-to-device once "
+                         "(a reference kernel of known traffic for counter scaling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (--gpus %d requested); "
+              "multi-GPU sharding is described in DESIGN.md, not implemented." % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "gl2160":
+        line = gl_roundtrip(args, 2160, not args.no_sctr, not args.no_cpu)
+    elif args.workload == "gl10800":
+        line = gl_roundtrip(args, 10800, False, False)
+    elif args.workload == "batch":
+        line = batch_bench(args)
+    else:
+        line = serve_bench(args)
     print(json.dumps(line), flush=True)
 
 
