@@ -7,12 +7,18 @@ ROCM=${ROCM_PATH:-/opt/rocm}
 mkdir -p "$HERE/_build"
 HIPFLAGS="--offload-arch=gfx950 -O3 -std=c++17 -fPIC -I$HERE/../../include -I$HERE \
  -Xclang -target-feature -Xclang -packed-fp32-ops -Wall -Wno-unused-function"
+pids=""
 for f in kernels point_api server; do
+  rm -f "$HERE/_build/$f.o"
   "$ROCM/bin/hipcc" $HIPFLAGS -c "$HERE/$f.hip" -o "$HERE/_build/$f.o" &
+  pids="$pids $!"
 done
+rm -f "$HERE/_build/host_api.o"
 g++ -O2 -std=c++17 -fPIC -Wall -I$HERE/../../include -I$HERE -c "$HERE/host_api.cpp" \
   -o "$HERE/_build/host_api.o" &
-wait
+pids="$pids $!"
+# every compile must succeed (a bare "wait" would hide a failed one)
+for p in $pids; do wait "$p"; done
 "$ROCM/bin/hipcc" --offload-arch=gfx950 -shared -fPIC -o "$OUT.tmp" "$HERE/_build/kernels.o" \
   "$HERE/_build/point_api.o" "$HERE/_build/server.o" "$HERE/_build/host_api.o" -L"$ROCM/lib" -lhipfft -lamdhip64 \
   -Wl,-rpath,"$ROCM/lib" -lpthread

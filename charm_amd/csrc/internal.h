@@ -49,9 +49,9 @@ struct ShsGridArgs {
     const double *ratio;  // unit == 0: R/r per row
     size_t rld;
     const unsigned char *mirror;  // [nring] 1 if row nlat-1-i is the mirror ring
-    const void *ab;    // (a_nm, b_nm) table, m-major, degree ab_nt
-    int ab_nt;
-    const void *cs;    // (C_nm, S_nm), m-major, degree nmax
+    const double *al;  // alpha_nm of the scaled recurrence, m-major, degree al_nt
+    int al_nt;
+    const void *cs;    // (C_nm s_nm, S_nm s_nm), m-major, degree nmax
     void *out;         // [row * ldo + m] double2
     size_t ldo;
     double mur;        // mu / R
@@ -72,9 +72,9 @@ struct ShaArgs {
     const double *cw;  // [nlat] quadrature weight times the global constant
     const void *spec;  // r2c spectrum of f, [row * lds + k] double2
     size_t lds;
-    const void *ab;
-    int ab_nt;
-    void *part;        // [tile][tri_len(nmax)] double2 partial sums
+    const double *al;  // alpha_nm, m-major, degree al_nt
+    int al_nt;
+    void *part;        // [tile][tri_len(nmax)] double2 partial sums (before s_nm)
     int k;             // ring pairs per lane (2..4)
 };
 
@@ -90,14 +90,19 @@ struct SctrArgs {
     const double *lon;
     const double *r;
     const double *dm;     // [nmax+1]
-    const void *ab;
-    int ab_nt;
-    const void *cs;
+    const double *al;     // alpha_nm, m-major, degree al_nt
+    int al_nt;
+    const void *cs;       // (C_nm s_nm, S_nm s_nm), m-major, degree nmax
     double *part;         // [chunk * npoint + i]
+    int k;                // points per lane (1, 2, 4)
 };
 
 // Kernel launchers (kernels.hip).  All return hipError_t as int.
-int launch_ab_table(int nt, const double *r_dev, const double *ri_dev, void *ab_dev,
+// alpha_nm and s_nm of degree nt; ab_tmp holds tri_len(nt) double2 of scratch
+int launch_leg_tables(int nt, const double *r_dev, const double *ri_dev, void *ab_tmp,
+                      double *al_dev, double *sn_dev, void *stream);
+// out = (C_nm s_nm, S_nm s_nm) for interleaved pairs cs of degree nmax
+int launch_scale_cs(int nmax, const void *cs_dev, const double *sn_dev, int nt, void *out_dev,
                     void *stream);
 int launch_seeds(int nring, const double *u_dev, const double *dm_dev, int nmax, size_t sld,
                  double *sx_dev, int *six_dev, void *stream);
@@ -107,12 +112,16 @@ int launch_shs_grid(const ShsGridArgs &a, void *stream);
 int launch_lon_direct(int nlat, int nmax, size_t nlon, const double *lon_dev, const void *lc_dev,
                       size_t ldl, double mur, double *f_dev, void *stream);
 int launch_sha(const ShaArgs &a, void *stream);
-int launch_sha_reduce(int nmax, int ntile, const void *part_dev, int nmax_out,
-                      const double *scale_dev, double *c_dev, double *s_dev, size_t stride,
-                      void *stream);
+int launch_sha_reduce(int nmax, int ntile, const void *part_dev, const double *sn_dev, int nt,
+                      int nmax_out, const double *scale_dev, double *c_dev, double *s_dev,
+                      size_t stride, void *stream);
 int launch_sctr(const SctrArgs &a, void *stream);
-int launch_sctr_reduce(size_t npoint, int nchunk, const double *part_dev, double mur,
-                       double *f_dev, void *stream);
+int launch_sctr_reduce(size_t npoint, int nchunk, const double *part_dev, const int *perm_dev,
+                       double mur, double *f_dev, void *stream);
+int launch_sort_points(size_t n, const double *lat_dev, const double *lon_dev, const double *r_dev,
+                       int *perm_dev, double *t_dev, double *u_dev, double *lon_out_dev,
+                       double *r_out_dev, void *stream);
+int sctr_points_per_block(int k);
 int launch_pack_cs(const double *c_dev, const double *s_dev, size_t n, void *cs_dev,
                    void *stream);
 int launch_sincos(size_t n, const double *lat_dev, double *t_dev, double *u_dev, void *stream);

@@ -1,18 +1,33 @@
 // HIP kernels for point synthesis (SHS) and analysis (SHA) on gfx950.
 //
 // Mapping.  Lanes run over latitude rings (or scattered points), work-groups
-// over orders m; the degree recurrence runs sequentially in each lane, with
-// the recurrence coefficients (a_nm, b_nm) and the model coefficients
-// (C_nm, S_nm) as wave-uniform scalar loads.  Equatorially symmetric rings are
-// folded: one Legendre evaluation per ring pair, with even and odd (n + m)
-// sums kept apart so that P_nm(-t) = (-1)^(n+m) P_nm(t) gives the mirror ring.
+// over orders m; the degree recurrence runs sequentially in each lane.  The
+// per-degree column data of an order (the recurrence factor alpha_nm and, for
+// synthesis, the scaled coefficients) are wave-uniform: each wave streams its
+// column through a 64-entry LDS window that is refilled from registers loaded
+// one window ahead, so the global loads never sit on the critical path.
+// Equatorially symmetric rings are folded: one Legendre evaluation per ring
+// pair, with even and odd (n + m) sums kept apart so that
+// P_nm(-t) = (-1)^(n+m) P_nm(t) gives the mirror ring.
 //
-// X-numbers (extended exponent, radix 2^960) carry the sectorial seeds and the
-// first tesseral terms of each column exactly as in CHarm
-// (src/leg/leg_func_xnum.h:299-443); once both terms of a lane have a zero
-// exponent that lane continues in ordinary doubles.  A wave switches to the
-// tight all-double loop when every lane has switched.
+// Recurrence.  CHarm runs P_n = (a_n t) P_{n-1} - b_n P_{n-2}
+// (src/leg/leg_func_xnum.h:29-31).  Here the same recurrence runs on
+// Q_n = P_n / s_n with s_m = s_{m+1} = 1 and s_n = b_n s_{n-2}:
+//   Q_n = (alpha_n Q_{n-1}) t - Q_{n-2},   alpha_n = a_n s_{n-1} / s_n,
+// two FP64 instructions per term instead of three, built from CHarm's own
+// (double) a_n, b_n.  Synthesis uses coefficients pre-multiplied by s_n,
+// analysis multiplies its sums by s_n; s_n stays within [0.1, 1.2].
+//
+// X-numbers.  CHarm carries the sectorial seeds and the first terms of each
+// column as X-numbers (mantissa in [2^-480, 2^480), exponent in units of
+// 2^960, src/leg/leg_func_xnum.h:299-443) and continues in doubles once both
+// terms of a lane are representable.  Here a lane carries scaled mantissas
+// with its own binary exponent (refreshed every 8 degrees); a wave runs the
+// plain loop once no lane has an exponent left.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
 
 #include "internal.h"
 
@@ -60,17 +75,12 @@ __device__ __forceinline__ bool wave_all(bool p)
     return __ballot(!p) == 0;
 }
 
-// Column state of one lane: P_{n-1} = x 2^e and P_{n-2} = y 2^e.
-//
-// CHarm carries the first terms of a column as X-numbers (mantissa in
-// [2^-480, 2^480) and exponent in units of 2^960, src/leg/leg_func_xnum.h)
-// and switches to plain doubles once both terms have a zero exponent, i.e.
-// once both true values lie in [2^-480, 2^480).  Scaling by a power of two is
-// exact, so the same true values come out of the plain recurrence on scaled
-// mantissas with a separate binary exponent e that is refreshed only every
-// 8 degrees; a lane drops e (e = 0) at the same point CHarm leaves
-// X-numbers.  P_n itself is ldexp(z, e), which rounds (also into the
-// subnormal range, and to 0 far below it) exactly as CHarm's x * BIG^ix.
+// Column state of one lane: Q_{n-1} = x 2^e and Q_{n-2} = y 2^e.  Scaling by
+// a power of two is exact, so the recurrence runs on the mantissas; the
+// exponent is refreshed every 8 degrees and dropped (e = 0) once both true
+// values are at least 2^-480, where CHarm leaves X-numbers.  A term is
+// ldexp(z, e), which rounds (into the subnormal range, and to 0 far below it)
+// as CHarm's x * BIG^ix does.
 __device__ __forceinline__ void lane_renorm(double &x, double &y, int &e)
 {
     if (e != 0) {
@@ -89,14 +99,15 @@ __device__ __forceinline__ void lane_renorm(double &x, double &y, int &e)
     }
 }
 
-// True |P| stays below 2^-1100 for the next 8 degrees (|a_nm t| < 2^7), so
-// these terms are exactly zero in double precision.
+// The true values stay below 2^-1100 for the next 8 degrees (each degree
+// grows them by at most |alpha t| + 1 < 2^9), so every term of the tile is
+// exactly zero in double precision.
 __device__ __forceinline__ bool lane_deep(double x, double y, int e)
 {
     int ex, ey;
     (void)frexp(x, &ex);
     (void)frexp(y, &ey);
-    return e != 0 && e + max(ex, ey) < -1160;
+    return e != 0 && e + max(ex, ey) < -1180;
 }
 
 __device__ __forceinline__ int uniform(int v)
@@ -104,11 +115,66 @@ __device__ __forceinline__ int uniform(int v)
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+// Orders the LDS accesses of one wave around a refill of its window (LDS
+// instructions of a wave execute in issue order).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per-degree column data of one order for one wave: alpha_n and (CS) the
+// scaled (C_n, S_n), in 64-degree windows.  Lane l loads degree n0 + 64 + l of
+// the next window while the wave works on the current one.
+template <bool CS>
+struct ColStage {
+    double *al;          // LDS window [64]
+    double2 *cs;         // LDS window [64] (CS)
+    const double *gal;   // entry n of the column
+    const double2 *gcs;  // entry n of the column (CS)
+    int nlast;           // last degree of the column
+    int n0;              // first degree of the window in LDS
+    int lane;
+    double pal;
+    double2 pcs;
+
+    __device__ __forceinline__ void fetch(int nb)
+    {
+        const int n = nb + lane;
+        const bool ok = n <= nlast;
+        pal = ok ? gal[n] : 0.0;
+        if (CS) pcs = ok ? gcs[n] : make_double2(0.0, 0.0);
+    }
+    __device__ __forceinline__ void store()
+    {
+        al[lane] = pal;
+        if (CS) cs[lane] = pcs;
+        wave_lds_sync();
+    }
+    __device__ __forceinline__ void start(int nb)
+    {
+        n0 = nb;
+        fetch(nb);
+        store();
+        fetch(nb + 64);
+    }
+    __device__ __forceinline__ void next()
+    {
+        wave_lds_sync();
+        n0 += 64;
+        store();
+        fetch(n0 + 64);
+    }
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// (a_nm, b_nm) table, m-major, degree nt (src/leg/leg_func_anm_bnm.c:14-65)
+// Recurrence tables of degree nt, m-major (entry n of column m at
+// dcol_off(nt, m) + n - m).
 // ---------------------------------------------------------------------------
+// (a_nm, b_nm) as CHarm computes them (src/leg/leg_func_anm_bnm.c:14-65)
 __global__ void ab_table_kernel(int nt, const double *__restrict__ r, const double *__restrict__ ri,
                                 double2 *__restrict__ ab)
 {
@@ -124,6 +190,47 @@ __global__ void ab_table_kernel(int nt, const double *__restrict__ r, const doub
             b = r[n - m - 1] * r[n + m - 1] * ri[2 * n - 3] * w;
         }
         col[n] = make_double2(a, b);
+    }
+}
+
+// alpha_nm and s_nm of the scaled recurrence, one thread per column
+__global__ void alpha_table_kernel(int nt, const double2 *__restrict__ ab, double *__restrict__ al,
+                                   double *__restrict__ sn)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m > nt) return;
+    const size_t o = dcol_off(nt, m) - m;
+    const double2 *c = ab + o;
+    double *A = al + o;
+    double *S = sn + o;
+    A[m] = 0.0;
+    S[m] = 1.0;
+    if (m + 1 > nt) return;
+    A[m + 1] = c[m + 1].x;
+    S[m + 1] = 1.0;
+    double s2 = 1.0, s1 = 1.0;
+    for (int n = m + 2; n <= nt; n++) {
+        const double2 v = c[n];
+        const double s = v.y * s2;
+        A[n] = (v.x * s1) / s;
+        S[n] = s;
+        s2 = s1;
+        s1 = s;
+    }
+}
+
+// (C_nm s_nm, S_nm s_nm) for a model of degree nmax
+__global__ void scale_cs_kernel(int nmax, const double2 *__restrict__ cs, const double *__restrict__ sn,
+                                int nt, double2 *__restrict__ out)
+{
+    const int m = blockIdx.y;
+    const double2 *src = cs + dcol_off(nmax, m) - m;
+    double2 *dst = out + dcol_off(nmax, m) - m;
+    const double *s = sn + dcol_off(nt, m) - m;
+    for (int n = m + blockIdx.x * blockDim.x + threadIdx.x; n <= nmax; n += gridDim.x * blockDim.x) {
+        const double2 v = src[n];
+        const double f = s[n];
+        dst[n] = make_double2(v.x * f, v.y * f);
     }
 }
 
@@ -173,9 +280,16 @@ __global__ void rpows_kernel(int nrow, const double *__restrict__ ratio, int kma
 // Blocks are numbered so that one XCD (blockIdx mod 8) owns every 8th order
 // quad: all tiles of a column share that XCD's L2.
 // ---------------------------------------------------------------------------
+template <int K>
+struct ShsSmem {
+    double2 stage[2][64 * K][4];
+    double al[4][64];
+    double2 cs[4][64];
+};
+
 template <int K, bool UNIT>
 __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, int quad,
-                                              double2 (*stage)[64 * K][4])
+                                              ShsSmem<K> &sm)
 {
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
@@ -199,7 +313,6 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
     for (int k = 0; k < K; k++) aN[k] = bN[k] = aS[k] = bS[k] = 0.0;
 
     if (m <= nmax) {
-        const double2 *__restrict__ abm = (const double2 *)A.ab + dcol_off(A.ab_nt, m) - m;
         const double2 *__restrict__ csm = (const double2 *)A.cs + dcol_off(nmax, m) - m;
         double x[K], y[K];
         int e[K];
@@ -209,8 +322,6 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
 #pragma unroll
         for (int k = 0; k < K; k++) {
             aE[k] = aO[k] = bE[k] = bO[k] = 0.0;
-            x[k] = y[k] = 0.0;
-            e[k] = 0;
             rpN[k] = rpS[k] = qN[k] = qS[k] = 1.0;
             if (!UNIT && valid[k]) {
                 const int rs = A.mirror[ring[k]] ? A.nlat - 1 - ring[k] : ring[k];
@@ -251,113 +362,112 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
         }                                                                  \
     } while (0)
 
-        int n;
-        if (m == 0) {
-            // zonal: P00 = 1, P10 = sqrt(3) t, then the ordinary recurrence
-            const double2 c0 = csm[0];
-#pragma unroll
-            for (int k = 0; k < K; k++) SHS_ACC(k, 1.0, c0, 0);
-            if (nmax >= 1) {
-                const double2 c1 = csm[1];
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const double p1 = ROOT3 * t[k];
-                    SHS_ACC(k, p1, c1, 1);
-                    x[k] = p1;
-                    y[k] = 1.0;
-                }
-            }
-            n = 2;
-        } else {
-            // sectorial seed P_mm (X-number from the seed table) and
-            // semi-sectorial P_{m+1,m} = (a t) P_mm
-            const double2 cmm = csm[m];
-            const double2 ab1 = (m < nmax) ? abm[m + 1] : make_double2(0.0, 0.0);
-            const double2 c1 = (m < nmax) ? csm[m + 1] : make_double2(0.0, 0.0);
+        // degree m: P_00 = 1 or the sectorial seed P_mm (X-number)
+        {
+            const double2 cm = csm[m];
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                const double xs = valid[k] ? A.sx[(size_t)(m - 1) * A.sld + ring[k]] : 0.0;
-                e[k] = valid[k] ? 960 * A.six[(size_t)(m - 1) * A.sld + ring[k]] : 0;
-                SHS_ACC(k, ldexp(xs, e[k]), cmm, 0);
-                if (m < nmax) {
-                    y[k] = xs;
-                    x[k] = (ab1.x * t[k]) * xs;
-                    SHS_ACC(k, ldexp(x[k], e[k]), c1, 1);
-                    lane_renorm(x[k], y[k], e[k]);
+                if (m == 0) {
+                    x[k] = valid[k] ? 1.0 : 0.0;
+                    e[k] = 0;
+                } else {
+                    x[k] = valid[k] ? A.sx[(size_t)(m - 1) * A.sld + ring[k]] : 0.0;
+                    e[k] = valid[k] ? 960 * A.six[(size_t)(m - 1) * A.sld + ring[k]] : 0;
                 }
+                y[k] = 0.0;
+                SHS_ACC(k, ldexp(x[k], e[k]), cm, 0);
             }
-            n = m + 2;
         }
 
-        // Tiles of 8 degrees; (n + m) is even at every tile start.  While a
-        // lane of the wave still carries an exponent, tiles convert each term
-        // with ldexp (or, when every lane is far below the double range, run
-        // the bare recurrence); afterwards the wave stays in the tight loop.
-        for (;;) {
-            if (n + 7 > nmax) break;
-            bool anyx = false, deep = true;
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                anyx = anyx || (e[k] != 0);
-                deep = deep && (!valid[k] || lane_deep(x[k], y[k], e[k]));
-            }
-            if (!wave_any(anyx)) break;
-            if (wave_all(deep)) {
-#pragma unroll
-                for (int j = 0; j < 8; j++) {
-                    const double2 abj = abm[n + j];
-#pragma unroll
-                    for (int k = 0; k < K; k++) {
-                        const double z = fma(abj.x * t[k], x[k], -(abj.y * y[k]));
-                        y[k] = x[k];
-                        x[k] = z;
-                        SHS_SKIP(k);
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < 8; j++) {
-                    const double2 abj = abm[n + j];
-                    const double2 cj = csm[n + j];
-#pragma unroll
-                    for (int k = 0; k < K; k++) {
-                        const double z = fma(abj.x * t[k], x[k], -(abj.y * y[k]));
-                        y[k] = x[k];
-                        x[k] = z;
-                        SHS_ACC(k, ldexp(z, e[k]), cj, j & 1);
-                    }
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < K; k++) lane_renorm(x[k], y[k], e[k]);
-            n += 8;
-        }
-        for (; n + 7 <= nmax; n += 8) {
-#pragma unroll
-            for (int j = 0; j < 8; j += 2) {
-                const double2 ab0 = abm[n + j], ab1 = abm[n + j + 1];
-                const double2 c0 = csm[n + j], c1 = csm[n + j + 1];
+        // degrees m+1..nmax in tiles of 8; (n + m) is odd at every tile start
+        if (m < nmax) {
+            ColStage<true> st;
+            st.al = sm.al[wave];
+            st.cs = sm.cs[wave];
+            st.gal = A.al + dcol_off(A.al_nt, m) - m;
+            st.gcs = csm;
+            st.nlast = nmax;
+            st.lane = lane;
+            st.start(m + 1);
+            int n = m + 1;
+            // phase 1: tiles while a lane of the wave still carries an exponent
+            for (; n + 7 <= nmax; n += 8) {
+                if (n - st.n0 == 64) st.next();
+                const int o = n - st.n0;
+                bool anyx = false, deep = true;
 #pragma unroll
                 for (int k = 0; k < K; k++) {
-                    const double z = fma(ab0.x * t[k], x[k], -(ab0.y * y[k]));
-                    SHS_ACC(k, z, c0, 0);
-                    const double w = fma(ab1.x * t[k], z, -(ab1.y * x[k]));
-                    SHS_ACC(k, w, c1, 1);
-                    y[k] = z;
-                    x[k] = w;
+                    anyx = anyx || (e[k] != 0);
+                    deep = deep && (!valid[k] || lane_deep(x[k], y[k], e[k]));
+                }
+                if (!wave_any(anyx)) break;
+                if (wave_all(deep)) {
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const double al = st.al[o + j];
+#pragma unroll
+                        for (int k = 0; k < K; k++) {
+                            const double z = fma(al * x[k], t[k], -y[k]);
+                            y[k] = x[k];
+                            x[k] = z;
+                            SHS_SKIP(k);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const double al = st.al[o + j];
+                        const double2 c = st.cs[o + j];
+#pragma unroll
+                        for (int k = 0; k < K; k++) {
+                            const double z = fma(al * x[k], t[k], -y[k]);
+                            y[k] = x[k];
+                            x[k] = z;
+                            SHS_ACC(k, ldexp(z, e[k]), c, (j & 1) ^ 1);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < K; k++) lane_renorm(x[k], y[k], e[k]);
+            }
+            // phase 2: every lane in plain doubles
+            for (; n + 7 <= nmax; n += 8) {
+                if (n - st.n0 == 64) st.next();
+                const int o = n - st.n0;
+#pragma unroll
+                for (int j = 0; j < 8; j += 2) {
+                    const double2 a2 = *(const double2 *)(st.al + o + j);
+                    const double2 c0 = st.cs[o + j], c1 = st.cs[o + j + 1];
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        const double z = fma(a2.x * x[k], t[k], -y[k]);
+                        SHS_ACC(k, z, c0, 1);
+                        const double w = fma(a2.y * z, t[k], -x[k]);
+                        SHS_ACC(k, w, c1, 0);
+                        y[k] = z;
+                        x[k] = w;
+                    }
                 }
             }
-        }
-        for (; n <= nmax; n++) {
-            const double2 abj = abm[n];
-            const double2 cj = csm[n];
-            const int par = (n + m) & 1;
+            // last, partial tile
+            if (n <= nmax) {
+                if (n - st.n0 == 64) st.next();
+                const int o = n - st.n0;
+                const int cnt = nmax - n + 1;
 #pragma unroll
-            for (int k = 0; k < K; k++) {
-                const double z = fma(abj.x * t[k], x[k], -(abj.y * y[k]));
-                y[k] = x[k];
-                x[k] = z;
-                SHS_ACC(k, ldexp(z, e[k]), cj, par);
+                for (int j = 0; j < 8; j++) {
+                    if (j < cnt) {
+                        const double al = st.al[o + j];
+                        const double2 c = st.cs[o + j];
+#pragma unroll
+                        for (int k = 0; k < K; k++) {
+                            const double z = fma(al * x[k], t[k], -y[k]);
+                            y[k] = x[k];
+                            x[k] = z;
+                            SHS_ACC(k, ldexp(z, e[k]), c, (j & 1) ^ 1);
+                        }
+                    }
+                }
             }
         }
 #undef SHS_ACC
@@ -389,13 +499,13 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
     const double sb = A.fft_layout ? -sc : sc;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        stage[0][k * 64 + lane][wave] = make_double2(sc * aN[k], sb * bN[k]);
-        stage[1][k * 64 + lane][wave] = make_double2(sc * aS[k], sb * bS[k]);
+        sm.stage[0][k * 64 + lane][wave] = make_double2(sc * aN[k], sb * bN[k]);
+        sm.stage[1][k * 64 + lane][wave] = make_double2(sc * aS[k], sb * bS[k]);
     }
     __syncthreads();
     double2 *out = (double2 *)A.out;
-    for (int e = threadIdx.x; e < 2 * 64 * K * 4; e += 256) {
-        const int col = e & 3, rowsel = e >> 2;
+    for (int ei = threadIdx.x; ei < 2 * 64 * K * 4; ei += 256) {
+        const int col = ei & 3, rowsel = ei >> 2;
         const int ns = rowsel / (64 * K), rl = rowsel % (64 * K);
         const int i = tile * 64 * K + rl;
         const int mm = m0 + col;
@@ -405,16 +515,16 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
             if (!A.mirror[i]) continue;
             row = A.nlat - 1 - i;
         }
-        out[(size_t)row * A.ldo + mm] = stage[ns][rl][col];
+        out[(size_t)row * A.ldo + mm] = sm.stage[ns][rl][col];
     }
 }
 
 template <int K, bool UNIT>
 __global__ __launch_bounds__(256) void shs_grid_kernel(ShsGridArgs A)
 {
-    __shared__ double2 stage[2][64 * K][4];
+    __shared__ __align__(16) ShsSmem<K> sm;
     const int b = blockIdx.x;
-    shs_grid_item<K, UNIT>(A, (b >> 3) % A.ntile, ((b >> 3) / A.ntile) * 8 + (b & 7), stage);
+    shs_grid_item<K, UNIT>(A, (b >> 3) % A.ntile, ((b >> 3) / A.ntile) * 8 + (b & 7), sm);
 }
 
 // Persistent form: blocks pull (request, ring tile, order quad) items from a
@@ -424,20 +534,21 @@ __global__ __launch_bounds__(256) void shs_batch_kernel(const ShsGridArgs *__res
                                                         const int2 *__restrict__ items, int nitems,
                                                         int *counter)
 {
-    __shared__ double2 stage[2][64 * K][4];
+    __shared__ __align__(16) ShsSmem<K> sm;
     __shared__ int cur;
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) cur = atomicAdd(counter, 1);
         __syncthreads();
-        const int it = cur;
+        const int it = uniform(cur);
         if (it >= nitems) break;
         const int2 w = items[it];
-        const ShsGridArgs A = reqs[w.x];
+        const ShsGridArgs A = reqs[uniform(w.x)];
+        const int wy = uniform(w.y);
         if (A.unit)
-            shs_grid_item<K, true>(A, w.y >> 16, w.y & 0xFFFF, stage);
+            shs_grid_item<K, true>(A, wy >> 16, wy & 0xFFFF, sm);
         else
-            shs_grid_item<K, false>(A, w.y >> 16, w.y & 0xFFFF, stage);
+            shs_grid_item<K, false>(A, wy >> 16, wy & 0xFFFF, sm);
     }
 }
 
@@ -471,10 +582,13 @@ __global__ void lon_direct_kernel(int nlat, int nmax, size_t nlon, const double 
 // Block = 4 waves = orders m0..m0+3 for one tile of 64*K ring pairs.  Each lane
 // sums its K ring pairs in registers; every 8 degrees the 64 lanes' partial
 // sums are transposed through LDS and reduced in a fixed order.  One partial
-// per (tile, n, m) goes to global memory; sha_reduce_kernel adds the tiles.
+// per (tile, n, m) goes to global memory; sha_reduce_kernel adds the tiles and
+// applies s_nm.
 // ---------------------------------------------------------------------------
 namespace {
-constexpr int RED_LD = 66;  // padded row (16-byte aligned) of the per-wave transpose buffer
+// Row pitch (doubles) of the per-wave transpose buffer: a multiple of 8 but
+// not of 16, so that the 16-byte reads of 16 lanes cover all 64 banks.
+constexpr int RED_LD = 72;
 
 __device__ __forceinline__ double dpp_quad_xor1(double v)
 {
@@ -496,8 +610,9 @@ __device__ __forceinline__ double dpp_quad_xor2(double v)
 // lanes of a wave, software-pipelined across tiles: a tile's values are
 // written to LDS after the tile is computed, read back at the start of the
 // next tile and summed after that tile's arithmetic, so the LDS latency hides
-// behind the recurrence.  Lane l sums value l/4 over 16 lanes; the 4 lanes of
-// a quad then combine through DPP.  The order of additions is fixed.
+// behind the recurrence.  Lane l sums value l/4 over the 16 columns
+// 2q + 8i, 2q + 8i + 1 (q = l mod 4, i = 0..7); the 4 lanes of a quad then
+// combine through DPP.  The order of additions is fixed.
 struct WaveReducer {
     double *red;      // [16][RED_LD] doubles, this wave's slice of LDS
     double *dst;      // destination of the pending tile (nullptr: none)
@@ -509,10 +624,10 @@ struct WaveReducer {
     {
         if (dst) {
             const int j = lane >> 2, q = lane & 3;
-            const double2 *row = (const double2 *)(red + j * RED_LD + q * 16);
+            const double *row = red + j * RED_LD + 2 * q;
 #pragma unroll
             for (int i = 0; i < 8; i++) {
-                const double2 w = row[i];
+                const double2 w = *(const double2 *)(row + 8 * i);
                 r[2 * i] = w.x;
                 r[2 * i + 1] = w.y;
             }
@@ -531,10 +646,16 @@ struct WaveReducer {
             dst = nullptr;
         }
     }
-    __device__ __forceinline__ void post(const double v[16], int c, double *d)
+    // values of degree j of the tile being computed (written as they are
+    // formed; the pending tile's reads were issued before and execute first)
+    __device__ __forceinline__ void put(int j, double c, double s)
     {
-#pragma unroll
-        for (int j = 0; j < 16; j++) red[j * RED_LD + lane] = v[j];
+        red[(2 * j) * RED_LD + lane] = c;
+        red[(2 * j + 1) * RED_LD + lane] = s;
+    }
+    // the tile just written becomes the pending one
+    __device__ __forceinline__ void commit(int c, double *d)
+    {
         dst = d;
         cnt = c;
     }
@@ -549,7 +670,13 @@ struct ShaLds {
 };
 
 template <int K>
-__device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, double *lds)
+struct ShaSmem {
+    double buf[ShaLds<K>::SIZE];
+    double al[4][64];
+};
+
+template <int K>
+__device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, ShaSmem<K> &sm)
 {
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
@@ -557,13 +684,14 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, d
     if (m0 > A.nmax) return;
     const int m = m0 + wave;
     const int nmax = A.nmax;
+    double *lds = sm.buf;
 
     // stage the weighted spectra A = cw Re F, B = -cw Im F of the tile's rows
     // for orders m0..m0+3 (64-byte runs per row)
     double2 *stage = (double2 *)lds;
     const double2 *spec = (const double2 *)A.spec;
-    for (int e = threadIdx.x; e < 2 * 64 * K * 4; e += 256) {
-        const int col = e & 3, rowsel = e >> 2;
+    for (int ei = threadIdx.x; ei < 2 * 64 * K * 4; ei += 256) {
+        const int col = ei & 3, rowsel = ei >> 2;
         const int ns = rowsel / (64 * K), rl = rowsel % (64 * K);
         const int i = tile * 64 * K + rl;
         const int mm = m0 + col;
@@ -577,7 +705,7 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, d
                 v = make_double2(cw * F.x, -cw * F.y);
             }
         }
-        stage[e] = v;
+        stage[ei] = v;
     }
     __syncthreads();
 
@@ -600,7 +728,6 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, d
     __syncthreads();
     if (m > nmax) return;
 
-    const double2 *__restrict__ abm = (const double2 *)A.ab + dcol_off(A.ab_nt, m) - m;
     double *dst = (double *)((double2 *)A.part + (size_t)tile * tri_len_dev(nmax) + dcol_off(nmax, m) - m);
     WaveReducer rd;
     rd.red = lds + wave * 16 * RED_LD;
@@ -610,51 +737,45 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, d
 
     double x[K], y[K];
     int e[K];
-    double v[16];
-#pragma unroll
-    for (int j = 0; j < 16; j++) v[j] = 0.0;
+    double v0 = 0.0, v1 = 0.0;
+
+    // degree m: P_00 = 1 or the sectorial seed P_mm (X-number)
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        x[k] = y[k] = 0.0;
-        e[k] = 0;
-    }
-
-    // degrees m and m + 1
-    if (m == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            v[0] += ampC[k];
-            if (nmax >= 1) {
-                const double p1 = ROOT3 * t[k];
-                v[2] = fma(p1, ammC[k], v[2]);
-                x[k] = p1;
-                y[k] = 1.0;
-            }
-        }
-    } else {
-        const double2 ab1 = (m < nmax) ? abm[m + 1] : make_double2(0.0, 0.0);
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            const double xs = valid[k] ? A.sx[(size_t)(m - 1) * A.sld + ring[k]] : 0.0;
+        if (m == 0) {
+            x[k] = valid[k] ? 1.0 : 0.0;
+            e[k] = 0;
+        } else {
+            x[k] = valid[k] ? A.sx[(size_t)(m - 1) * A.sld + ring[k]] : 0.0;
             e[k] = valid[k] ? 960 * A.six[(size_t)(m - 1) * A.sld + ring[k]] : 0;
-            const double p0 = ldexp(xs, e[k]);
-            v[0] = fma(p0, ampC[k], v[0]);
-            v[1] = fma(p0, ampS[k], v[1]);
-            if (m < nmax) {
-                y[k] = xs;
-                x[k] = (ab1.x * t[k]) * xs;
-                const double p1 = ldexp(x[k], e[k]);
-                v[2] = fma(p1, ammC[k], v[2]);
-                v[3] = fma(p1, ammS[k], v[3]);
-                lane_renorm(x[k], y[k], e[k]);
-            }
         }
+        y[k] = 0.0;
+        const double p0 = ldexp(x[k], e[k]);
+        v0 = fma(p0, ampC[k], v0);
+        v1 = fma(p0, ampS[k], v1);
     }
-    rd.post(v, (m < nmax) ? 2 : 1, dst + 2 * m);
+    rd.put(0, v0, v1);
+    rd.commit(1, dst + 2 * m);
+    if (m == nmax) {
+        rd.begin();
+        rd.finish();
+        return;
+    }
 
-    int n = m + 2;
-    for (;;) {
-        if (n + 7 > nmax) break;
+    // degrees m+1..nmax in tiles of 8; (n + m) is odd at every tile start
+    ColStage<false> st;
+    st.al = sm.al[wave];
+    st.cs = nullptr;
+    st.gal = A.al + dcol_off(A.al_nt, m) - m;
+    st.gcs = nullptr;
+    st.nlast = nmax;
+    st.lane = lane;
+    st.start(m + 1);
+    int n = m + 1;
+    // phase 1: tiles while a lane of the wave still carries an exponent
+    for (; n + 7 <= nmax; n += 8) {
+        if (n - st.n0 == 64) st.next();
+        const int o = n - st.n0;
         bool anyx = false, deep = true;
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -667,10 +788,10 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, d
             // every term of the tile is exactly zero
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                const double2 abj = abm[n + j];
+                const double al = st.al[o + j];
 #pragma unroll
                 for (int k = 0; k < K; k++) {
-                    const double z = fma(abj.x * t[k], x[k], -(abj.y * y[k]));
+                    const double z = fma(al * x[k], t[k], -y[k]);
                     y[k] = x[k];
                     x[k] = z;
                 }
@@ -680,75 +801,76 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, d
         } else {
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                const double2 abj = abm[n + j];
+                const double al = st.al[o + j];
                 double c = 0.0, sv = 0.0;
 #pragma unroll
                 for (int k = 0; k < K; k++) {
-                    const double z = fma(abj.x * t[k], x[k], -(abj.y * y[k]));
+                    const double z = fma(al * x[k], t[k], -y[k]);
                     y[k] = x[k];
                     x[k] = z;
                     const double p = ldexp(z, e[k]);
-                    c = fma(p, (j & 1) ? ammC[k] : ampC[k], c);
-                    sv = fma(p, (j & 1) ? ammS[k] : ampS[k], sv);
+                    c = fma(p, (j & 1) ? ampC[k] : ammC[k], c);
+                    sv = fma(p, (j & 1) ? ampS[k] : ammS[k], sv);
                 }
-                v[2 * j] = c;
-                v[2 * j + 1] = sv;
+                rd.put(j, c, sv);
             }
             rd.finish();
-            rd.post(v, 8, dst + 2 * n);
+            rd.commit(8, dst + 2 * n);
         }
 #pragma unroll
         for (int k = 0; k < K; k++) lane_renorm(x[k], y[k], e[k]);
-        n += 8;
     }
+    // phase 2: every lane in plain doubles
     for (; n + 7 <= nmax; n += 8) {
+        if (n - st.n0 == 64) st.next();
+        const int o = n - st.n0;
         rd.begin();
 #pragma unroll
         for (int j = 0; j < 8; j += 2) {
-            const double2 ab0 = abm[n + j], ab1 = abm[n + j + 1];
+            const double2 a2 = *(const double2 *)(st.al + o + j);
             double c0 = 0.0, s0 = 0.0, c1 = 0.0, s1 = 0.0;
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                const double z = fma(ab0.x * t[k], x[k], -(ab0.y * y[k]));
-                c0 = fma(z, ampC[k], c0);
-                s0 = fma(z, ampS[k], s0);
-                const double w = fma(ab1.x * t[k], z, -(ab1.y * x[k]));
-                c1 = fma(w, ammC[k], c1);
-                s1 = fma(w, ammS[k], s1);
+                const double z = fma(a2.x * x[k], t[k], -y[k]);
+                c0 = fma(z, ammC[k], c0);
+                s0 = fma(z, ammS[k], s0);
+                const double w = fma(a2.y * z, t[k], -x[k]);
+                c1 = fma(w, ampC[k], c1);
+                s1 = fma(w, ampS[k], s1);
                 y[k] = z;
                 x[k] = w;
             }
-            v[2 * j] = c0;
-            v[2 * j + 1] = s0;
-            v[2 * j + 2] = c1;
-            v[2 * j + 3] = s1;
+            rd.put(j, c0, s0);
+            rd.put(j + 1, c1, s1);
         }
         rd.finish();
-        rd.post(v, 8, dst + 2 * n);
+        rd.commit(8, dst + 2 * n);
     }
+    // last, partial tile
     if (n <= nmax) {
+        if (n - st.n0 == 64) st.next();
+        const int o = n - st.n0;
         const int cnt = nmax - n + 1;
         rd.begin();
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             double c = 0.0, sv = 0.0;
             if (j < cnt) {
-                const double2 abj = abm[n + j];
+                const double al = st.al[o + j];
 #pragma unroll
                 for (int k = 0; k < K; k++) {
-                    const double z = fma(abj.x * t[k], x[k], -(abj.y * y[k]));
+                    const double z = fma(al * x[k], t[k], -y[k]);
                     y[k] = x[k];
                     x[k] = z;
                     const double p = ldexp(z, e[k]);
-                    c = fma(p, (j & 1) ? ammC[k] : ampC[k], c);
-                    sv = fma(p, (j & 1) ? ammS[k] : ampS[k], sv);
+                    c = fma(p, (j & 1) ? ampC[k] : ammC[k], c);
+                    sv = fma(p, (j & 1) ? ampS[k] : ammS[k], sv);
                 }
             }
-            v[2 * j] = c;
-            v[2 * j + 1] = sv;
+            rd.put(j, c, sv);
         }
         rd.finish();
-        rd.post(v, cnt, dst + 2 * n);
+        rd.commit(cnt, dst + 2 * n);
     }
     rd.begin();
     rd.finish();
@@ -757,9 +879,9 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, d
 template <int K>
 __global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
 {
-    __shared__ __align__(16) double lds[ShaLds<K>::SIZE];
+    __shared__ __align__(16) ShaSmem<K> sm;
     const int b = blockIdx.x;
-    sha_item<K>(A, (b >> 3) % A.ntile, ((b >> 3) / A.ntile) * 8 + (b & 7), lds);
+    sha_item<K>(A, (b >> 3) % A.ntile, ((b >> 3) / A.ntile) * 8 + (b & 7), sm);
 }
 
 template <int K>
@@ -767,26 +889,28 @@ __global__ __launch_bounds__(256) void sha_batch_kernel(const ShaArgs *__restric
                                                         const int2 *__restrict__ items, int nitems,
                                                         int *counter)
 {
-    __shared__ __align__(16) double lds[ShaLds<K>::SIZE];
+    __shared__ __align__(16) ShaSmem<K> sm;
     __shared__ int cur;
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) cur = atomicAdd(counter, 1);
         __syncthreads();
-        const int it = cur;
+        const int it = uniform(cur);
         if (it >= nitems) break;
         const int2 w = items[it];
-        const ShaArgs A = reqs[w.x];
-        sha_item<K>(A, w.y >> 16, w.y & 0xFFFF, lds);
+        const ShaArgs A = reqs[uniform(w.x)];
+        const int wy = uniform(w.y);
+        sha_item<K>(A, wy >> 16, wy & 0xFFFF, sm);
     }
 }
 
-// Sum the tile partials in tile order, rescale by scale[n] (if given) and
-// write the coefficients of a model of degree nmax_out (zeros above nmax)
+// Sum the tile partials in tile order, apply s_nm and scale[n] (if given),
+// and write the coefficients of a model of degree nmax_out (zeros above nmax)
 // (src/sha/sha_point.c:381,1088-1108; src/shc/shc_rescale.c:24-90).
 __global__ void sha_reduce_kernel(int nmax, int ntile, const double2 *__restrict__ part,
-                                  int nmax_out, const double *__restrict__ scale,
-                                  double *__restrict__ c, double *__restrict__ s, size_t stride)
+                                  const double *__restrict__ sn, int nt, int nmax_out,
+                                  const double *__restrict__ scale, double *__restrict__ c,
+                                  double *__restrict__ s, size_t stride)
 {
     const int m = blockIdx.y;
     const size_t P = tri_len_dev(nmax);
@@ -801,6 +925,9 @@ __global__ void sha_reduce_kernel(int nmax, int ntile, const double2 *__restrict
                 vc += p.x;
                 vs += p.y;
             }
+            const double f = sn[dcol_off(nt, m) + (n - m)];
+            vc *= f;
+            vs *= f;
             if (scale) {
                 vc *= scale[n];
                 vs *= scale[n];
@@ -814,131 +941,230 @@ __global__ void sha_reduce_kernel(int nmax, int ntile, const double2 *__restrict
 
 // ---------------------------------------------------------------------------
 // SHS at scattered points (src/shs/shs_point_sctr.c:294-687,
-// shs_sctr_mulc.c:36-57).  One lane per point, sequential over the orders of
-// one chunk; the sectorial seed and (R/r)^(m+1) advance with m in the lane.
+// shs_sctr_mulc.c:36-57).  K points per lane (points sorted by latitude, so
+// that a wave's points leave the X-number range together), sequential over
+// the orders of one chunk; the sectorial seed and (R/r)^(m+1) advance with m
+// in the lane.  With r != R the factor (R/r)^(n+1) is carried by the
+// recurrence itself: Q~_n = alpha_n (t q) Q~_{n-1} - q^2 Q~_{n-2}, q = R/r.
 // ---------------------------------------------------------------------------
-template <bool UNIT>
+template <int K, bool UNIT>
 __global__ __launch_bounds__(256) void sctr_kernel(SctrArgs A)
 {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ __align__(16) double s_al[4][64];
+    __shared__ __align__(16) double2 s_cs[4][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = uniform(threadIdx.x >> 6);
+    const size_t base = ((size_t)blockIdx.x * 4 + wave) * 64 * K;
     const int chunk = blockIdx.y;
     const int mb = A.mchunk[chunk], me = A.mchunk[chunk + 1];
-    const bool valid = i < A.npoint;
-    const double t = valid ? A.lat_t[i] : 0.0;
-    const double u = valid ? A.lat_u[i] : 0.0;
-    const double lon = valid ? A.lon[i] : 0.0;
-    const double q = (!UNIT && valid) ? A.R / A.r[i] : 1.0;
     const int nmax = A.nmax;
 
-    // P_{mb,mb} (for mb >= 1) as an X-number, and (R/r)^(mb+1)
-    double xs = ROOT3 * u;
-    int ixs = 0;
-    for (int mm = 1; mm < mb; mm++) {
-        xs = (A.dm[mm] * u) * xs;
-        xnorm(xs, ixs);
+    bool valid[K];
+    double t[K], tq[K], q2[K], u[K], lon[K], q[K];
+    double xs[K], rpc[K], f[K];
+    int ixs[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const size_t i = base + k * 64 + lane;
+        valid[k] = i < A.npoint;
+        t[k] = valid[k] ? A.lat_t[i] : 0.0;
+        u[k] = valid[k] ? A.lat_u[i] : 0.0;
+        lon[k] = valid[k] ? A.lon[i] : 0.0;
+        q[k] = (!UNIT && valid[k]) ? A.R / A.r[i] : 1.0;
+        tq[k] = UNIT ? t[k] : t[k] * q[k];
+        q2[k] = q[k] * q[k];
+        // P_{mb,mb} (mb >= 1) as an X-number, and (R/r)^(mb+1)
+        xs[k] = ROOT3 * u[k];
+        ixs[k] = 0;
+        for (int mm = 1; mm < mb; mm++) {
+            xs[k] = (A.dm[mm] * u[k]) * xs[k];
+            xnorm(xs[k], ixs[k]);
+        }
+        rpc[k] = q[k];
+        if (!UNIT)
+            for (int kk = 2; kk <= mb + 1; kk++) rpc[k] = rpc[k] * q[k];
+        f[k] = 0.0;
     }
-    double rpc = 1.0;
-    if (!UNIT)
-        for (int k = 1; k <= mb + 1; k++) rpc = rpc * q;
 
-    double f = 0.0;
+    ColStage<true> st;
+    st.al = s_al[wave];
+    st.cs = s_cs[wave];
+    st.nlast = nmax;
+    st.lane = lane;
     for (int m = mb; m < me; m++) {
-        const double2 *__restrict__ abm = (const double2 *)A.ab + dcol_off(A.ab_nt, m) - m;
         const double2 *__restrict__ csm = (const double2 *)A.cs + dcol_off(nmax, m) - m;
-        double a = 0.0, b = 0.0, rp = rpc;
-        double x = 0.0, y = 0.0;
-        int e = 0;
-#define SC_ACC(p, c)                              \
-    do {                                          \
-        const double pq_ = UNIT ? (p) : rp * (p); \
-        a = fma(pq_, (c).x, a);                   \
-        b = fma(pq_, (c).y, b);                   \
-        if (!UNIT) rp *= q;                       \
-    } while (0)
-        int n;
-        if (m == 0) {
-            SC_ACC(1.0, csm[0]);
-            if (nmax >= 1) {
-                x = ROOT3 * t;
-                y = 1.0;
-                SC_ACC(x, csm[1]);
+        double x[K], y[K], a[K], b[K];
+        int e[K];
+        const double2 cm = csm[m];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            // Q~_m = P_mm (R/r)^(m+1) as a mantissa and a binary exponent
+            double xm_ = (m == 0) ? 1.0 : xs[k];
+            int em = (m == 0) ? 0 : 960 * ixs[k];
+            if (!UNIT) {
+                int er;
+                const double rm = frexp(rpc[k], &er);
+                xm_ *= rm;
+                em += er;
             }
-            n = 2;
-        } else {
-            e = valid ? 960 * ixs : 0;
-            SC_ACC(ldexp(xs, e), csm[m]);
-            if (m < nmax) {
-                y = xs;
-                x = (abm[m + 1].x * t) * xs;
-                SC_ACC(ldexp(x, e), csm[m + 1]);
-                lane_renorm(x, y, e);
-            }
-            n = m + 2;
+            x[k] = valid[k] ? xm_ : 0.0;
+            e[k] = valid[k] ? em : 0;
+            y[k] = 0.0;
+            const double p = ldexp(x[k], e[k]);
+            a[k] = p * cm.x;
+            b[k] = p * cm.y;
         }
-        for (;;) {
-            if (n + 7 > nmax || !wave_any(e != 0)) break;
-            if (wave_all(!valid || lane_deep(x, y, e))) {
+        if (m < nmax) {
+            st.gal = A.al + dcol_off(A.al_nt, m) - m;
+            st.gcs = csm;
+            st.start(m + 1);
+#define SC_REC(k, al)                                                      \
+    (UNIT ? fma((al) * x[k], tq[k], -y[k]) : fma((al) * x[k], tq[k], -(q2[k] * y[k])))
+            int n = m + 1;
+            // phase 1: tiles while a lane of the wave still carries an exponent
+            for (; n + 7 <= nmax; n += 8) {
+                if (n - st.n0 == 64) st.next();
+                const int o = n - st.n0;
+                bool anyx = false, deep = true;
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    anyx = anyx || (e[k] != 0);
+                    deep = deep && (!valid[k] || lane_deep(x[k], y[k], e[k]));
+                }
+                if (!wave_any(anyx)) break;
+                if (wave_all(deep)) {
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const double al = st.al[o + j];
+#pragma unroll
+                        for (int k = 0; k < K; k++) {
+                            const double z = SC_REC(k, al);
+                            y[k] = x[k];
+                            x[k] = z;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const double al = st.al[o + j];
+                        const double2 c = st.cs[o + j];
+#pragma unroll
+                        for (int k = 0; k < K; k++) {
+                            const double z = SC_REC(k, al);
+                            y[k] = x[k];
+                            x[k] = z;
+                            const double p = ldexp(z, e[k]);
+                            a[k] = fma(p, c.x, a[k]);
+                            b[k] = fma(p, c.y, b[k]);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < K; k++) lane_renorm(x[k], y[k], e[k]);
+            }
+            // phase 2: every lane in plain doubles
+            for (; n + 7 <= nmax; n += 8) {
+                if (n - st.n0 == 64) st.next();
+                const int o = n - st.n0;
+#pragma unroll
+                for (int j = 0; j < 8; j += 2) {
+                    const double2 a2 = *(const double2 *)(st.al + o + j);
+                    const double2 c0 = st.cs[o + j], c1 = st.cs[o + j + 1];
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        const double z = SC_REC(k, a2.x);
+                        a[k] = fma(z, c0.x, a[k]);
+                        b[k] = fma(z, c0.y, b[k]);
+                        y[k] = x[k];
+                        x[k] = z;
+                        const double w = SC_REC(k, a2.y);
+                        a[k] = fma(w, c1.x, a[k]);
+                        b[k] = fma(w, c1.y, b[k]);
+                        y[k] = x[k];
+                        x[k] = w;
+                    }
+                }
+            }
+            // last, partial tile
+            if (n <= nmax) {
+                if (n - st.n0 == 64) st.next();
+                const int o = n - st.n0;
+                const int cnt = nmax - n + 1;
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
-                    const double2 ab = abm[n + j];
-                    const double z = fma(ab.x * t, x, -(ab.y * y));
-                    y = x;
-                    x = z;
-                    if (!UNIT) rp *= q;
-                }
-            } else {
+                    if (j < cnt) {
+                        const double al = st.al[o + j];
+                        const double2 c = st.cs[o + j];
 #pragma unroll
-                for (int j = 0; j < 8; j++) {
-                    const double2 ab = abm[n + j];
-                    const double z = fma(ab.x * t, x, -(ab.y * y));
-                    y = x;
-                    x = z;
-                    SC_ACC(ldexp(z, e), csm[n + j]);
+                        for (int k = 0; k < K; k++) {
+                            const double z = SC_REC(k, al);
+                            y[k] = x[k];
+                            x[k] = z;
+                            const double p = ldexp(z, e[k]);
+                            a[k] = fma(p, c.x, a[k]);
+                            b[k] = fma(p, c.y, b[k]);
+                        }
+                    }
                 }
             }
-            lane_renorm(x, y, e);
-            n += 8;
+#undef SC_REC
         }
-        for (; n + 7 <= nmax; n += 8) {
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const double2 ab = abm[n + j];
-                const double z = fma(ab.x * t, x, -(ab.y * y));
-                SC_ACC(z, csm[n + j]);
-                y = x;
-                x = z;
+        for (int k = 0; k < K; k++) {
+            if (m == 0) b[k] = 0.0;
+            double sn, cn;
+            sincos((double)m * lon[k], &sn, &cn);
+            f[k] = f[k] + (a[k] * cn + b[k] * sn);
+            // advance the seed to P_{m+1,m+1} and the radius power
+            if (m >= 1) {
+                xs[k] = (A.dm[m] * u[k]) * xs[k];
+                xnorm(xs[k], ixs[k]);
             }
+            if (!UNIT) rpc[k] = rpc[k] * q[k];
         }
-        for (; n <= nmax; n++) {
-            const double2 ab = abm[n];
-            const double z = fma(ab.x * t, x, -(ab.y * y));
-            y = x;
-            x = z;
-            SC_ACC(ldexp(z, e), csm[n]);
-        }
-#undef SC_ACC
-        if (m == 0) b = 0.0;
-        double sn, cn;
-        sincos((double)m * lon, &sn, &cn);
-        f = f + (a * cn + b * sn);
-        // advance the seed to P_{m+1,m+1} and the radius power
-        if (m >= 1) {
-            xs = (A.dm[m] * u) * xs;
-            xnorm(xs, ixs);
-        }
-        if (!UNIT) rpc = rpc * q;
     }
-    if (valid) A.part[(size_t)chunk * A.npoint + i] = f;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const size_t i = base + k * 64 + lane;
+        if (valid[k]) A.part[(size_t)chunk * A.npoint + i] = f[k];
+    }
 }
 
+// f[perm[i]] = mu/R * sum over chunks (perm == nullptr: identity)
 __global__ void sctr_reduce_kernel(size_t npoint, int nchunk, const double *__restrict__ part,
-                                   double mur, double *__restrict__ f)
+                                   const int *__restrict__ perm, double mur, double *__restrict__ f)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npoint) return;
     double s = part[i];
     for (int c = 1; c < nchunk; c++) s += part[(size_t)c * npoint + i];
-    f[i] = mur * s;
+    f[perm ? (size_t)perm[i] : i] = mur * s;
+}
+
+// points in the order of perm: t, u from lat; lon and r gathered
+__global__ void gather_points_kernel(size_t n, const int *__restrict__ perm,
+                                     const double *__restrict__ lat, const double *__restrict__ lon,
+                                     const double *__restrict__ r, double *__restrict__ t_out,
+                                     double *__restrict__ u_out, double *__restrict__ lon_out,
+                                     double *__restrict__ r_out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t p = (size_t)perm[i];
+    sincos(lat[p], &t_out[i], &u_out[i]);
+    lon_out[i] = lon[p];
+    r_out[i] = r[p];
+}
+
+// sort key of a point: |sin lat| (a monotone function of the distance from the
+// equator), and the identity permutation
+__global__ void sort_key_kernel(size_t n, const double *__restrict__ lat, double *__restrict__ key,
+                                int *__restrict__ idx)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    key[i] = fabs(sin(lat[i]));
+    idx[i] = (int)i;
 }
 
 __global__ void sincos_kernel(size_t n, const double *__restrict__ lat, double *__restrict__ t,
@@ -981,10 +1207,23 @@ static inline hipStream_t S(void *s)
     return (hipStream_t)s;
 }
 
-int launch_ab_table(int nt, const double *r, const double *ri, void *ab, void *stream)
+int launch_leg_tables(int nt, const double *r, const double *ri, void *ab_tmp, double *al,
+                      double *sn, void *stream)
 {
     hipLaunchKernelGGL(ab_table_kernel, dim3(nt + 1), dim3(256), 0, S(stream), nt, r, ri,
-                       (double2 *)ab);
+                       (double2 *)ab_tmp);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(alpha_table_kernel, dim3((nt + 1 + 63) / 64), dim3(64), 0, S(stream), nt,
+                       (const double2 *)ab_tmp, al, sn);
+    return (int)hipGetLastError();
+}
+
+int launch_scale_cs(int nmax, const void *cs, const double *sn, int nt, void *out, void *stream)
+{
+    const unsigned gx = (unsigned)((nmax + 1 + 255) / 256);
+    hipLaunchKernelGGL(scale_cs_kernel, dim3(gx, (unsigned)nmax + 1), dim3(256), 0, S(stream), nmax,
+                       (const double2 *)cs, sn, nt, (double2 *)out);
     return (int)hipGetLastError();
 }
 
@@ -1066,31 +1305,84 @@ int launch_sha(const ShaArgs &a, void *stream)
     return (int)hipGetLastError();
 }
 
-int launch_sha_reduce(int nmax, int ntile, const void *part, int nmax_out, const double *scale,
-                      double *c, double *s, size_t stride, void *stream)
+int launch_sha_reduce(int nmax, int ntile, const void *part, const double *sn, int nt, int nmax_out,
+                      const double *scale, double *c, double *s, size_t stride, void *stream)
 {
     const unsigned gx = (unsigned)((nmax_out + 1 + 255) / 256);
     hipLaunchKernelGGL(sha_reduce_kernel, dim3(gx, nmax_out + 1), dim3(256), 0, S(stream), nmax,
-                       ntile, (const double2 *)part, nmax_out, scale, c, s, stride);
+                       ntile, (const double2 *)part, sn, nt, nmax_out, scale, c, s, stride);
     return (int)hipGetLastError();
+}
+
+int sctr_points_per_block(int k)
+{
+    return 256 * k;
+}
+
+template <int K>
+static void sctr_launch_k(const SctrArgs &a, hipStream_t s)
+{
+    dim3 grid((unsigned)((a.npoint + 256 * K - 1) / (256 * K)), (unsigned)a.nchunk);
+    if (a.unit)
+        hipLaunchKernelGGL((sctr_kernel<K, true>), grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((sctr_kernel<K, false>), grid, dim3(256), 0, s, a);
 }
 
 int launch_sctr(const SctrArgs &a, void *stream)
 {
-    dim3 grid((unsigned)((a.npoint + 255) / 256), (unsigned)a.nchunk);
-    if (a.unit)
-        hipLaunchKernelGGL((sctr_kernel<true>), grid, dim3(256), 0, S(stream), a);
-    else
-        hipLaunchKernelGGL((sctr_kernel<false>), grid, dim3(256), 0, S(stream), a);
+    switch (a.k) {
+    case 1: sctr_launch_k<1>(a, S(stream)); break;
+    case 4: sctr_launch_k<4>(a, S(stream)); break;
+    default: sctr_launch_k<2>(a, S(stream)); break;
+    }
     return (int)hipGetLastError();
 }
 
-int launch_sctr_reduce(size_t npoint, int nchunk, const double *part, double mur, double *f,
-                       void *stream)
+int launch_sctr_reduce(size_t npoint, int nchunk, const double *part, const int *perm, double mur,
+                       double *f, void *stream)
 {
     hipLaunchKernelGGL(sctr_reduce_kernel, dim3((unsigned)((npoint + 255) / 256)), dim3(256), 0,
-                       S(stream), npoint, nchunk, part, mur, f);
+                       S(stream), npoint, nchunk, part, perm, mur, f);
     return (int)hipGetLastError();
+}
+
+// Points ordered by |sin lat|: perm (device, n ints) and the gathered
+// t = sin lat, u = cos lat, lon, r.  Temporary storage comes from the
+// stream-ordered pool.
+int launch_sort_points(size_t n, const double *lat, const double *lon, const double *r, int *perm,
+                       double *t_out, double *u_out, double *lon_out, double *r_out, void *stream)
+{
+    hipStream_t s = S(stream);
+    double *k0 = nullptr, *k1 = nullptr;
+    int *i0 = nullptr;
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    hipError_t e = hipMallocAsync((void **)&k0, n * sizeof(double), s);
+    if (e == hipSuccess) e = hipMallocAsync((void **)&k1, n * sizeof(double), s);
+    if (e == hipSuccess) e = hipMallocAsync((void **)&i0, n * sizeof(int), s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(sort_key_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                           lat, k0, i0);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0, k1, i0, perm, (int)n, 0,
+                                               (int)(8 * sizeof(double)), s);
+    if (e == hipSuccess) e = hipMallocAsync(&tmp, std::max<size_t>(tmp_bytes, 16), s);
+    if (e == hipSuccess)
+        e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k1, i0, perm, (int)n, 0,
+                                               (int)(8 * sizeof(double)), s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           n, perm, lat, lon, r, t_out, u_out, lon_out, r_out);
+        e = hipGetLastError();
+    }
+    if (tmp) (void)hipFreeAsync(tmp, s);
+    if (i0) (void)hipFreeAsync(i0, s);
+    if (k1) (void)hipFreeAsync(k1, s);
+    if (k0) (void)hipFreeAsync(k0, s);
+    return (int)e;
 }
 
 int launch_sincos(size_t n, const double *lat, double *t, double *u, void *stream)
@@ -1114,7 +1406,7 @@ int launch_all_near(size_t n, const double *x, double ref, double eps, int *resu
     }
     if (e == hipSuccess) e = hipMemcpyAsync(result, flag, sizeof(int), hipMemcpyDeviceToHost, S(stream));
     if (e == hipSuccess) e = hipStreamSynchronize(S(stream));
-    hipFreeAsync(flag, S(stream));
+    (void)hipFreeAsync(flag, S(stream));
     return (int)e;
 }
 
@@ -1158,15 +1450,29 @@ int batch_ring_k(int kind)
     return kind == 0 ? BATCH_SHS_K : BATCH_SHA_K;
 }
 
+// Resident blocks per CU of a 256-thread kernel on gfx950: 8 wave slots per
+// SIMD, a 512-entry VGPR file per SIMD lane (allocated in granules of 8) and
+// 160 KiB of LDS per CU.  (hipOccupancyMaxActiveBlocksPerMultiprocessor
+// budgets 256 VGPRs per lane and under-fills the persistent grids.)
+template <class F>
+int resident_blocks(F kernel)
+{
+    hipFuncAttributes fa{};
+    if (hipFuncGetAttributes(&fa, (const void *)kernel) != hipSuccess) return 1;
+    const int vg = std::max(8, (fa.numRegs + 7) / 8 * 8);
+    int per = std::min(8, 512 / vg);
+    const size_t lds = fa.sharedSizeBytes;
+    if (lds > 0) per = std::min<int>(per, (int)((160 * 1024) / lds));
+    return std::max(per, 1);
+}
+
 int batch_grid_size(int kind)
 {
-    int dev = 0, ncu = 0, per = 0;
+    int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    hipError_t e = (kind == 0)
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, shs_batch_kernel<BATCH_SHS_K>, 256, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, sha_batch_kernel<BATCH_SHA_K>, 256, 0);
-    if (e != hipSuccess || per < 1) per = 1;
+    const int per = (kind == 0) ? resident_blocks(shs_batch_kernel<BATCH_SHS_K>)
+                                : resident_blocks(sha_batch_kernel<BATCH_SHA_K>);
     return ncu * per;
 }
 

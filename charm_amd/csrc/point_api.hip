@@ -219,8 +219,9 @@ hipfftHandle fft_plan(int kind, size_t n, size_t batch, size_t ldc, hipStream_t 
 // (src/leg/leg_func_r_ri.c, leg_func_dm.c, leg_func_anm_bnm.c)
 // ---------------------------------------------------------------------------
 struct Tables {
-    void *ab = nullptr;
-    double *dm = nullptr;
+    double *al = nullptr;  // alpha_nm of the scaled recurrence, m-major, degree nt
+    double *sn = nullptr;  // s_nm, same layout
+    double *dm = nullptr;  // dm[n] of the sectorial seeds
     int nt = -1;
 };
 std::mutex g_tab_mu;
@@ -232,6 +233,8 @@ Tables tables(int nmax, hipStream_t stream)
     std::lock_guard<std::mutex> lk(g_tab_mu);
     if (g_tab.nt >= nmax) return g_tab;
     int nt = std::max(nmax, std::max(64, g_tab.nt + g_tab.nt / 4));
+    // r[k] = sqrt(k), ri[k] = 1/sqrt(k) (src/leg/leg_func_r_ri.c:15-30) and
+    // dm[n] (src/leg/leg_func_dm.c:14-23)
     std::vector<double> r(2 * (size_t)nt + 4), ri(2 * (size_t)nt + 4), dm((size_t)nt + 1);
     r[0] = ri[0] = 0.0;
     for (size_t k = 1; k <= 2 * (size_t)nt + 3; k++) {
@@ -242,23 +245,28 @@ Tables tables(int nmax, hipStream_t stream)
     dm[0] = 0.0;
     for (size_t n = 1; n <= (size_t)nt; n++) dm[n] = r[2 * n + 3] * ri[2 * n + 2];
     void *ab = nullptr;
-    double *ddm = nullptr, *dr = nullptr, *dri = nullptr;
+    double *al = nullptr, *sn = nullptr, *ddm = nullptr, *dr = nullptr, *dri = nullptr;
     HIPCHK(hipMalloc(&ab, tri_len(nt) * 2 * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&al, tri_len(nt) * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&sn, tri_len(nt) * sizeof(double)));
     HIPCHK(hipMalloc((void **)&ddm, dm.size() * sizeof(double)));
     HIPCHK(hipMalloc((void **)&dr, r.size() * sizeof(double)));
     HIPCHK(hipMalloc((void **)&dri, ri.size() * sizeof(double)));
     HIPCHK(hipMemcpyAsync(ddm, dm.data(), dm.size() * sizeof(double), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(dr, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(dri, ri.data(), ri.size() * sizeof(double), hipMemcpyHostToDevice, stream));
-    HIPCHK((hipError_t)launch_ab_table(nt, dr, dri, ab, stream));
+    HIPCHK((hipError_t)launch_leg_tables(nt, dr, dri, ab, al, sn, stream));
     HIPCHK(hipStreamSynchronize(stream));
     HIPCHK(hipFree(dr));
     HIPCHK(hipFree(dri));
-    if (g_tab.ab) {
-        g_retired.push_back(g_tab.ab);
+    HIPCHK(hipFree(ab));
+    if (g_tab.al) {
+        g_retired.push_back(g_tab.al);
+        g_retired.push_back(g_tab.sn);
         g_retired.push_back(g_tab.dm);
     }
-    g_tab.ab = ab;
+    g_tab.al = al;
+    g_tab.sn = sn;
     g_tab.dm = ddm;
     g_tab.nt = nt;
     return g_tab;
@@ -442,9 +450,12 @@ void run_shs_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev, bo
     a.ratio = d.ratio;
     a.rld = d.rld;
     a.mirror = d.mirror;
-    a.ab = tb.ab;
-    a.ab_nt = tb.nt;
-    a.cs = cs_dev;
+    a.al = tb.al;
+    a.al_nt = tb.nt;
+    // coefficients times s_nm for the scaled recurrence
+    DevBuf cs2(tri_len(nmax) * 2 * sizeof(double), stream);
+    HIPCHK((hipError_t)launch_scale_cs((int)nmax, cs_dev, tb.sn, tb.nt, cs2.p, stream));
+    a.cs = cs2.p;
     a.mur = mur;
     auto pick_k = [&](ShsGridArgs &x) {
         x.k = tuned_k(0, (int)g.nring, nmax, 1, 3, stream, [&](int k) {
@@ -564,12 +575,42 @@ void shs_grid_host(const charm_point *pnt, const charm_shc *shcs, unsigned long 
 // ---------------------------------------------------------------------------
 // Scattered points
 // ---------------------------------------------------------------------------
-void run_sctr(size_t npoint, const double *t_dev, const double *u_dev, const double *lon_dev,
-              const double *r_dev, bool unit, unsigned long nmax, double mu, double R,
-              const void *cs_dev, const Tables &tb, double *f_dev, hipStream_t s)
+int sctr_k()
 {
+    if (const char *v = std::getenv("CHARM_AMD_SCTR_K")) {
+        const int k = std::atoi(v);
+        if (k == 1 || k == 2 || k == 4) return k;
+    }
+    return 2;
+}
+
+// Scattered synthesis on device arrays: lat/lon/r of npoint points, unscaled
+// coefficients cs (interleaved pairs, degree nmax).  Points are processed in
+// the order of |sin lat| (results go back to the caller's order).
+void run_sctr(size_t npoint, const double *lat_dev, const double *lon_dev, const double *r_dev,
+              bool unit, unsigned long nmax, double mu, double R, const void *cs_dev,
+              const Tables &tb, double *f_dev, hipStream_t s)
+{
+    const int K = sctr_k();
+    DevBuf cs2(tri_len(nmax) * 2 * sizeof(double), s);
+    HIPCHK((hipError_t)launch_scale_cs((int)nmax, cs_dev, tb.sn, tb.nt, cs2.p, s));
+    DevBuf dt(npoint * sizeof(double), s), du(npoint * sizeof(double), s);
+    DevBuf dl, dr, perm;
+    const double *lon_use = lon_dev, *r_use = r_dev;
+    if (npoint < (size_t)INT32_MAX) {
+        perm.alloc(npoint * sizeof(int), s);
+        dl.alloc(npoint * sizeof(double), s);
+        dr.alloc(npoint * sizeof(double), s);
+        HIPCHK((hipError_t)launch_sort_points(npoint, lat_dev, lon_dev, r_dev, perm.as<int>(),
+                                              dt.as<double>(), du.as<double>(), dl.as<double>(),
+                                              dr.as<double>(), s));
+        lon_use = dl.as<double>();
+        r_use = dr.as<double>();
+    } else {
+        HIPCHK((hipError_t)launch_sincos(npoint, lat_dev, dt.as<double>(), du.as<double>(), s));
+    }
     // split orders into chunks of equal work when there are few points
-    const size_t nwave = (npoint + 63) / 64;
+    const size_t nwave = (npoint + 64 * K - 1) / (64 * K);
     int nchunk = (int)std::min<size_t>(64, std::max<size_t>(1, 4096 / std::max<size_t>(nwave, 1)));
     nchunk = (int)std::min<unsigned long>((unsigned long)nchunk, nmax + 1);
     std::vector<int> mc(nchunk + 1);
@@ -595,20 +636,22 @@ void run_sctr(size_t npoint, const double *t_dev, const double *u_dev, const dou
     a.mchunk = dmc.as<int>();
     a.unit = unit;
     a.R = R;
-    a.lat_t = t_dev;
-    a.lat_u = u_dev;
-    a.lon = lon_dev;
-    a.r = r_dev;
+    a.lat_t = dt.as<double>();
+    a.lat_u = du.as<double>();
+    a.lon = lon_use;
+    a.r = r_use;
     a.dm = tb.dm;
-    a.ab = tb.ab;
-    a.ab_nt = tb.nt;
-    a.cs = cs_dev;
+    a.al = tb.al;
+    a.al_nt = tb.nt;
+    a.cs = cs2.p;
     a.part = part.as<double>();
+    a.k = K;
     {
         ProfScope ps("sctr_kernel", s);
         HIPCHK((hipError_t)launch_sctr(a, s));
     }
-    HIPCHK((hipError_t)launch_sctr_reduce(npoint, nchunk, part.as<double>(), mu / R, f_dev, s));
+    HIPCHK((hipError_t)launch_sctr_reduce(npoint, nchunk, part.as<double>(), perm.as<int>(),
+                                          mu / R, f_dev, s));
 }
 
 void shs_sctr_host(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax, double *f,
@@ -634,20 +677,14 @@ void shs_sctr_host(const charm_point *pnt, const charm_shc *shcs, unsigned long 
     pack_coeffs(shcs, nmax, hcs);
     DevBuf cs(P * 2 * sizeof(double), s);
     h2d(cs.as<double>(), hcs, 2 * P, s);
-    std::vector<double> t(npoint), u(npoint);
-    for (size_t i = 0; i < npoint; i++) {
-        t[i] = std::sin(pnt->lat[i]);
-        u[i] = std::cos(pnt->lat[i]);
-    }
-    DevBuf dt(npoint * sizeof(double), s), du(npoint * sizeof(double), s);
-    DevBuf dl(npoint * sizeof(double), s), dr(npoint * sizeof(double), s);
-    h2d(dt.as<double>(), t.data(), npoint, s);
-    h2d(du.as<double>(), u.data(), npoint, s);
+    DevBuf dlat(npoint * sizeof(double), s), dl(npoint * sizeof(double), s);
+    DevBuf dr(npoint * sizeof(double), s);
+    h2d(dlat.as<double>(), pnt->lat, npoint, s);
     h2d(dl.as<double>(), pnt->lon, npoint, s);
     h2d(dr.as<double>(), pnt->r, npoint, s);
     DevBuf df(npoint * sizeof(double), s);
-    run_sctr(npoint, dt.as<double>(), du.as<double>(), dl.as<double>(), dr.as<double>(), unit,
-             nmax, shcs->mu, shcs->r, cs.p, tb, df.as<double>(), s);
+    run_sctr(npoint, dlat.as<double>(), dl.as<double>(), dr.as<double>(), unit, nmax, shcs->mu,
+             shcs->r, cs.p, tb, df.as<double>(), s);
     HIPCHK(hipMemcpyAsync(f, df.p, npoint * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
 }
@@ -701,8 +738,8 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
     a.cw = cw_dev;
     a.spec = spec.p;
     a.lds = nfc;
-    a.ab = tb.ab;
-    a.ab_nt = tb.nt;
+    a.al = tb.al;
+    a.al_nt = tb.nt;
     // partial sums sized for the smallest tile width (most tiles)
     DevBuf part((size_t)ring_tiles(g.nring, 2) * tri_len(nmax) * 2 * sizeof(double), s);
     a.part = part.p;
@@ -719,8 +756,8 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
     }
     {
         ProfScope ps("sha_reduce_kernel", s);
-        HIPCHK((hipError_t)launch_sha_reduce((int)nmax, a.ntile, part.p, (int)nmax_out, scale_dev,
-                                             c_dev, s_dev, stride, s));
+        HIPCHK((hipError_t)launch_sha_reduce((int)nmax, a.ntile, part.p, tb.sn, tb.nt,
+                                             (int)nmax_out, scale_dev, c_dev, s_dev, stride, s));
     }
 }
 
@@ -1070,13 +1107,10 @@ void charm_amd_shs_sctr_dev(size_t npoint, const double *lat_dev, const double *
     try {
         hipStream_t s = (hipStream_t)stream;
         Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
-        DevBuf dt(npoint * sizeof(double), s), du(npoint * sizeof(double), s);
-        HIPCHK((hipError_t)launch_sincos(npoint, lat_dev, dt.as<double>(), du.as<double>(), s));
         // radii equal to R within glob_threshold (src/shs/shs_r_eq_rref.c:17-30)
         int unit = 0;
         HIPCHK((hipError_t)launch_all_near(npoint, r_dev, R, charm_glob_threshold, &unit, s));
-        run_sctr(npoint, dt.as<double>(), du.as<double>(), lon_dev, r_dev, unit != 0, nmax, mu, R,
-                 cs_dev, tb, f_dev, s);
+        run_sctr(npoint, lat_dev, lon_dev, r_dev, unit != 0, nmax, mu, R, cs_dev, tb, f_dev, s);
         HIPCHK(hipGetLastError());
     } catch (const HipFail &e) {
         report(err, e, __func__);
@@ -1099,12 +1133,14 @@ struct charm_amd_batch {
     struct Job {
         size_t r;
         void *spec;
-        void *part;
+        void *part;   // SHA: tile partials; SHS: coefficients times s_nm
         int ntile;
         const double *scale;
         size_t nlat, nlon, nfc;
     };
     std::vector<Job> shs, sha;
+    const double *sn = nullptr;  // s_nm table (degree nt) the batch was built with
+    int nt = 0;
     int grid_shs = 0, grid_sha = 0;
     bool use_graph = false;
     hipGraphExec_t exec = nullptr;
@@ -1142,8 +1178,12 @@ void batch_enqueue(charm_amd_batch *b, hipStream_t s)
                                         b->d_counter + 1, b->grid_sha, s));
     for (const auto &j : b->sha) {
         const charm_amd_request &q = b->req[j.r];
-        HIPCHK((hipError_t)launch_sha_reduce((int)q.nmax, j.ntile, j.part, (int)q.nmax, j.scale,
-                                             q.cs, q.cs + 1, 2, s));
+        HIPCHK((hipError_t)launch_sha_reduce((int)q.nmax, j.ntile, j.part, b->sn, b->nt,
+                                             (int)q.nmax, j.scale, q.cs, q.cs + 1, 2, s));
+    }
+    for (const auto &j : b->shs) {
+        const charm_amd_request &q = b->req[j.r];
+        HIPCHK((hipError_t)launch_scale_cs((int)q.nmax, q.cs, b->sn, b->nt, j.part, s));
     }
     HIPCHK((hipError_t)launch_shs_batch(b->d_shs, b->d_shs_items, b->n_shs_items, b->d_counter,
                                         b->grid_shs, s));
@@ -1180,6 +1220,8 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
         unsigned long nmax_all = 1;
         for (auto &q : b->req) nmax_all = std::max(nmax_all, q.nmax);
         Tables tb = tables((int)nmax_all, s);
+        b->sn = tb.sn;
+        b->nt = tb.nt;
         std::vector<ShsGridArgs> shs_args;
         std::vector<ShaArgs> sha_args;
         struct Item {
@@ -1205,9 +1247,10 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
                 a.six = g->six;
                 a.sld = g->sld;
                 a.mirror = g->mirror;
-                a.ab = tb.ab;
-                a.ab_nt = tb.nt;
-                a.cs = q.cs;
+                a.al = tb.al;
+                a.al_nt = tb.nt;
+                void *cs2 = dmalloc(b, tri_len(q.nmax) * 2 * sizeof(double));
+                a.cs = cs2;
                 a.mur = q.mu / q.R;
                 a.fft_layout = 1;
                 a.mcount = (int)nfc;
@@ -1227,7 +1270,7 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
                 a.out = spec;
                 const int idx = (int)shs_args.size();
                 shs_args.push_back(a);
-                b->shs.push_back({i, spec, nullptr, a.ntile, nullptr, nlat, nlon, nfc});
+                b->shs.push_back({i, spec, cs2, a.ntile, nullptr, nlat, nlon, nfc});
                 const int nquad = (a.mcount + 3) / 4;
                 for (int qd = 0; qd < nquad; qd++)
                     for (int tl = 0; tl < a.ntile; tl++) {
@@ -1255,8 +1298,8 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
                 void *spec = dmalloc(b, nlat * nfc * 2 * sizeof(double));
                 a.spec = spec;
                 a.lds = nfc;
-                a.ab = tb.ab;
-                a.ab_nt = tb.nt;
+                a.al = tb.al;
+                a.al_nt = tb.nt;
                 void *part = dmalloc(b, (size_t)a.ntile * tri_len(q.nmax) * 2 * sizeof(double));
                 a.part = part;
                 std::vector<double> sc;
