@@ -138,6 +138,13 @@ int launch_sha_batch(const void *reqs_dev, const void *items_dev, int nitems, in
 int launch_all_near(size_t n, const double *x_dev, double ref, double eps, int *result,
                     void *stream);
 
+// Longitude transforms (fft.hip): rows of n reals <-> n/2+1 complex values
+// (hipFFT conventions, unnormalised); n even and at most 8192.
+bool row_fft_supported(size_t n);
+int launch_row_fft(int c2r, size_t n, size_t nrow, double *f, size_t ldf, void *spec, size_t lds,
+                   void *stream);
+int prepare_row_fft(size_t n, void *stream);
+
 // Ring pairs per lane for nring rings, kmin <= k <= kmax: the fewest idle
 // lanes in the last ring tile, then the most rings per lane.
 int ring_k(int nring, int kmin, int kmax);

@@ -214,6 +214,37 @@ hipfftHandle fft_plan(int kind, size_t n, size_t batch, size_t ldc, hipStream_t 
     return h;
 }
 
+// Row transforms of nrow rows of n reals <-> n/2+1 complex values: the LDS
+// Bluestein kernels for even n up to 8192, hipFFT otherwise.
+void rows_c2r(size_t n, size_t nrow, const void *spec, size_t ldc, double *f, hipStream_t s)
+{
+    if (row_fft_supported(n)) {
+        HIPCHK((hipError_t)launch_row_fft(1, n, nrow, f, n, const_cast<void *>(spec), ldc, s));
+    } else {
+        hipfftHandle h = fft_plan(0, n, nrow, ldc, s);
+        FFTCHK(hipfftExecZ2D(h, (hipfftDoubleComplex *)spec, f));
+    }
+}
+
+void rows_r2c(size_t n, size_t nrow, const double *f, void *spec, size_t ldc, hipStream_t s)
+{
+    if (row_fft_supported(n)) {
+        HIPCHK((hipError_t)launch_row_fft(0, n, nrow, const_cast<double *>(f), n, spec, ldc, s));
+    } else {
+        hipfftHandle h = fft_plan(1, n, nrow, ldc, s);
+        FFTCHK(hipfftExecD2Z(h, (hipfftDoubleReal *)f, (hipfftDoubleComplex *)spec));
+    }
+}
+
+// plans (and their tables) made before a stream capture
+void rows_prepare(int kind, size_t n, size_t nrow, size_t ldc, hipStream_t s)
+{
+    if (row_fft_supported(n))
+        HIPCHK((hipError_t)prepare_row_fft(n, s));
+    else
+        (void)fft_plan(kind, n, nrow, ldc, s);
+}
+
 // ---------------------------------------------------------------------------
 // Shared tables: (a_nm, b_nm) for all m <= n <= nt, and dm[n], n <= nt
 // (src/leg/leg_func_r_ri.c, leg_func_dm.c, leg_func_anm_bnm.c)
@@ -478,10 +509,9 @@ void run_shs_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev, bo
             ProfScope ps("shs_grid_kernel", stream);
             HIPCHK((hipError_t)launch_shs_grid(a, stream));
         }
-        hipfftHandle h = fft_plan(0, nlon, nlat, nfc, stream);
         {
             ProfScope ps("fft_c2r", stream);
-            FFTCHK(hipfftExecZ2D(h, (hipfftDoubleComplex *)spec.p, f_dev));
+            rows_c2r(nlon, nlat, spec.p, nfc, f_dev, stream);
         }
     } else {
         const size_t ldl = nmax + 1;
@@ -721,10 +751,9 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
 {
     const size_t nlat = g.nlat, nlon = g.nlon, nfc = nlon / 2 + 1;
     DevBuf spec(nlat * nfc * 2 * sizeof(double), s);
-    hipfftHandle h = fft_plan(1, nlon, nlat, nfc, s);
     {
         ProfScope ps("fft_r2c", s);
-        FFTCHK(hipfftExecD2Z(h, (hipfftDoubleReal *)f_dev, (hipfftDoubleComplex *)spec.p));
+        rows_r2c(nlon, nlat, f_dev, spec.p, nfc, s);
     }
     ShaArgs a{};
     a.nmax = (int)nmax;
@@ -1171,8 +1200,7 @@ void batch_enqueue(charm_amd_batch *b, hipStream_t s)
     HIPCHK((hipError_t)launch_zero_ints(b->d_counter, 2, s));
     for (const auto &j : b->sha) {
         const charm_amd_request &q = b->req[j.r];
-        hipfftHandle h = fft_plan(1, j.nlon, j.nlat, j.nfc, s);
-        FFTCHK(hipfftExecD2Z(h, (hipfftDoubleReal *)q.f, (hipfftDoubleComplex *)j.spec));
+        rows_r2c(j.nlon, j.nlat, q.f, j.spec, j.nfc, s);
     }
     HIPCHK((hipError_t)launch_sha_batch(b->d_sha, b->d_sha_items, b->n_sha_items,
                                         b->d_counter + 1, b->grid_sha, s));
@@ -1189,8 +1217,7 @@ void batch_enqueue(charm_amd_batch *b, hipStream_t s)
                                         b->grid_shs, s));
     for (const auto &j : b->shs) {
         const charm_amd_request &q = b->req[j.r];
-        hipfftHandle h = fft_plan(0, j.nlon, j.nlat, j.nfc, s);
-        FFTCHK(hipfftExecZ2D(h, (hipfftDoubleComplex *)j.spec, q.f));
+        rows_c2r(j.nlon, j.nlat, j.spec, j.nfc, q.f, s);
     }
 }
 
@@ -1359,8 +1386,8 @@ void charm_amd_batch_run(charm_amd_batch *b, void *stream, charm_err *err)
             if (!b->cap) HIPCHK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
             // FFT plans for the capture stream are made (and warmed) before the
             // capture: plan creation allocates and synchronises
-            for (const auto &j : b->sha) (void)fft_plan(1, j.nlon, j.nlat, j.nfc, b->cap);
-            for (const auto &j : b->shs) (void)fft_plan(0, j.nlon, j.nlat, j.nfc, b->cap);
+            for (const auto &j : b->sha) rows_prepare(1, j.nlon, j.nlat, j.nfc, b->cap);
+            for (const auto &j : b->shs) rows_prepare(0, j.nlon, j.nlat, j.nfc, b->cap);
             batch_enqueue(b, b->cap);
             HIPCHK(hipStreamSynchronize(b->cap));
             hipGraph_t graph = nullptr;
