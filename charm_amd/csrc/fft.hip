@@ -39,56 +39,88 @@ struct Cfg {
     static constexpr int SLOTS = M + (M >> 4);
 };
 
-template <int P>
-__device__ __forceinline__ void dif_passes(cplx *x, const cplx *tw)
+// Twiddle indices of radix-8 pass on blocks of S points for butterfly t
+__device__ __forceinline__ fft::Tw8 tw8_for(const cplx *tw, int M, int S, int t)
 {
-    constexpr int M = Cfg<P>::M, T = Cfg<P>::T;
-    int S = M;
-    const int np = fft::pass_count(P);
-    for (int i = 0; i < np; i++) {
-        const int R = fft::pass_radix(P, i);
-        if (R == 8) {
-            for (int t = threadIdx.x; t < M / 8; t += T) fft::dif_butterfly<8>(x, tw, M, S, t);
-        } else if (R == 4) {
-            for (int t = threadIdx.x; t < M / 4; t += T) fft::dif_butterfly<4>(x, tw, M, S, t);
-        } else {
-            for (int t = threadIdx.x; t < M / 2; t += T) fft::dif_butterfly<2>(x, tw, M, S, t);
-        }
-        S /= R;
-        __syncthreads();
-    }
+    const int st = S / 8;
+    const int j = t % st;
+    return fft::tw8_load(tw, j * (M / S), M);
 }
 
-template <int P>
-__device__ __forceinline__ void dit_passes(cplx *x, const cplx *tw)
-{
-    constexpr int M = Cfg<P>::M, T = Cfg<P>::T;
-    const int np = fft::pass_count(P);
-    // block size of pass i is M / (R_0 ... R_{i-1}); run the passes backwards
-    for (int i = np - 1; i >= 0; i--) {
-        int S = M;
-        for (int q = 0; q < i; q++) S /= fft::pass_radix(P, q);
-        const int R = fft::pass_radix(P, i);
-        if (R == 8) {
-            for (int t = threadIdx.x; t < M / 8; t += T) fft::dit_butterfly<8>(x, tw, M, S, t);
-        } else if (R == 4) {
-            for (int t = threadIdx.x; t < M / 4; t += T) fft::dit_butterfly<4>(x, tw, M, S, t);
-        } else {
-            for (int t = threadIdx.x; t < M / 2; t += T) fft::dit_butterfly<2>(x, tw, M, S, t);
-        }
-        __syncthreads();
-    }
-}
-
-// conv = M * (a * b~) in natural order from a in LDS (natural order)
+// conv = M * (a * b~) in natural order from a in LDS (natural order):
+// decimation-in-frequency passes down to 16-point blocks; per block the last
+// forward pass, the product with B and the first inverse pass in registers;
+// decimation-in-time passes back up.  The twiddles of each radix-8 pass are
+// requested before the barrier that precedes it.
 template <int P>
 __device__ __forceinline__ void convolve(cplx *x, const cplx *B, const cplx *tw)
 {
     constexpr int M = Cfg<P>::M, T = Cfg<P>::T;
-    dif_passes<P>(x, tw);
-    for (int e = threadIdx.x; e < M; e += T) x[fft::slot(e)] = fft::mul(x[fft::slot(e)], B[e]);
+    constexpr int F = fft::plan_first(P), N8 = (P == 3) ? 0 : fft::plan_n8(P);
+    constexpr int LAST = (P == 3) ? 8 : 16;  // radix of the fused middle pass
+    const int t = threadIdx.x;
+    constexpr int S8 = (F ? M / F : M);      // block size of the first radix-8 pass
+
+    // forward
+    fft::Tw8 nxt{};
+    if (N8 > 0 && t < M / 8) nxt = tw8_for(tw, M, S8, t);
+    if (F) {
+        for (int u = t; u < M / F; u += T) {
+            if (F == 2)
+                fft::dif_butterfly<2>(x, tw, M, M, u);
+            else
+                fft::dif_butterfly<4>(x, tw, M, M, u);
+        }
+        __syncthreads();
+    }
+    int S = S8;
+#pragma unroll
+    for (int i = 0; i < N8; i++) {
+        const fft::Tw8 cur = nxt;
+        if (i + 1 < N8 && t < M / 8) nxt = tw8_for(tw, M, S / 8, t);
+        if (t < M / 8) {
+            const int st = S / 8, b = t / st, j = t - b * st;
+            fft::dif_butterfly8(x, cur, S, b, j);
+        }
+        S /= 8;
+        __syncthreads();
+    }
+    // middle: contiguous blocks of LAST points
+    if (N8 > 0 && t < M / 8) nxt = tw8_for(tw, M, LAST * 8, t);
+    for (int u = t; u < M / LAST; u += T) {
+        cplx v[LAST];
+#pragma unroll
+        for (int r = 0; r < LAST; r++) v[r] = x[fft::slot(u * LAST + r)];
+        fft::dft<LAST, -1>(v);
+#pragma unroll
+        for (int r = 0; r < LAST; r++) v[r] = fft::mul(v[r], B[u * LAST + r]);
+        fft::dft<LAST, +1>(v);
+#pragma unroll
+        for (int r = 0; r < LAST; r++) x[fft::slot(u * LAST + r)] = v[r];
+    }
     __syncthreads();
-    dit_passes<P>(x, tw);
+    // inverse
+    S = LAST * 8;
+#pragma unroll
+    for (int i = N8 - 1; i >= 0; i--) {
+        const fft::Tw8 cur = nxt;
+        if (i > 0 && t < M / 8) nxt = tw8_for(tw, M, S * 8, t);
+        if (t < M / 8) {
+            const int st = S / 8, b = t / st, j = t - b * st;
+            fft::dit_butterfly8(x, cur, S, b, j);
+        }
+        S *= 8;
+        __syncthreads();
+    }
+    if (F) {
+        for (int u = t; u < M / F; u += T) {
+            if (F == 2)
+                fft::dit_butterfly<2>(x, tw, M, M, u);
+            else
+                fft::dit_butterfly<4>(x, tw, M, M, u);
+        }
+        __syncthreads();
+    }
 }
 
 // f (2L reals per row) -> spec (L+1 complex per row), F_k = sum_j f_j e^{-2 pi i jk/2L}
@@ -157,13 +189,15 @@ __global__ __launch_bounds__(Cfg<P>::T) void c2r_kernel(RowArgs a)
     }
 }
 
-// DIF_M of the chirp filter b~ (b~_n = conj w_n for |n| < L, cyclic), kept in
-// the digit-reversed order the passes leave it in
+// Spectrum of the chirp filter b~ (b~_n = conj w_n for |n| < L, cyclic) in the
+// order the forward passes of convolve() leave data in
 template <int P>
 __global__ __launch_bounds__(Cfg<P>::T) void filter_kernel(int L, const cplx *w, const cplx *tw,
                                                            cplx *B)
 {
     constexpr int M = Cfg<P>::M, T = Cfg<P>::T;
+    constexpr int F = fft::plan_first(P), N8 = (P == 3) ? 0 : fft::plan_n8(P);
+    constexpr int LAST = (P == 3) ? 8 : 16;
     __shared__ cplx x[Cfg<P>::SLOTS];
     for (int n = threadIdx.x; n < M; n += T) {
         cplx v = fft::mk(0.0, 0.0);
@@ -174,8 +208,30 @@ __global__ __launch_bounds__(Cfg<P>::T) void filter_kernel(int L, const cplx *w,
         x[fft::slot(n)] = v;
     }
     __syncthreads();
-    dif_passes<P>(x, tw);
-    for (int e = threadIdx.x; e < M; e += T) B[e] = x[fft::slot(e)];
+    const int t = threadIdx.x;
+    if (F) {
+        for (int u = t; u < M / F; u += T) {
+            if (F == 2)
+                fft::dif_butterfly<2>(x, tw, M, M, u);
+            else
+                fft::dif_butterfly<4>(x, tw, M, M, u);
+        }
+        __syncthreads();
+    }
+    int S = F ? M / F : M;
+    for (int i = 0; i < N8; i++) {
+        for (int u = t; u < M / 8; u += T) fft::dif_butterfly<8>(x, tw, M, S, u);
+        S /= 8;
+        __syncthreads();
+    }
+    for (int u = t; u < M / LAST; u += T) {
+        cplx v[LAST];
+#pragma unroll
+        for (int r = 0; r < LAST; r++) v[r] = x[fft::slot(u * LAST + r)];
+        fft::dft<LAST, -1>(v);
+#pragma unroll
+        for (int r = 0; r < LAST; r++) B[u * LAST + r] = v[r];
+    }
 }
 
 __global__ void chirp_table_kernel(int L, cplx *w, cplx *pt)

@@ -115,12 +115,47 @@ FFT_HD void dft8(cplx *v)
     }
 }
 
+// 16 points as 4 x 4: DFTs of the 4 interleaved columns, twiddles
+// exp(SIGN 2 pi i n2 k1 / 16), DFTs of the rows
+template <int SIGN>
+FFT_HD void dft16(cplx *v)
+{
+    constexpr double C1 = 0.92387953251128675613;  // cos(pi/8)
+    constexpr double S1 = 0.38268343236508977173;  // sin(pi/8)
+    constexpr double H = 0.70710678118654752440;
+    // cos and sin of 2 pi m / 16 for m = 0..9
+    constexpr double CS[10][2] = {{1.0, 0.0}, {C1, S1}, {H, H},   {S1, C1},  {0.0, 1.0},
+                                  {-S1, C1},  {-H, H},  {-C1, S1}, {-1.0, 0.0}, {-C1, -S1}};
+    cplx y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) {
+        cplx c[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+        dft4<SIGN>(c);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) {
+            const int m = n2 * k1;
+            if (m == 0)
+                y[n2][k1] = c[k1];
+            else
+                y[n2][k1] = mul(c[k1], mk(CS[m][0], SIGN * CS[m][1]));
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+        cplx r[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+        dft4<SIGN>(r);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = r[k2];
+    }
+}
+
 template <int R, int SIGN>
 FFT_HD void dft(cplx *v)
 {
     if (R == 2) dft2<SIGN>(v);
     if (R == 4) dft4<SIGN>(v);
     if (R == 8) dft8<SIGN>(v);
+    if (R == 16) dft16<SIGN>(v);
 }
 
 // Twiddles w^r, r = 1..R-1, with w = tw[q] = exp(-2 pi i q / M) (tw has M
@@ -139,6 +174,65 @@ FFT_HD void twiddles(const cplx *tw, int q, int M, cplx *w)
         w[6] = mul(w[4], w[2]);
         w[7] = mul(w[4], w[3]);
     }
+    if (R >= 16) {
+        w[8] = tw[(8 * q) & (M - 1)];
+        for (int r = 9; r < 16; r++) w[r] = mul(w[8], w[r - 8]);
+    }
+}
+
+// The three table values a radix-8 pass needs: w^1, w^2, w^4
+struct Tw8 {
+    cplx w1, w2, w4;
+};
+FFT_HD Tw8 tw8_load(const cplx *tw, int q, int M)
+{
+    Tw8 t;
+    t.w1 = tw[q];
+    t.w2 = tw[(2 * q) & (M - 1)];
+    t.w4 = tw[(4 * q) & (M - 1)];
+    return t;
+}
+FFT_HD void tw8_expand(const Tw8 &t, cplx *w)
+{
+    w[1] = t.w1;
+    w[2] = t.w2;
+    w[3] = mul(t.w1, t.w2);
+    w[4] = t.w4;
+    w[5] = mul(t.w4, t.w1);
+    w[6] = mul(t.w4, t.w2);
+    w[7] = mul(t.w4, w[3]);
+}
+
+// Radix-8 butterflies with the twiddles already loaded (j = 0 gives w = 1)
+FFT_HD void dif_butterfly8(cplx *x, const Tw8 &t, int S, int b, int j)
+{
+    const int st = S / 8;
+    const int base = b * S + j;
+    cplx v[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) v[r] = x[slot(base + r * st)];
+    dft8<-1>(v);
+    cplx w[8];
+    tw8_expand(t, w);
+#pragma unroll
+    for (int r = 1; r < 8; r++) v[r] = mul(v[r], w[r]);
+#pragma unroll
+    for (int r = 0; r < 8; r++) x[slot(base + r * st)] = v[r];
+}
+FFT_HD void dit_butterfly8(cplx *x, const Tw8 &t, int S, int b, int j)
+{
+    const int st = S / 8;
+    const int base = b * S + j;
+    cplx v[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) v[r] = x[slot(base + r * st)];
+    cplx w[8];
+    tw8_expand(t, w);
+#pragma unroll
+    for (int r = 1; r < 8; r++) v[r] = mul(v[r], conj(w[r]));
+    dft8<+1>(v);
+#pragma unroll
+    for (int r = 0; r < 8; r++) x[slot(base + r * st)] = v[r];
 }
 
 // One butterfly of the decimation-in-frequency pass of radix R on blocks of
@@ -185,17 +279,33 @@ FFT_HD void dit_butterfly(cplx *x, const cplx *tw, int M, int S, int t)
     for (int r = 0; r < R; r++) x[slot(base + r * st)] = v[r];
 }
 
-// Radix of pass i (0 = first decimation-in-frequency pass, S = M) for
-// M = 2^p: radix 8 throughout, the remainder (2 or 4) in the last pass.
-FFT_HD int pass_count(int p)
+// Pass plan for M = 2^p (pass 0 = first decimation-in-frequency pass, on
+// blocks of S = M): an optional radix-2 or -4 pass, radix-8 passes, and a
+// final radix-16 pass on contiguous 16-point blocks (p = 3: one radix-8
+// pass).  Every pass but the last then has a stride S/R of at least 16, so
+// 16 consecutive lanes touch 16 consecutive elements, and in the last pass
+// lane b touches 16b..16b+15, i.e. slots 17b..17b+15: distinct banks either way.
+constexpr FFT_HD int plan_first(int p)
 {
-    return (p + 2) / 3;
+    if (p <= 4) return 0;
+    const int rem = (p - 4) % 3;
+    return rem == 0 ? 0 : (1 << rem);
 }
-FFT_HD int pass_radix(int p, int i)
+constexpr FFT_HD int plan_n8(int p)
 {
-    const int n8 = p / 3, rem = p - 3 * n8;
-    if (i < n8) return 8;
-    return rem == 1 ? 2 : 4;
+    return p == 3 ? 1 : (p <= 4 ? 0 : (p - 4) / 3);
+}
+constexpr FFT_HD int pass_count(int p)
+{
+    return (plan_first(p) ? 1 : 0) + plan_n8(p) + (p >= 4 ? 1 : 0);
+}
+constexpr FFT_HD int pass_radix(int p, int i)
+{
+    if (plan_first(p)) {
+        if (i == 0) return plan_first(p);
+        i--;
+    }
+    return i < plan_n8(p) ? 8 : 16;
 }
 
 // exp(SIGN i pi (n^2 mod 2L) / L) with the exponent reduced exactly

@@ -29,6 +29,7 @@ void dif(std::vector<cplx> &x, const HostPlan &p)
     for (int i = 0; i < pass_count(p.P); i++) {
         const int R = pass_radix(p.P, i);
         for (int t = 0; t < p.M / R; t++) {
+            if (R == 16) dif_butterfly<16>(x.data(), p.tw.data(), p.M, S, t);
             if (R == 8) dif_butterfly<8>(x.data(), p.tw.data(), p.M, S, t);
             if (R == 4) dif_butterfly<4>(x.data(), p.tw.data(), p.M, S, t);
             if (R == 2) dif_butterfly<2>(x.data(), p.tw.data(), p.M, S, t);
@@ -44,6 +45,7 @@ void dit(std::vector<cplx> &x, const HostPlan &p)
         for (int q = 0; q < i; q++) S /= pass_radix(p.P, q);
         const int R = pass_radix(p.P, i);
         for (int t = 0; t < p.M / R; t++) {
+            if (R == 16) dit_butterfly<16>(x.data(), p.tw.data(), p.M, S, t);
             if (R == 8) dit_butterfly<8>(x.data(), p.tw.data(), p.M, S, t);
             if (R == 4) dit_butterfly<4>(x.data(), p.tw.data(), p.M, S, t);
             if (R == 2) dit_butterfly<2>(x.data(), p.tw.data(), p.M, S, t);
