@@ -35,7 +35,9 @@ struct RowArgs {
 template <int P>
 struct Cfg {
     static constexpr int M = 1 << P;
+    // threads: one radix-8 butterfly each
     static constexpr int T = (M / 8 < 64) ? 64 : (M / 8 > 1024 ? 1024 : M / 8);
+    static constexpr int NB8 = (M / 8 + T - 1) / T;  // radix-8 butterflies per thread
     static constexpr int SLOTS = M + (M >> 4);
 };
 
@@ -55,15 +57,17 @@ __device__ __forceinline__ fft::Tw8 tw8_for(const cplx *tw, int M, int S, int t)
 template <int P>
 __device__ __forceinline__ void convolve(cplx *x, const cplx *B, const cplx *tw)
 {
-    constexpr int M = Cfg<P>::M, T = Cfg<P>::T;
+    constexpr int M = Cfg<P>::M, T = Cfg<P>::T, NB = Cfg<P>::NB8;
     constexpr int F = fft::plan_first(P), N8 = (P == 3) ? 0 : fft::plan_n8(P);
     constexpr int LAST = (P == 3) ? 8 : 16;  // radix of the fused middle pass
     const int t = threadIdx.x;
     constexpr int S8 = (F ? M / F : M);      // block size of the first radix-8 pass
 
     // forward
-    fft::Tw8 nxt{};
-    if (N8 > 0 && t < M / 8) nxt = tw8_for(tw, M, S8, t);
+    fft::Tw8 nxt[NB];
+#pragma unroll
+    for (int q = 0; q < NB; q++)
+        if (N8 > 0 && t + q * T < M / 8) nxt[q] = tw8_for(tw, M, S8, t + q * T);
     if (F) {
         for (int u = t; u < M / F; u += T) {
             if (F == 2)
@@ -76,17 +80,26 @@ __device__ __forceinline__ void convolve(cplx *x, const cplx *B, const cplx *tw)
     int S = S8;
 #pragma unroll
     for (int i = 0; i < N8; i++) {
-        const fft::Tw8 cur = nxt;
-        if (i + 1 < N8 && t < M / 8) nxt = tw8_for(tw, M, S / 8, t);
-        if (t < M / 8) {
-            const int st = S / 8, b = t / st, j = t - b * st;
-            fft::dif_butterfly8(x, cur, S, b, j);
+        fft::Tw8 cur[NB];
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int u = t + q * T;
+            cur[q] = nxt[q];
+            if (i + 1 < N8 && u < M / 8) nxt[q] = tw8_for(tw, M, S / 8, u);
+        }
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int u = t + q * T;
+            if (u < M / 8) {
+                const int st = S / 8, b = u / st, j = u - b * st;
+                fft::dif_butterfly8(x, cur[q], S, b, j);
+            }
         }
         S /= 8;
         __syncthreads();
     }
-    // middle: contiguous blocks of LAST points
-    if (N8 > 0 && t < M / 8) nxt = tw8_for(tw, M, LAST * 8, t);
+    // middle: contiguous blocks of LAST points (the twiddles of the first
+    // inverse pass are requested after it: nothing else stays live across it)
     for (int u = t; u < M / LAST; u += T) {
         cplx v[LAST];
 #pragma unroll
@@ -98,16 +111,28 @@ __device__ __forceinline__ void convolve(cplx *x, const cplx *B, const cplx *tw)
 #pragma unroll
         for (int r = 0; r < LAST; r++) x[fft::slot(u * LAST + r)] = v[r];
     }
+#pragma unroll
+    for (int q = 0; q < NB; q++)
+        if (N8 > 0 && t + q * T < M / 8) nxt[q] = tw8_for(tw, M, LAST * 8, t + q * T);
     __syncthreads();
     // inverse
     S = LAST * 8;
 #pragma unroll
     for (int i = N8 - 1; i >= 0; i--) {
-        const fft::Tw8 cur = nxt;
-        if (i > 0 && t < M / 8) nxt = tw8_for(tw, M, S * 8, t);
-        if (t < M / 8) {
-            const int st = S / 8, b = t / st, j = t - b * st;
-            fft::dit_butterfly8(x, cur, S, b, j);
+        fft::Tw8 cur[NB];
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int u = t + q * T;
+            cur[q] = nxt[q];
+            if (i > 0 && u < M / 8) nxt[q] = tw8_for(tw, M, S * 8, u);
+        }
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int u = t + q * T;
+            if (u < M / 8) {
+                const int st = S / 8, b = u / st, j = u - b * st;
+                fft::dit_butterfly8(x, cur[q], S, b, j);
+            }
         }
         S *= 8;
         __syncthreads();

@@ -876,12 +876,17 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
     rd.finish();
 }
 
+// Block b runs on XCD b mod 8 (round-robin dispatch).  XCD x takes the order
+// quad pairs p = 8g + x; its consecutive blocks alternate between the two
+// quads of a pair for each ring tile, so the two 64-byte halves of every
+// 128-byte spectrum line are read through the same L2.
 template <int K>
 __global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
 {
     __shared__ __align__(16) ShaSmem<K> sm;
     const int b = blockIdx.x;
-    sha_item<K>(A, (b >> 3) % A.ntile, ((b >> 3) / A.ntile) * 8 + (b & 7), sm);
+    const int s = b >> 3, h = s & 1, tile = (s >> 1) % A.ntile, g = (s >> 1) / A.ntile;
+    sha_item<K>(A, tile, 2 * (8 * g + (b & 7)) + h, sm);
 }
 
 template <int K>
@@ -1295,8 +1300,8 @@ int launch_lon_direct(int nlat, int nmax, size_t nlon, const double *lon, const 
 int launch_sha(const ShaArgs &a, void *stream)
 {
     const int nquad = (a.nmax + 1 + 3) / 4;
-    const int nqg = (nquad + 7) / 8;
-    const unsigned nblk = 8u * (unsigned)nqg * (unsigned)a.ntile;
+    const int nqg = (nquad + 15) / 16;  // groups of 8 quad pairs
+    const unsigned nblk = 16u * (unsigned)nqg * (unsigned)a.ntile;
     switch (a.k) {
     case 2: hipLaunchKernelGGL((sha_kernel<2>), dim3(nblk), dim3(256), 0, S(stream), a); break;
     case 3: hipLaunchKernelGGL((sha_kernel<3>), dim3(nblk), dim3(256), 0, S(stream), a); break;
