@@ -31,6 +31,11 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X vector FP64 (also its FP64 matrix peak)
 UNIT = "(nmax+1)^2*nlat/s"
+# HBM bytes per launch at nmax = 2160 (tuned ring tiles K = 3 for analysis,
+# K = 2 for synthesis), measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE in separate runs and scaled by a 256 MiB device copy measured in
+# the same runs (HBM bytes = 2.0 FETCH_SIZE + 1.0 WRITE_SIZE).
+TRAFFIC_GL2160 = {"sha_kernel": 408.4e6, "shs_grid_kernel": 166.1e6}
 
 
 def kernel_ms(lib, name):
@@ -135,7 +140,9 @@ def gl_roundtrip(args, N, with_sctr, with_cpu):
         "roundtrip_rel_err": rt_err,
         "roofline": {"bound": "mfma", "compute": "vector FP64 (VALU)", "kernel": dom,
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": args.traffic,
+                     "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": args.traffic if args.traffic is not None else
+                     (TRAFFIC_GL2160.get(dom) if N == 2160 else None),
                      "flops_per_launch": flops,
                      "frac_by_kernel": {k: flops / (avg[k] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
                                         for k in ("shs_grid_kernel", "sha_kernel")}},
