@@ -361,7 +361,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="gl2160",
                     choices=("gl2160", "gl10800", "batch", "serve"))
-    ap.add_argument("--cpu-rows", type=int, default=256)
+    ap.add_argument("--cpu-rows", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sctr", action="store_true")
     ap.add_argument("--sctr-points", type=int, default=1000000)
