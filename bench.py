@@ -12,6 +12,8 @@ Workloads (--workload):
   gl10800 the same round trip at nmax = 10800 (10801 x 21602 points).
   batch   64 requests, nmax ~ U{90..2700}, each with its own coefficients and
           GL grid, half SHS and half SHA: per-call vs one batched launch.
+  multi   256 coefficient sets at nmax = 720 on one GL grid (721 x 1442):
+          synthesis by the FP64 matrix-core contraction vs one call per set.
   serve   Poisson arrivals against a resident nmax = 2190 model (60% scattered
           SHS of 1e3..1e6 points, 30% GL SHS on grids nmax ~ U{180..5400}
           with the model truncated to the grid, 10% GL SHA nmax ~ U{180..2160});
@@ -287,6 +289,76 @@ def batch_bench(args):
     return line
 
 
+def multi_bench(args):
+    """C4b: nset coefficient sets at N on one shared GL grid, synthesis by the
+    FP64 matrix-core contraction vs one synthesis call per set."""
+    import numpy as np
+
+    import charm_amd
+    from charm_amd.device import DevArray, DeviceGrid, Stream
+    from charm_amd.workloads import GM, R_EARTH, kaula_coeffs
+
+    lib = charm_amd.lib()
+    N, nset = args.multi_nmax, args.multi_sets
+    grid = DeviceGrid(N, R_EARTH)
+    nlat, nlon = grid.nlat, grid.nlon
+    P = (N + 1) * (N + 2) // 2
+    rng = np.random.default_rng(720)
+    sig = np.zeros(P)
+    from charm_amd.workloads import degree_order
+    n_of, m_of = degree_order(N)
+    sig[n_of >= 2] = 1e-5 / n_of[n_of >= 2] ** 2.0
+    cs = rng.standard_normal((nset, P, 2)) * sig[None, :, None]
+    cs[:, :, 1][:, m_of == 0] = 0.0
+    cs[:, 0, 0] = 1.0
+    dcs = DevArray.from_numpy(cs)
+    out = DevArray((nset, nlat, nlon))
+    st = Stream()
+    for _ in range(args.warmup):
+        grid.shs_multi(dcs, N, nset, GM, R_EARTH, out, st)
+    st.synchronize()
+    lib.charm_amd_prof_reset()
+    lib.charm_amd_prof_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        grid.shs_multi(dcs, N, nset, GM, R_EARTH, out, st)
+    st.synchronize()
+    t_multi = (time.perf_counter() - t0) / args.steps
+    lib.charm_amd_prof_enable(0)
+    kms, kcnt = kernel_ms(lib, "shs_multi_kernel")
+    fms, fcnt = kernel_ms(lib, "fft_c2r_multi")
+    kavg, favg = kms / max(kcnt, 1), fms / max(fcnt, 1)
+    # one call per set on the same grid (device-resident)
+    views = [DevArray.from_numpy(cs[i]) for i in range(nset)]
+    f1 = DevArray((nlat, nlon))
+    grid.shs(views[0], N, GM, R_EARTH, f1, st)
+    st.synchronize()
+    t0 = time.perf_counter()
+    for i in range(nset):
+        grid.shs(views[i], N, GM, R_EARTH, f1, st)
+    st.synchronize()
+    t_single = time.perf_counter() - t0
+    got = out.numpy()[nset - 1]
+    ref = f1.numpy()
+    err = float(np.abs(got - ref).max() / np.abs(ref).max())
+    units = float(nset) * (N + 1) ** 2 * nlat
+    flops = 4.0 * nset * P * ((nlat + 1) // 2)
+    achieved = flops / (kavg * 1e-3) / 1e12
+    line = base_line("(nmax+1)^2*nlat/s FP64 SHS of %d sets on one GL grid (matrix cores)" % nset,
+                     units / t_multi, args.steps, args.warmup, 1e3 * t_multi,
+                     {"workload": "%d coefficient sets, nmax=%d, one GL grid %dx%d, r=R" % (nset, N, nlat, nlon),
+                      "sets": nset, "nmax": N, "parallelism": "single GPU"})
+    line.update({"kernel_avg_ms": {"shs_multi_kernel": kavg, "fft_c2r_multi": favg},
+                 "per_set_calls_units_per_s": units / t_single,
+                 "speedup_vs_per_set_calls": t_single / t_multi,
+                 "last_set_rel_diff_vs_single": err,
+                 "roofline": {"bound": "mfma", "compute": "FP64 matrix cores (v_mfma_f64_16x16x4f64)",
+                              "kernel": "shs_multi_kernel", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                              "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                              "flops_per_launch": flops}})
+    return line
+
+
 def serve_bench(args):
     import numpy as np
 
@@ -360,12 +432,14 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="gl2160",
-                    choices=("gl2160", "gl10800", "batch", "serve"))
+                    choices=("gl2160", "gl10800", "batch", "multi", "serve"))
     ap.add_argument("--cpu-rows", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sctr", action="store_true")
     ap.add_argument("--sctr-points", type=int, default=1000000)
     ap.add_argument("--check-rings", action="store_true")
+    ap.add_argument("--multi-sets", type=int, default=256)
+    ap.add_argument("--multi-nmax", type=int, default=720)
     ap.add_argument("--requests", type=int, default=40)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--inflight", type=int, default=4)
@@ -387,6 +461,8 @@ def main():
         line = gl_roundtrip(args, 10800, False, False)
     elif args.workload == "batch":
         line = batch_bench(args)
+    elif args.workload == "multi":
+        line = multi_bench(args)
     else:
         line = serve_bench(args)
     print(json.dumps(line), flush=True)

@@ -97,6 +97,32 @@ struct SctrArgs {
     int k;                // points per lane (1, 2, 4)
 };
 
+// Many coefficient sets on one grid (multi.hip): synthesis by FP64 MFMA.
+struct MultiShsArgs {
+    int nmax;
+    int nring;            // ring pairs
+    int nlat;
+    int nset;
+    int mcount;           // orders written per row (nfc; orders above nmax get zeros)
+    int nrt;              // ring tiles of 64 ring pairs
+    int npair;            // set tiles (16 sets) x ring tiles
+    const double *t;
+    const double *sx;
+    const int *six;
+    size_t sld;
+    const unsigned char *mirror;
+    const double *al;     // alpha_nm, degree al_nt
+    int al_nt;
+    const void *cs;       // (C s, S s) pairs, set-major, cs_stride pairs per set
+    size_t cs_stride;
+    void *spec;           // [set][row][lds] double2
+    size_t lds;
+    double mur;
+};
+int launch_shs_multi(const MultiShsArgs &a, void *stream);
+int launch_scale_cs_multi(int nmax, size_t nset, const void *cs_dev, const double *sn_dev, int nt,
+                          void *out_dev, void *stream);
+
 // Kernel launchers (kernels.hip).  All return hipError_t as int.
 // alpha_nm and s_nm of degree nt; ab_tmp holds tri_len(nt) double2 of scratch
 int launch_leg_tables(int nt, const double *r_dev, const double *ri_dev, void *ab_tmp,

@@ -1114,6 +1114,64 @@ void charm_amd_sha_grid_dev(const charm_amd_grid *grd, unsigned long nmax, doubl
     }
 }
 
+// nset coefficient sets (set-major, each an m-major triangle of interleaved
+// (C, S) pairs of degree nmax) -> nset grids [set][row][lon]; the sums over
+// degrees run as one FP64 matrix-core contraction per order.
+void charm_amd_shs_grid_multi_dev(const charm_amd_grid *grd, unsigned long nmax, size_t nset,
+                                  double mu, double R, const double *cs_dev, double *f_dev,
+                                  void *stream, charm_err *err)
+{
+    if (!grd || nmax > grd->nmax_grd) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG, "\"nmax\" exceeds the degree of the grid.");
+        return;
+    }
+    if (!nearly_equal(grd->r0, R, charm_glob_threshold)) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG,
+                      "Multi-set synthesis needs the grid radius to equal the reference radius.");
+        return;
+    }
+    if (nset == 0) return;
+    try {
+        hipStream_t s = (hipStream_t)stream;
+        Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
+        const size_t P = tri_len(nmax), nlat = grd->g.nlat, nlon = grd->g.nlon, nfc = nlon / 2 + 1;
+        DevBuf cs2(nset * P * 2 * sizeof(double), s);
+        HIPCHK((hipError_t)launch_scale_cs_multi((int)nmax, nset, cs_dev, tb.sn, tb.nt, cs2.p, s));
+        DevBuf spec(nset * nlat * nfc * 2 * sizeof(double), s);
+        MultiShsArgs a{};
+        a.nmax = (int)nmax;
+        a.nring = (int)grd->g.nring;
+        a.nlat = (int)nlat;
+        a.nset = (int)nset;
+        a.mcount = (int)nfc;
+        a.nrt = (int)((grd->g.nring + 63) / 64);
+        a.npair = (int)((nset + 15) / 16) * a.nrt;
+        a.t = grd->t;
+        a.sx = grd->sx;
+        a.six = grd->six;
+        a.sld = grd->sld;
+        a.mirror = grd->mirror;
+        a.al = tb.al;
+        a.al_nt = tb.nt;
+        a.cs = cs2.p;
+        a.cs_stride = P;
+        a.spec = spec.p;
+        a.lds = nfc;
+        a.mur = mu / R;
+        {
+            ProfScope ps("shs_multi_kernel", s);
+            HIPCHK((hipError_t)launch_shs_multi(a, s));
+        }
+        {
+            ProfScope ps("fft_c2r_multi", s);
+            rows_c2r(nlon, nset * nlat, spec.p, nfc, f_dev, s);
+        }
+        HIPCHK(hipGetLastError());
+    } catch (const HipFail &e) {
+        report(err, e, __func__);
+    }
+}
+
 void charm_amd_cs_truncate(const double *src_dev, unsigned long nmax_src, double *dst_dev,
                            unsigned long nmax, void *stream, charm_err *err)
 {

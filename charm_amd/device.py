@@ -141,6 +141,15 @@ class DeviceGrid:
               self._h, int(nmax), float(mu), float(R), f.ptr, out.ptr, _s(stream))
         return out
 
+    def shs_multi(self, cs, nmax, nset, mu, R, out, stream=None):
+        """Synthesis of ``nset`` coefficient sets ``cs`` (nset, P, 2) into
+        ``out`` (nset, nlat, nlon), the sums over degrees on the FP64 matrix
+        cores; the grid radius must equal ``R``."""
+        _call("charm_amd_shs_grid_multi_dev",
+              [_vp, _ct.c_ulong, _ct.c_size_t, _ct.c_double, _ct.c_double, _vp, _vp, _vp],
+              self._h, int(nmax), int(nset), float(mu), float(R), cs.ptr, out.ptr, _s(stream))
+        return out
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

@@ -191,6 +191,16 @@ void charm_amd_shs_sctr_dev(size_t npoint, const double *lat_dev, const double *
                             const double *cs_dev, double *f_dev, void *stream,
                             charm_err *err);
 
+/* Synthesis of nset coefficient sets on one device-resident GL grid (no
+ * counterpart in CHarm, where each set is a separate charm_shs_point call,
+ * src/shs/shs.h.in:168-173).  cs_dev: nset consecutive m-major triangles of
+ * interleaved (C, S) pairs of degree nmax; f_dev: nset grids [set][row][lon].
+ * The grid radius must equal R.  The sums over degrees run on the FP64
+ * matrix cores. */
+void charm_amd_shs_grid_multi_dev(const charm_amd_grid *grd, unsigned long nmax, size_t nset,
+                                  double mu, double R, const double *cs_dev, double *f_dev,
+                                  void *stream, charm_err *err);
+
 /* Copy the (C, S) pairs of a device model of degree nmax_src, cut to degree
  * nmax <= nmax_src, into dst_dev (the layout the *_dev functions take). */
 void charm_amd_cs_truncate(const double *src_dev, unsigned long nmax_src, double *dst_dev,
