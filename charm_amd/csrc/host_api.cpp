@@ -1,6 +1,7 @@
 // Host-side data types of the C ABI: charm_err, charm_shc, charm_point,
 // Gauss-Legendre grids and the global tunables.  Written for this library;
 // behaviour (argument checks, ownership, error codes) follows CHarm 0.4.9.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -22,6 +23,13 @@ double charm_glob_polar_optimization_a2 = -1.0;
 }
 
 namespace charm_amd {
+
+double polar_threshold(unsigned long nmax)
+{
+    if (charm_glob_polar_optimization_a2 < 0.0) return -1.0;
+    return std::max((double)charm_glob_polar_optimization_a1,
+                    charm_glob_polar_optimization_a2 * (double)nmax);
+}
 
 // misc_is_nearly_equal (src/misc/misc_is_nearly_equal.c:16-32)
 bool nearly_equal(double a, double b, double eps)

@@ -57,6 +57,8 @@ struct ShsGridArgs {
     double mur;        // mu / R
     int fft_layout;    // 1: out = Hermitian half-spectrum for c2r; 0: raw (a, b)
     int k;             // ring pairs per lane (1..3)
+    const double *u;   // [nring] cos(lat)
+    double polar;      // polar optimisation threshold (< 0: off)
 };
 
 struct ShaArgs {
@@ -76,6 +78,8 @@ struct ShaArgs {
     int al_nt;
     void *part;        // [tile][tri_len(nmax)] double2 partial sums (before s_nm)
     int k;             // ring pairs per lane (2..4)
+    const double *u;   // [nring] cos(lat)
+    double polar;      // polar optimisation threshold (< 0: off)
 };
 
 struct SctrArgs {
@@ -95,6 +99,7 @@ struct SctrArgs {
     const void *cs;       // (C_nm s_nm, S_nm s_nm), m-major, degree nmax
     double *part;         // [chunk * npoint + i]
     int k;                // points per lane (1, 2, 4)
+    double polar;         // polar optimisation threshold (< 0: off)
 };
 
 // Many coefficient sets on one grid (multi.hip): synthesis by FP64 MFMA.
@@ -118,6 +123,8 @@ struct MultiShsArgs {
     void *spec;           // [set][row][lds] double2
     size_t lds;
     double mur;
+    const double *u;      // [nring] cos(lat)
+    double polar;         // polar optimisation threshold (< 0: off)
 };
 int launch_shs_multi(const MultiShsArgs &a, void *stream);
 int launch_scale_cs_multi(int nmax, size_t nset, const void *cs_dev, const double *sn_dev, int nt,
@@ -170,6 +177,11 @@ bool row_fft_supported(size_t n);
 int launch_row_fft(int c2r, size_t n, size_t nrow, double *f, size_t ldf, void *spec, size_t lds,
                    void *stream);
 int prepare_row_fft(size_t n, void *stream);
+
+// Polar optimisation threshold for a transform of degree nmax from the
+// globals: max(a1, a2 nmax), or -1 (off) when a2 < 0
+// (src/misc/misc_polar_optimization_threshold.c:16-20).
+double polar_threshold(unsigned long nmax);
 
 // Ring pairs per lane for nring rings, kmin <= k <= kmax: the fewest idle
 // lanes in the last ring tile, then the most rings per lane.

@@ -312,7 +312,17 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
 #pragma unroll
     for (int k = 0; k < K; k++) aN[k] = bN[k] = aS[k] = bS[k] = 0.0;
 
-    if (m <= nmax) {
+    // polar optimisation: order m is dropped at rings with m - nmax u > polar
+    // (src/misc/misc_polar_optimization_apply.c:16-41, tested per ring here)
+    bool skip[K], allskip = true;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        skip[k] = !valid[k] || (A.polar >= 0.0 && (double)m - (double)nmax * A.u[ring[k]] > A.polar);
+        allskip = allskip && skip[k];
+    }
+    allskip = wave_all(allskip);
+
+    if (m <= nmax && !allskip) {
         const double2 *__restrict__ csm = (const double2 *)A.cs + dcol_off(nmax, m) - m;
         double x[K], y[K];
         int e[K];
@@ -368,11 +378,11 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 if (m == 0) {
-                    x[k] = valid[k] ? 1.0 : 0.0;
+                    x[k] = skip[k] ? 0.0 : 1.0;
                     e[k] = 0;
                 } else {
-                    x[k] = valid[k] ? A.sx[(size_t)(m - 1) * A.sld + ring[k]] : 0.0;
-                    e[k] = valid[k] ? 960 * A.six[(size_t)(m - 1) * A.sld + ring[k]] : 0;
+                    x[k] = skip[k] ? 0.0 : A.sx[(size_t)(m - 1) * A.sld + ring[k]];
+                    e[k] = skip[k] ? 0 : 960 * A.six[(size_t)(m - 1) * A.sld + ring[k]];
                 }
                 y[k] = 0.0;
                 SHS_ACC(k, ldexp(x[k], e[k]), cm, 0);
@@ -729,6 +739,21 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
     if (m > nmax) return;
 
     double *dst = (double *)((double2 *)A.part + (size_t)tile * tri_len_dev(nmax) + dcol_off(nmax, m) - m);
+    // polar optimisation: order m is dropped at rings with m - nmax u > polar
+    // (src/misc/misc_polar_optimization_apply.c:16-41, tested per ring here)
+    bool skip[K], allskip = true;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        skip[k] = !valid[k] || (A.polar >= 0.0 && (double)m - (double)nmax * A.u[ring[k]] > A.polar);
+        allskip = allskip && skip[k];
+    }
+    if (wave_all(allskip)) {
+        for (int n = m + lane; n <= nmax; n += 64) {
+            dst[2 * n] = 0.0;
+            dst[2 * n + 1] = 0.0;
+        }
+        return;
+    }
     WaveReducer rd;
     rd.red = lds + wave * 16 * RED_LD;
     rd.dst = nullptr;
@@ -743,11 +768,11 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
 #pragma unroll
     for (int k = 0; k < K; k++) {
         if (m == 0) {
-            x[k] = valid[k] ? 1.0 : 0.0;
+            x[k] = skip[k] ? 0.0 : 1.0;
             e[k] = 0;
         } else {
-            x[k] = valid[k] ? A.sx[(size_t)(m - 1) * A.sld + ring[k]] : 0.0;
-            e[k] = valid[k] ? 960 * A.six[(size_t)(m - 1) * A.sld + ring[k]] : 0;
+            x[k] = skip[k] ? 0.0 : A.sx[(size_t)(m - 1) * A.sld + ring[k]];
+            e[k] = skip[k] ? 0 : 960 * A.six[(size_t)(m - 1) * A.sld + ring[k]];
         }
         y[k] = 0.0;
         const double p0 = ldexp(x[k], e[k]);
@@ -1001,6 +1026,15 @@ __global__ __launch_bounds__(256) void sctr_kernel(SctrArgs A)
         double x[K], y[K], a[K], b[K];
         int e[K];
         const double2 cm = csm[m];
+        // polar optimisation (src/misc/misc_polar_optimization_apply.c:16-41,
+        // tested per point here)
+        bool skip[K], allskip = true;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            skip[k] = !valid[k] || (A.polar >= 0.0 && (double)m - (double)nmax * u[k] > A.polar);
+            allskip = allskip && skip[k];
+        }
+        allskip = wave_all(allskip);
 #pragma unroll
         for (int k = 0; k < K; k++) {
             // Q~_m = P_mm (R/r)^(m+1) as a mantissa and a binary exponent
@@ -1012,14 +1046,14 @@ __global__ __launch_bounds__(256) void sctr_kernel(SctrArgs A)
                 xm_ *= rm;
                 em += er;
             }
-            x[k] = valid[k] ? xm_ : 0.0;
-            e[k] = valid[k] ? em : 0;
+            x[k] = skip[k] ? 0.0 : xm_;
+            e[k] = skip[k] ? 0 : em;
             y[k] = 0.0;
             const double p = ldexp(x[k], e[k]);
             a[k] = p * cm.x;
             b[k] = p * cm.y;
         }
-        if (m < nmax) {
+        if (m < nmax && !allskip) {
             st.gal = A.al + dcol_off(A.al_nt, m) - m;
             st.gcs = csm;
             st.start(m + 1);

@@ -106,15 +106,18 @@ __global__ __launch_bounds__(64) void shs_multi_kernel(MultiShsArgs A)
 #pragma unroll
             for (int p = 0; p < 2; p++) acc[q][c][p] = (d4){0.0, 0.0, 0.0, 0.0};
 
-    if (m <= nmax) {
+    // polar optimisation: order m is dropped at rings with m - nmax u > polar
+    // (src/misc/misc_polar_optimization_apply.c:16-41, tested per ring here)
+    const bool skip = !valid || (A.polar >= 0.0 && (double)m - (double)nmax * A.u[ring] > A.polar);
+    if (m <= nmax && !all64(skip)) {
         const double t = valid ? A.t[ring] : 0.0;
         double xq, yq = 0.0;
         int e = 0;
         if (m == 0) {
-            xq = valid ? 1.0 : 0.0;
+            xq = skip ? 0.0 : 1.0;
         } else {
-            xq = valid ? A.sx[(size_t)(m - 1) * A.sld + ring] : 0.0;
-            e = valid ? 960 * A.six[(size_t)(m - 1) * A.sld + ring] : 0;
+            xq = skip ? 0.0 : A.sx[(size_t)(m - 1) * A.sld + ring];
+            e = skip ? 0 : 960 * A.six[(size_t)(m - 1) * A.sld + ring];
         }
         const double *alm = A.al + mcol_off(A.al_nt, m) - m;
         // coefficient block loader: lane -> set set0 + lane/4, degrees 4(lane%4)..+3

@@ -483,6 +483,8 @@ void run_shs_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev, bo
     a.mirror = d.mirror;
     a.al = tb.al;
     a.al_nt = tb.nt;
+    a.u = d.u;
+    a.polar = polar_threshold(nmax);
     // coefficients times s_nm for the scaled recurrence
     DevBuf cs2(tri_len(nmax) * 2 * sizeof(double), stream);
     HIPCHK((hipError_t)launch_scale_cs((int)nmax, cs_dev, tb.sn, tb.nt, cs2.p, stream));
@@ -676,6 +678,7 @@ void run_sctr(size_t npoint, const double *lat_dev, const double *lon_dev, const
     a.cs = cs2.p;
     a.part = part.as<double>();
     a.k = K;
+    a.polar = polar_threshold(nmax);
     {
         ProfScope ps("sctr_kernel", s);
         HIPCHK((hipError_t)launch_sctr(a, s));
@@ -769,6 +772,8 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
     a.lds = nfc;
     a.al = tb.al;
     a.al_nt = tb.nt;
+    a.u = d.u;
+    a.polar = polar_threshold(nmax);
     // partial sums sized for the smallest tile width (most tiles)
     DevBuf part((size_t)ring_tiles(g.nring, 2) * tri_len(nmax) * 2 * sizeof(double), s);
     a.part = part.p;
@@ -1158,6 +1163,8 @@ void charm_amd_shs_grid_multi_dev(const charm_amd_grid *grd, unsigned long nmax,
         a.spec = spec.p;
         a.lds = nfc;
         a.mur = mu / R;
+        a.u = grd->u;
+        a.polar = polar_threshold(nmax);
         {
             ProfScope ps("shs_multi_kernel", s);
             HIPCHK((hipError_t)launch_shs_multi(a, s));
@@ -1334,6 +1341,8 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
                 a.mirror = g->mirror;
                 a.al = tb.al;
                 a.al_nt = tb.nt;
+                a.u = g->u;
+                a.polar = polar_threshold(q.nmax);
                 void *cs2 = dmalloc(b, tri_len(q.nmax) * 2 * sizeof(double));
                 a.cs = cs2;
                 a.mur = q.mu / q.R;
@@ -1385,6 +1394,8 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
                 a.lds = nfc;
                 a.al = tb.al;
                 a.al_nt = tb.nt;
+                a.u = g->u;
+                a.polar = polar_threshold(q.nmax);
                 void *part = dmalloc(b, (size_t)a.ntile * tri_len(q.nmax) * 2 * sizeof(double));
                 a.part = part;
                 std::vector<double> sc;

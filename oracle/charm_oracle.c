@@ -31,6 +31,12 @@
  *   src/crd/crd_point_gl_chunk.c:34-234, crd_grd_check_symm.c:21-72,
  *   crd_point_quad_equator.c:22-53
  *   src/misc/misc_is_nearly_equal.c:16-32, src/glob/glob.c:16-23
+ *   src/misc/misc_polar_optimization_threshold.c:16-20,
+ *   misc_polar_optimization_apply.c:16-41 (polar optimisation, off by
+ *   default; CHarm tests the criterion jointly for a block of SIMD_SIZE x
+ *   BLOCK_S points, a size that depends on its build, and skips the order for
+ *   the block when every point passes; here it is tested per ring or point.
+ *   Parity unpinned: no CHarm test or fixture covers the polar optimisation.)
  */
 #include <math.h>
 #include <stdint.h>
@@ -393,6 +399,28 @@ static int fft_ok(size_t nlon, const real *lon, unsigned long nmax)
 /* ------------------------------------------------------------------------ */
 /* Public oracle API                                                        */
 /* ------------------------------------------------------------------------ */
+/* Polar optimisation (glob_polar_optimization_a1/a2): order m is skipped at a
+ * point with cos(lat) = u when m - nmax u > max(a1, a2 nmax); a2 < 0 turns it
+ * off (misc_polar_optimization_threshold.c:16-20,
+ * misc_polar_optimization_apply.c:16-41). */
+static unsigned long polar_a1 = 100;
+static double polar_a2 = -1.0;
+
+void orc_set_polar(unsigned long a1, double a2)
+{
+    polar_a1 = a1;
+    polar_a2 = a2;
+}
+
+static int polar_skip(unsigned long m, unsigned long nmax, real u)
+{
+    if (polar_a2 < 0.0) return 0;
+    real thr = (real)polar_a1;
+    real t2 = (real)polar_a2 * (real)nmax;
+    if (t2 > thr) thr = t2;
+    return thr < (real)m - (real)nmax * u;
+}
+
 int orc_ncomp(void) { return NCOMP; }
 
 /* Gauss-Legendre grid (crd_point_gl_chunk.c:105-229).  lat, w: nlat = nmax+1
@@ -509,6 +537,7 @@ int orc_shs_grid(unsigned long nmax, unsigned long nmax_shc, const double *C_i,
         leg_seed(u, dm, nmax, ps, ips);
         rpows(R, rr[i], unit, nmax + 1, rp);
         for (unsigned long m = 0; m <= nmax; m++) {
+            if (polar_skip(m, nmax, u)) { a[m] = 0; b[m] = 0; continue; }
             leg_anm_bnm(nmax, m, rt, rit, anm, bnm);
             size_t off = col_off(nmax_shc, m);
             lumped_ring(nmax, m, C + off, S + off, anm, bnm, t, ps, ips, rp, NULL,
@@ -559,10 +588,12 @@ int orc_lumped(unsigned long nmax, unsigned long nmax_shc, const double *C_i,
     rpows(R, r, nearly_equal(r, R, GLOB_THRESHOLD), nmax + 1, rp);
     real t = RSIN(lat);
     for (unsigned long m = 0; m <= nmax; m++) {
-        real a, b;
-        leg_anm_bnm(nmax, m, rt, rit, anm, bnm);
-        size_t off = col_off(nmax_shc, m);
-        lumped_ring(nmax, m, C + off, S + off, anm, bnm, t, ps, ips, rp, NULL, &a, &b, NULL, NULL);
+        real a = 0, b = 0;
+        if (!polar_skip(m, nmax, RCOS(lat))) {
+            leg_anm_bnm(nmax, m, rt, rit, anm, bnm);
+            size_t off = col_off(nmax_shc, m);
+            lumped_ring(nmax, m, C + off, S + off, anm, bnm, t, ps, ips, rp, NULL, &a, &b, NULL, NULL);
+        }
         st(a_o, nmax + 1, m, a);
         st(b_o, nmax + 1, m, b);
     }
@@ -604,6 +635,7 @@ int orc_shs_sctr(unsigned long nmax, unsigned long nmax_shc, const double *C_i,
         real fi = 0;
         for (unsigned long m = 0; m <= nmax; m++) {
             real a, b;
+            if (polar_skip(m, nmax, u)) continue;
             leg_anm_bnm(nmax, m, rt, rit, anm, bnm);
             size_t off = col_off(nmax_shc, m);
             lumped_ring(nmax, m, C + off, S + off, anm, bnm, t, ps, ips, rp, NULL, &a, &b, NULL, NULL);
@@ -694,6 +726,7 @@ int orc_sha_gl(unsigned long nmax, unsigned long nmax_shc, size_t nlat, size_t n
         leg_anm_bnm(nmax, m, rt, rit, anm, bnm);
         real *Cm = C + col_off(nmax_shc, m), *Sm = S + col_off(nmax_shc, m);
         for (size_t i = 0; i < nlatdo; i++) {
+            if (polar_skip(m, nmax, uu[i])) continue;
             real t = tt[i];
             size_t q = m * nlatdo + i;
             real ap = amp[q], am = amm[q], bp = bmp[q], bm = bmm[q];

@@ -52,6 +52,7 @@ class Oracle:
         L.orc_shs_sctr.argtypes = [_ul, _ul, _dp, _dp, _dp, _dp, _sz, _dp, _dp, _dp, _dp]
         L.orc_sha_gl.argtypes = [_ul, _ul, _sz, _sz, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_leg_pnm.argtypes = [_ul, _dp, _dp]
+        L.orc_set_polar.argtypes = [_ul, ct.c_double]
 
     # -- layout helpers ------------------------------------------------------
     def pack(self, x):
@@ -98,6 +99,10 @@ class Oracle:
         return a.ctypes.data_as(_dp)
 
     # -- API -------------------------------------------------------------------
+    def set_polar(self, a1, a2):
+        """Polar optimisation constants (a2 < 0: off, the default)."""
+        self.lib.orc_set_polar(int(a1), float(a2))
+
     def gl_grid(self, nmax, exact=False):
         L = nmax + 1
         lat, w, lon = self.empty(L), self.empty(L), self.empty(2 * L)
