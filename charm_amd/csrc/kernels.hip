@@ -695,6 +695,10 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
     const int m = m0 + wave;
     const int nmax = A.nmax;
     double *lds = sm.buf;
+    // each tile's transpose reads are issued before the next tile's
+    // arithmetic and summed after it, which hides the LDS latency (an
+    // immediate sum frees 32 VGPRs but measured slower for every K)
+    constexpr bool PIPE = true;
 
     // stage the weighted spectra A = cw Re F, B = -cw Im F of the tile's rows
     // for orders m0..m0+3 (64-byte runs per row)
@@ -781,6 +785,10 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
     }
     rd.put(0, v0, v1);
     rd.commit(1, dst + 2 * m);
+    if (!PIPE) {
+        rd.begin();
+        rd.finish();
+    }
     if (m == nmax) {
         rd.begin();
         rd.finish();
@@ -808,7 +816,7 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
             deep = deep && (!valid[k] || lane_deep(x[k], y[k], e[k]));
         }
         if (!wave_any(anyx)) break;
-        rd.begin();
+        if (PIPE) rd.begin();
         if (wave_all(deep)) {
             // every term of the tile is exactly zero
 #pragma unroll
@@ -821,7 +829,7 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
                     x[k] = z;
                 }
             }
-            rd.finish();
+            if (PIPE) rd.finish();
             if (lane < 16) dst[2 * n + lane] = 0.0;
         } else {
 #pragma unroll
@@ -839,8 +847,12 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
                 }
                 rd.put(j, c, sv);
             }
-            rd.finish();
+            if (PIPE) rd.finish();
             rd.commit(8, dst + 2 * n);
+            if (!PIPE) {
+                rd.begin();
+                rd.finish();
+            }
         }
 #pragma unroll
         for (int k = 0; k < K; k++) lane_renorm(x[k], y[k], e[k]);
@@ -849,7 +861,7 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
     for (; n + 7 <= nmax; n += 8) {
         if (n - st.n0 == 64) st.next();
         const int o = n - st.n0;
-        rd.begin();
+        if (PIPE) rd.begin();
 #pragma unroll
         for (int j = 0; j < 8; j += 2) {
             const double2 a2 = *(const double2 *)(st.al + o + j);
@@ -868,15 +880,19 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
             rd.put(j, c0, s0);
             rd.put(j + 1, c1, s1);
         }
-        rd.finish();
+        if (PIPE) rd.finish();
         rd.commit(8, dst + 2 * n);
+        if (!PIPE) {
+            rd.begin();
+            rd.finish();
+        }
     }
     // last, partial tile
     if (n <= nmax) {
         if (n - st.n0 == 64) st.next();
         const int o = n - st.n0;
         const int cnt = nmax - n + 1;
-        rd.begin();
+        if (PIPE) rd.begin();
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             double c = 0.0, sv = 0.0;
@@ -894,7 +910,7 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
             }
             rd.put(j, c, sv);
         }
-        rd.finish();
+        if (PIPE) rd.finish();
         rd.commit(cnt, dst + 2 * n);
     }
     rd.begin();
@@ -906,7 +922,8 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
 // quads of a pair for each ring tile, so the two 64-byte halves of every
 // 128-byte spectrum line are read through the same L2.
 template <int K>
-__global__ __launch_bounds__(256) void sha_kernel(ShaArgs A)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 3 ? 4 : 3)))
+void sha_kernel(ShaArgs A)
 {
     __shared__ __align__(16) ShaSmem<K> sm;
     const int b = blockIdx.x;

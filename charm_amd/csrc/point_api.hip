@@ -448,6 +448,9 @@ int tuned_k(int kind, int nring, unsigned long nmax, int kmin, int kmax, hipStre
         HIPCHK(hipEventSynchronize(b));
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, a, b));
+        if (std::getenv("CHARM_AMD_TUNE_LOG"))
+            std::fprintf(stderr, "charm_amd tune: kind %d nring %d nmax %lu K=%d %.3f ms\n", kind,
+                         nring, nmax, k, (double)ms);
         if (k == kmin || ms < best_ms) {
             best_ms = ms;
             best = k;
