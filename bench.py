@@ -149,21 +149,6 @@ def gl_roundtrip(args, N, with_sctr, with_cpu):
                      "frac_by_kernel": {k: flops / (avg[k] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
                                         for k in ("shs_grid_kernel", "sha_kernel")}},
     })
-    if args.check_rings:
-        # oracle on sampled rings: polar (X-number region), mid-latitude, equator
-        from oracle.oracle import Oracle
-
-        grid.shs(cs, N, GM, R_EARTH, f, stream)
-        stream.synchronize()
-        f_np = f.numpy()
-        rings = np.array([0, 1, nlat // 7, nlat // 2, nlat - 1], dtype=np.int64)
-        pnt = grid.pnt
-        t0 = time.perf_counter()
-        ref = Oracle("d").shs_grid(N, C, S, GM, R_EARTH, pnt.lat, pnt.r, pnt.lon, gl=True,
-                                   rings=rings)
-        line["shs_rel_err_vs_oracle_sampled_rings"] = float(
-            np.abs(f_np[rings] - ref).max() / np.abs(ref).max())
-        line["oracle_check_s"] = time.perf_counter() - t0
     del f, out
     if with_sctr:
         line.update(sctr_measure(N, cs, args.sctr_points, stream))
@@ -437,7 +422,6 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sctr", action="store_true")
     ap.add_argument("--sctr-points", type=int, default=1000000)
-    ap.add_argument("--check-rings", action="store_true")
     ap.add_argument("--multi-sets", type=int, default=256)
     ap.add_argument("--multi-nmax", type=int, default=720)
     ap.add_argument("--requests", type=int, default=40)

@@ -622,12 +622,15 @@ int orc_shs_sctr(unsigned long nmax, unsigned long nmax_shc, const double *C_i,
     real *rit = (real *)malloc((2 * nmax + 4) * sizeof(real));
     real *dm = (real *)malloc((nmax + 1) * sizeof(real));
     leg_tables(nmax, rt, rit, dm);
+    const real mur = mu / R;
+#pragma omp parallel
+    {
     real *anm = (real *)malloc((nmax + 2) * sizeof(real));
     real *bnm = (real *)malloc((nmax + 2) * sizeof(real));
     real *ps = (real *)malloc((nmax + 1) * sizeof(real));
     int64_t *ips = (int64_t *)malloc((nmax + 1) * sizeof(int64_t));
     real *rp = (real *)malloc((nmax + 3) * sizeof(real));
-    const real mur = mu / R;
+#pragma omp for schedule(dynamic)
     for (size_t i = 0; i < npoint; i++) {
         real t = RSIN(lat[i]), u = RCOS(lat[i]);
         leg_seed(u, dm, nmax, ps, ips);
@@ -644,8 +647,9 @@ int orc_shs_sctr(unsigned long nmax, unsigned long nmax_shc, const double *C_i,
         }
         st(f_o, npoint, i, mur * fi);
     }
-    free(C); free(S); free(lat); free(lon); free(rr); free(rt); free(rit); free(dm);
     free(anm); free(bnm); free(ps); free(ips); free(rp);
+    }
+    free(C); free(S); free(lat); free(lon); free(rr); free(rt); free(rit); free(dm);
     return 0;
 }
 
