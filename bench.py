@@ -12,6 +12,8 @@ Workloads (--workload):
   gl10800 the same round trip at nmax = 10800 (10801 x 21602 points).
   batch   64 requests, nmax ~ U{90..2700}, each with its own coefficients and
           GL grid, half SHS and half SHA: per-call vs one batched launch.
+          Beside it (key sctr_batch): 32 scattered SHS requests, nmax ~
+          U{90..2700}, 1e3..1e5 points each, per-call vs batched.
   multi   256 coefficient sets at nmax = 720 on one GL grid (721 x 1442):
           synthesis by the FP64 matrix-core contraction vs one call per set.
   serve   Poisson arrivals against a resident nmax = 2190 model (60% scattered
@@ -271,7 +273,56 @@ def batch_bench(args):
                  "host_api_per_call_s": t_host,
                  "speedup_vs_host_api_per_call": t_host / best,
                  "batch_vs_alone_max_rel_diff": err})
+    line.update(batch_sctr_bench(args))
     return line
+
+
+def batch_sctr_bench(args):
+    """32 scattered synthesis requests, nmax ~ U{90..2700}, 1e3..1e5 points
+    each (log-uniform), r = R + U(0, 9000 m): one call per request vs one
+    batch.  The batch sorts each point set once, at creation."""
+    import numpy as np
+
+    from charm_amd.device import SHS_SCTR, Batch, DevArray, Stream, shs_sctr
+    from charm_amd.workloads import GM, R_EARTH, kaula_coeffs, scattered_points
+
+    rng = np.random.default_rng(65)
+    degs = rng.integers(90, 2701, 32)
+    npts = np.round(10.0 ** rng.uniform(3, 5, 32)).astype(int)
+    st = Stream()
+    reqs = []
+    units = 0.0
+    for i, (N, P) in enumerate(zip(degs, npts)):
+        N, P = int(N), int(P)
+        C, S = kaula_coeffs(N, 2000 + i)
+        pts = tuple(DevArray.from_numpy(a) for a in scattered_points(P, 3000 + i))
+        reqs.append((SHS_SCTR, N, GM, R_EARTH, pts, DevArray.from_numpy(np.stack([C, S], axis=1)),
+                     DevArray(P)))
+        units += float(N + 1) ** 2 * P
+
+    def per_call():
+        for kind, N, mu, R, pts, cs, f in reqs:
+            shs_sctr(*pts, cs, N, mu, R, f, st)
+        st.synchronize()
+
+    def timed(fn, reps):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) / reps
+
+    t_call = timed(per_call, args.steps)
+    ref = [r[6].numpy() for r in reqs]
+    b = Batch(reqs, use_graph=True)
+    t_batch = timed(lambda: (b.run(st), st.synchronize()), args.steps)
+    same = all(np.array_equal(x, r[6].numpy()) for x, r in zip(ref, reqs))
+    print("# batch: timed scattered per-call and batched", file=sys.stderr, flush=True)
+    return {"sctr_batch": {"requests": 32, "points": int(npts.sum()),
+                           "per_call_units_per_s": units / t_call,
+                           "batch_graph_units_per_s": units / t_batch,
+                           "speedup_vs_per_call_dev": t_call / t_batch,
+                           "graph_used": b.uses_graph, "bitwise_equal_to_alone": bool(same)}}
 
 
 def multi_bench(args):

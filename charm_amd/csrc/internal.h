@@ -161,13 +161,17 @@ int launch_sincos(size_t n, const double *lat_dev, double *t_dev, double *u_dev,
 int launch_zero_ints(int *p, int n, void *stream);
 int launch_cs_truncate(const double *src_dev, unsigned long nmax_src, double *dst_dev,
                        unsigned long nmax, void *stream);
-// Persistent batch kernels: kind 0 = synthesis, 1 = analysis.
+// Persistent batch kernels: kind 0 = grid synthesis, 1 = analysis,
+// 2 = scattered synthesis.
 int batch_ring_k(int kind);
 int batch_grid_size(int kind);
 int launch_shs_batch(const void *reqs_dev, const void *items_dev, int nitems, int *counter,
                      int grid, void *stream);
 int launch_sha_batch(const void *reqs_dev, const void *items_dev, int nitems, int *counter,
                      int grid, void *stream);
+// kind 2 = scattered synthesis; items (request, point block << 6 | chunk)
+int launch_sctr_batch(const void *reqs_dev, const void *items_dev, int nitems, int *counter,
+                      int grid, void *stream);
 int launch_all_near(size_t n, const double *x_dev, double ref, double eps, int *result,
                     void *stream);
 

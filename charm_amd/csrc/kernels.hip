@@ -994,15 +994,18 @@ __global__ void sha_reduce_kernel(int nmax, int ntile, const double2 *__restrict
 // in the lane.  With r != R the factor (R/r)^(n+1) is carried by the
 // recurrence itself: Q~_n = alpha_n (t q) Q~_{n-1} - q^2 Q~_{n-2}, q = R/r.
 // ---------------------------------------------------------------------------
+struct SctrSmem {
+    double al[4][64];
+    double2 cs[4][64];
+};
+
+// block blk of 256 K points, order chunk `chunk`
 template <int K, bool UNIT>
-__global__ __launch_bounds__(256) void sctr_kernel(SctrArgs A)
+__device__ __forceinline__ void sctr_item(const SctrArgs &A, int blk, int chunk, SctrSmem &sm)
 {
-    __shared__ __align__(16) double s_al[4][64];
-    __shared__ __align__(16) double2 s_cs[4][64];
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
-    const size_t base = ((size_t)blockIdx.x * 4 + wave) * 64 * K;
-    const int chunk = blockIdx.y;
+    const size_t base = ((size_t)blk * 4 + wave) * 64 * K;
     const int mb = A.mchunk[chunk], me = A.mchunk[chunk + 1];
     const int nmax = A.nmax;
 
@@ -1034,8 +1037,8 @@ __global__ __launch_bounds__(256) void sctr_kernel(SctrArgs A)
     }
 
     ColStage<true> st;
-    st.al = s_al[wave];
-    st.cs = s_cs[wave];
+    st.al = sm.al[wave];
+    st.cs = sm.cs[wave];
     st.nlast = nmax;
     st.lane = lane;
     for (int m = mb; m < me; m++) {
@@ -1183,6 +1186,39 @@ __global__ __launch_bounds__(256) void sctr_kernel(SctrArgs A)
     for (int k = 0; k < K; k++) {
         const size_t i = base + k * 64 + lane;
         if (valid[k]) A.part[(size_t)chunk * A.npoint + i] = f[k];
+    }
+}
+
+template <int K, bool UNIT>
+__global__ __launch_bounds__(256) void sctr_kernel(SctrArgs A)
+{
+    __shared__ __align__(16) SctrSmem sm;
+    sctr_item<K, UNIT>(A, blockIdx.x, blockIdx.y, sm);
+}
+
+// Persistent form: blocks pull (request, point block, order chunk) items,
+// packed as (request, block << 6 | chunk), from a list sorted by decreasing
+// cost until the list is exhausted.
+template <int K>
+__global__ __launch_bounds__(256) void sctr_batch_kernel(const SctrArgs *__restrict__ reqs,
+                                                         const int2 *__restrict__ items, int nitems,
+                                                         int *counter)
+{
+    __shared__ __align__(16) SctrSmem sm;
+    __shared__ int cur;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) cur = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = uniform(cur);
+        if (it >= nitems) break;
+        const int2 w = items[it];
+        const SctrArgs A = reqs[uniform(w.x)];
+        const int wy = uniform(w.y);
+        if (A.unit)
+            sctr_item<K, true>(A, wy >> 6, wy & 63, sm);
+        else
+            sctr_item<K, false>(A, wy >> 6, wy & 63, sm);
     }
 }
 
@@ -1500,10 +1536,11 @@ int launch_zero_ints(int *p, int n, void *stream)
 
 constexpr int BATCH_SHS_K = 2;
 constexpr int BATCH_SHA_K = 3;
+constexpr int BATCH_SCTR_K = 2;
 
 int batch_ring_k(int kind)
 {
-    return kind == 0 ? BATCH_SHS_K : BATCH_SHA_K;
+    return kind == 0 ? BATCH_SHS_K : (kind == 1 ? BATCH_SHA_K : BATCH_SCTR_K);
 }
 
 // Resident blocks per CU of a 256-thread kernel on gfx950: 8 wave slots per
@@ -1527,8 +1564,9 @@ int batch_grid_size(int kind)
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    const int per = (kind == 0) ? resident_blocks(shs_batch_kernel<BATCH_SHS_K>)
-                                : resident_blocks(sha_batch_kernel<BATCH_SHA_K>);
+    const int per = (kind == 0)   ? resident_blocks(shs_batch_kernel<BATCH_SHS_K>)
+                    : (kind == 1) ? resident_blocks(sha_batch_kernel<BATCH_SHA_K>)
+                                  : resident_blocks(sctr_batch_kernel<BATCH_SCTR_K>);
     return ncu * per;
 }
 
@@ -1538,6 +1576,15 @@ int launch_shs_batch(const void *reqs, const void *items, int nitems, int *count
     if (nitems <= 0) return 0;
     hipLaunchKernelGGL((shs_batch_kernel<BATCH_SHS_K>), dim3(grid), dim3(256), 0, S(stream),
                        (const ShsGridArgs *)reqs, (const int2 *)items, nitems, counter);
+    return (int)hipGetLastError();
+}
+
+int launch_sctr_batch(const void *reqs, const void *items, int nitems, int *counter, int grid,
+                      void *stream)
+{
+    if (nitems <= 0) return 0;
+    hipLaunchKernelGGL((sctr_batch_kernel<BATCH_SCTR_K>), dim3(grid), dim3(256), 0, S(stream),
+                       (const SctrArgs *)reqs, (const int2 *)items, nitems, counter);
     return (int)hipGetLastError();
 }
 

@@ -619,6 +619,27 @@ int sctr_k()
     return 2;
 }
 
+// Order chunks of equal work (boundaries m, nchunk + 1 entries): more chunks
+// when there are few points, so that the grid still fills the device
+std::vector<int> sctr_chunks(size_t npoint, int K, unsigned long nmax)
+{
+    const size_t nwave = (npoint + 64 * K - 1) / (64 * K);
+    int nchunk = (int)std::min<size_t>(64, std::max<size_t>(1, 4096 / std::max<size_t>(nwave, 1)));
+    nchunk = (int)std::min<unsigned long>((unsigned long)nchunk, nmax + 1);
+    std::vector<int> mc(nchunk + 1);
+    const double total = 0.5 * (double)(nmax + 1) * (double)(nmax + 2);
+    double acc = 0.0;
+    int c = 1;
+    mc[0] = 0;
+    for (unsigned long m = 0; m <= nmax && c < nchunk; m++) {
+        acc += (double)(nmax + 1 - m);
+        if (acc >= total * c / nchunk) mc[c++] = (int)m + 1;
+    }
+    while (c < nchunk) mc[c++] = (int)nmax + 1;
+    mc[nchunk] = (int)nmax + 1;
+    return mc;
+}
+
 // Scattered synthesis on device arrays: lat/lon/r of npoint points, unscaled
 // coefficients cs (interleaved pairs, degree nmax).  Points are processed in
 // the order of |sin lat| (results go back to the caller's order).
@@ -644,23 +665,8 @@ void run_sctr(size_t npoint, const double *lat_dev, const double *lon_dev, const
     } else {
         HIPCHK((hipError_t)launch_sincos(npoint, lat_dev, dt.as<double>(), du.as<double>(), s));
     }
-    // split orders into chunks of equal work when there are few points
-    const size_t nwave = (npoint + 64 * K - 1) / (64 * K);
-    int nchunk = (int)std::min<size_t>(64, std::max<size_t>(1, 4096 / std::max<size_t>(nwave, 1)));
-    nchunk = (int)std::min<unsigned long>((unsigned long)nchunk, nmax + 1);
-    std::vector<int> mc(nchunk + 1);
-    {
-        const double total = 0.5 * (double)(nmax + 1) * (double)(nmax + 2);
-        double acc = 0.0;
-        int c = 1;
-        mc[0] = 0;
-        for (unsigned long m = 0; m <= nmax && c < nchunk; m++) {
-            acc += (double)(nmax + 1 - m);
-            if (acc >= total * c / nchunk) mc[c++] = (int)m + 1;
-        }
-        while (c < nchunk) mc[c++] = (int)nmax + 1;
-        mc[nchunk] = (int)nmax + 1;
-    }
+    const std::vector<int> mc = sctr_chunks(npoint, K, nmax);
+    const int nchunk = (int)mc.size() - 1;
     DevBuf dmc((nchunk + 1) * sizeof(int), s);
     h2d(dmc.as<int>(), mc.data(), nchunk + 1, s);
     DevBuf part((size_t)nchunk * npoint * sizeof(double), s);
@@ -1238,6 +1244,18 @@ struct charm_amd_batch {
     std::vector<Job> shs, sha;
     const double *sn = nullptr;  // s_nm table (degree nt) the batch was built with
     int nt = 0;
+    struct SJob {
+        size_t r;
+        void *cs2;     // coefficients times s_nm
+        double *part;  // [chunk][point]
+        int nchunk;
+        const int *perm;
+    };
+    std::vector<SJob> sct;
+    SctrArgs *d_sct = nullptr;
+    int2 *d_sct_items = nullptr;
+    int n_sct_items = 0;
+    int grid_sct = 0;
     int grid_shs = 0, grid_sha = 0;
     bool use_graph = false;
     hipGraphExec_t exec = nullptr;
@@ -1265,7 +1283,7 @@ T *dupload(charm_amd_batch *b, const std::vector<T> &v, hipStream_t s)
 
 void batch_enqueue(charm_amd_batch *b, hipStream_t s)
 {
-    HIPCHK((hipError_t)launch_zero_ints(b->d_counter, 2, s));
+    HIPCHK((hipError_t)launch_zero_ints(b->d_counter, 3, s));
     for (const auto &j : b->sha) {
         const charm_amd_request &q = b->req[j.r];
         rows_r2c(j.nlon, j.nlat, q.f, j.spec, j.nfc, s);
@@ -1287,6 +1305,16 @@ void batch_enqueue(charm_amd_batch *b, hipStream_t s)
         const charm_amd_request &q = b->req[j.r];
         rows_c2r(j.nlon, j.nlat, j.spec, j.nfc, q.f, s);
     }
+    for (const auto &j : b->sct) {
+        const charm_amd_request &q = b->req[j.r];
+        HIPCHK((hipError_t)launch_scale_cs((int)q.nmax, q.cs, b->sn, b->nt, j.cs2, s));
+    }
+    HIPCHK((hipError_t)launch_sctr_batch(b->d_sct, b->d_sct_items, b->n_sct_items, b->d_counter + 2,
+                                         b->grid_sct, s));
+    for (const auto &j : b->sct) {
+        const charm_amd_request &q = b->req[j.r];
+        HIPCHK((hipError_t)launch_sctr_reduce(q.npoint, j.nchunk, j.part, j.perm, q.mu / q.R, q.f, s));
+    }
 }
 
 }  // namespace
@@ -1299,8 +1327,13 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
 {
     for (size_t i = 0; i < nreq; i++) {
         const charm_amd_request &q = reqs[i];
-        if (!q.grid || (q.kind != CHARM_AMD_SHS_GRID && q.kind != CHARM_AMD_SHA_GRID) ||
-            q.nmax > q.grid->nmax_grd || !q.cs || !q.f) {
+        const bool sct = q.kind == CHARM_AMD_SHS_SCTR;
+        const bool bad =
+            sct ? (!q.lat || !q.lon || !q.r || q.npoint == 0 || q.npoint >= (size_t)INT32_MAX ||
+                   (q.npoint + 511) / 512 >= (1u << 24))
+                : (!q.grid || (q.kind != CHARM_AMD_SHS_GRID && q.kind != CHARM_AMD_SHA_GRID) ||
+                   q.nmax > q.grid->nmax_grd);
+        if (bad || !q.cs || !q.f) {
             char msg[CHARM_ERR_MAX_MSG];
             std::snprintf(msg, sizeof msg, "Invalid batch request %zu.", i);
             CHARM_AMD_ERR(err, CHARM_EFUNCARG, msg);
@@ -1324,8 +1357,56 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
             int2 w;
         };
         std::vector<Item> shs_items, sha_items;
+        std::vector<SctrArgs> sct_args;
+        std::vector<Item> sct_items;
         for (size_t i = 0; i < nreq; i++) {
             const charm_amd_request &q = b->req[i];
+            if (q.kind == CHARM_AMD_SHS_SCTR) {
+                // points ordered by latitude once, as charm_amd_shs_sctr_dev does per call
+                const size_t np = q.npoint;
+                const int K = batch_ring_k(2);
+                int *perm = (int *)dmalloc(b, np * sizeof(int));
+                double *pt = (double *)dmalloc(b, np * sizeof(double));
+                double *pu = (double *)dmalloc(b, np * sizeof(double));
+                double *pl = (double *)dmalloc(b, np * sizeof(double));
+                double *pr = (double *)dmalloc(b, np * sizeof(double));
+                HIPCHK((hipError_t)launch_sort_points(np, q.lat, q.lon, q.r, perm, pt, pu, pl, pr, s));
+                int unit = 0;
+                HIPCHK((hipError_t)launch_all_near(np, q.r, q.R, charm_glob_threshold, &unit, s));
+                const std::vector<int> mc = sctr_chunks(np, K, q.nmax);
+                const int nchunk = (int)mc.size() - 1;
+                SctrArgs a{};
+                a.nmax = (int)q.nmax;
+                a.npoint = np;
+                a.nchunk = nchunk;
+                a.mchunk = dupload(b, mc, s);
+                a.unit = unit != 0;
+                a.R = q.R;
+                a.lat_t = pt;
+                a.lat_u = pu;
+                a.lon = pl;
+                a.r = pr;
+                a.dm = tb.dm;
+                a.al = tb.al;
+                a.al_nt = tb.nt;
+                void *cs2 = dmalloc(b, tri_len(q.nmax) * 2 * sizeof(double));
+                a.cs = cs2;
+                double *part = (double *)dmalloc(b, (size_t)nchunk * np * sizeof(double));
+                a.part = part;
+                a.k = K;
+                a.polar = polar_threshold(q.nmax);
+                const int idx = (int)sct_args.size();
+                sct_args.push_back(a);
+                b->sct.push_back({i, cs2, part, nchunk, perm});
+                const int nblk = (int)((np + 256 * K - 1) / (256 * K));
+                for (int c = 0; c < nchunk; c++) {
+                    long work = 0;
+                    for (int m = mc[c]; m < mc[c + 1]; m++) work += (long)q.nmax + 1 - m;
+                    for (int bk = 0; bk < nblk; bk++)
+                        sct_items.push_back({work, make_int2(idx, (bk << 6) | c)});
+                }
+                continue;
+            }
             const charm_amd_grid *g = q.grid;
             const size_t nlat = g->g.nlat, nlon = g->g.nlon, nfc = nlon / 2 + 1;
             const int nmax = (int)q.nmax;
@@ -1421,13 +1502,18 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
             return w;
         };
         std::vector<int2> wi = sort_items(shs_items), wa = sort_items(sha_items);
+        std::vector<int2> wsc = sort_items(sct_items);
+        b->n_sct_items = (int)wsc.size();
+        if (!sct_args.empty()) b->d_sct = dupload(b, sct_args, s);
+        if (!wsc.empty()) b->d_sct_items = dupload(b, wsc, s);
+        b->grid_sct = batch_grid_size(2);
         b->n_shs_items = (int)wi.size();
         b->n_sha_items = (int)wa.size();
         if (!shs_args.empty()) b->d_shs = dupload(b, shs_args, s);
         if (!sha_args.empty()) b->d_sha = dupload(b, sha_args, s);
         if (!wi.empty()) b->d_shs_items = dupload(b, wi, s);
         if (!wa.empty()) b->d_sha_items = dupload(b, wa, s);
-        b->d_counter = (int *)dmalloc(b, 2 * sizeof(int));
+        b->d_counter = (int *)dmalloc(b, 3 * sizeof(int));
         b->grid_shs = batch_grid_size(0);
         b->grid_sha = batch_grid_size(1);
         HIPCHK(hipStreamSynchronize(s));

@@ -78,6 +78,13 @@ class DevArray:
         _check(hip().hipMemcpy(d.ptr, a.ctypes.data_as(_vp), d.nbytes, 1), "hipMemcpy H2D")
         return d
 
+    def copy_from_numpy(self, a):
+        """Overwrite the contents with the host array ``a`` (same size)."""
+        a = _np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch: %d bytes into %d" % (a.nbytes, self.nbytes))
+        _check(hip().hipMemcpy(self.ptr, a.ctypes.data_as(_vp), self.nbytes, 1), "hipMemcpy H2D")
+
     def numpy(self):
         a = _np.empty(self.shape, dtype=self.dtype)
         _check(hip().hipMemcpy(a.ctypes.data_as(_vp), self.ptr, self.nbytes, 2), "hipMemcpy D2H")
@@ -171,6 +178,7 @@ def shs_sctr(lat, lon, r, cs, nmax, mu, R, out, stream=None):
 
 SHS_GRID = 0
 SHA_GRID = 1
+SHS_SCTR = 2
 
 
 class _Request(_ct.Structure):
@@ -180,21 +188,33 @@ class _Request(_ct.Structure):
                 ("R", _ct.c_double),
                 ("grid", _vp),
                 ("cs", _vp),
-                ("f", _vp)]
+                ("f", _vp),
+                ("npoint", _ct.c_size_t),
+                ("lat", _vp),
+                ("lon", _vp),
+                ("r", _vp)]
 
 
 class Batch:
     """Many independent grid requests executed together (one persistent
     launch per transform kind), optionally replayed as a HIP graph.
 
-    ``requests``: sequence of (kind, nmax, mu, R, DeviceGrid, cs DevArray,
-    f DevArray) with kind SHS_GRID (cs -> f) or SHA_GRID (f -> cs)."""
+    ``requests``: sequence of (kind, nmax, mu, R, where, cs DevArray,
+    f DevArray) with kind SHS_GRID (cs -> f on the DeviceGrid ``where``),
+    SHA_GRID (f -> cs on the DeviceGrid ``where``) or SHS_SCTR (cs -> f at the
+    points ``where`` = (lat, lon, r) DevArrays, read once at creation)."""
 
     def __init__(self, requests, use_graph=True):
         self._keep = list(requests)
         arr = (_Request * len(self._keep))()
-        for i, (kind, nmax, mu, R, grid, cs, f) in enumerate(self._keep):
-            arr[i] = _Request(int(kind), int(nmax), float(mu), float(R), grid._h, cs.ptr, f.ptr)
+        for i, (kind, nmax, mu, R, where, cs, f) in enumerate(self._keep):
+            if int(kind) == SHS_SCTR:
+                lat, lon, r = where
+                arr[i] = _Request(int(kind), int(nmax), float(mu), float(R), None, cs.ptr, f.ptr,
+                                  lat.size, lat.ptr, lon.ptr, r.ptr)
+            else:
+                arr[i] = _Request(int(kind), int(nmax), float(mu), float(R), where._h, cs.ptr,
+                                  f.ptr, 0, None, None, None)
         fn = lib().charm_amd_batch_create
         fn.restype = _vp
         fn.argtypes = [_ct.POINTER(_Request), _ct.c_size_t, _ct.c_int, _ct.POINTER(_err._Err)]

@@ -210,10 +210,13 @@ void charm_amd_cs_truncate(const double *src_dev, unsigned long nmax_src, double
  * transform kind, optionally replayed as a HIP graph.  Synthesis requests
  * (kind CHARM_AMD_SHS_GRID) read cs and write f; analysis requests
  * (CHARM_AMD_SHA_GRID) read f and write cs; layouts as for
- * charm_amd_shs_grid_dev / charm_amd_sha_grid_dev.  All pointers are device
- * memory and must stay valid for the life of the batch; their contents may
- * change between runs. */
-enum { CHARM_AMD_SHS_GRID = 0, CHARM_AMD_SHA_GRID = 1 };
+ * charm_amd_shs_grid_dev / charm_amd_sha_grid_dev.  Scattered synthesis
+ * requests (CHARM_AMD_SHS_SCTR) read cs and write f[npoint] at the points
+ * lat/lon/r, as charm_amd_shs_sctr_dev (grid unused); their coordinates are
+ * read when the batch is created.  All pointers are device memory and must
+ * stay valid for the life of the batch; the contents of cs and f may change
+ * between runs. */
+enum { CHARM_AMD_SHS_GRID = 0, CHARM_AMD_SHA_GRID = 1, CHARM_AMD_SHS_SCTR = 2 };
 
 typedef struct {
     int kind;
@@ -223,6 +226,8 @@ typedef struct {
     const charm_amd_grid *grid;
     double *cs;
     double *f;
+    size_t npoint;            /* CHARM_AMD_SHS_SCTR only */
+    const double *lat, *lon, *r;
 } charm_amd_request;
 
 typedef struct charm_amd_batch charm_amd_batch;

@@ -41,6 +41,27 @@ def test_struct_layouts_match_charm():
     assert charm_amd._err._Err.msg.offset == 40
 
 
+def test_request_layout_matches_header(tmp_path):
+    """The ctypes mirror of charm_amd_request has the header's offsets."""
+    from charm_amd import device
+    fields = [f for f, _ in device._Request._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "charm_amd.h"\n'
+                   "int main(void) {\n"
+                   '    printf("%zu", sizeof(charm_amd_request));\n'
+                   + "".join('    printf(" %%zu", offsetof(charm_amd_request, %s));\n' % f
+                             for f in fields)
+                   + "    return 0;\n}\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    want = [ct.sizeof(device._Request)] + [getattr(device._Request, f).offset for f in fields]
+    assert got == want
+
+
 def test_globals_default_values():
     lib = charm_amd.lib()
     eps = np.finfo(float).eps
