@@ -1,13 +1,15 @@
 // Longitude transforms of Gauss-Legendre rows (nlon = 2L points per row):
 // r2c (analysis) and c2r (synthesis) with hipFFT's unnormalised conventions,
-// as one complex DFT of length L per row evaluated by Bluestein's chirp
-// convolution of length M = 2^P <= 8192 entirely in LDS, one row per
-// work-group (src/shs/shs_grd_fft.c:99-130 and src/sha/sha_point.c:490-553
-// use FFTW's r2c/c2r for the same transforms).
+// as one complex DFT of length L per row, one row per work-group, entirely
+// in LDS: mixed-radix passes directly when L = 2^a 3^b 5^c 7^d, else Rader's
+// cyclic convolution of length L - 1 (L prime, L - 1 smooth) or Bluestein's
+// chirp convolution over a smooth length M >= 2L - 1, whichever is cheaper
+// (src/shs/shs_grd_fft.c:99-130 and src/sha/sha_point.c:490-553 use FFTW's
+// r2c/c2r for the same transforms).
 #include <hip/hip_runtime.h>
 
-#include <map>
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -21,10 +23,13 @@ using fft::cplx;
 namespace {
 
 struct RowArgs {
+    fft::MixPlan plan;
     int L;
-    const cplx *w;    // [L]   exp(-i pi n^2 / L)
+    const cplx *w;    // [L]   exp(-i pi n^2 / L) (Bluestein)
     const cplx *pt;   // [L+1] exp(-i pi k / L)
-    const cplx *B;    // [M]   DIF_M of the chirp filter (digit-reversed order)
+    const cplx *B;    // [M]   chirp filter spectrum, digit-reversed (Bluestein)
+    const int *rev;   // [L]   position of output k after the passes (direct)
+    const int *lg;    // [L]   log_g n (Rader)
     const cplx *tw;   // [M]   exp(-2 pi i q / M)
     double *f;        // rows of 2L reals
     size_t ldf;       // doubles between rows of f
@@ -32,231 +37,280 @@ struct RowArgs {
     size_t lds;       // complex values between rows of spec
 };
 
-template <int P>
-struct Cfg {
-    static constexpr int M = 1 << P;
-    // threads: one radix-8 butterfly each
-    static constexpr int T = (M / 8 < 64) ? 64 : (M / 8 > 1024 ? 1024 : M / 8);
-    static constexpr int NB8 = (M / 8 + T - 1) / T;  // radix-8 butterflies per thread
-    static constexpr int SLOTS = M + (M >> 4);
-};
-
-// Twiddle indices of radix-8 pass on blocks of S points for butterfly t
-__device__ __forceinline__ fft::Tw8 tw8_for(const cplx *tw, int M, int S, int t)
+template <int R, bool INV, bool POW2>
+__device__ __forceinline__ void run_pass_g(cplx *x, const fft::TwSplit &tw, int M,
+                                           const fft::PassGeom &g)
 {
-    const int st = S / 8;
-    const int j = t % st;
-    return fft::tw8_load(tw, j * (M / S), M);
+    for (int u = threadIdx.x; u < M / R; u += blockDim.x) {
+        if (INV)
+            fft::dit_butterfly<R, POW2>(x, tw, g, u);
+        else
+            fft::dif_butterfly<R, POW2>(x, tw, g, u);
+    }
 }
 
-// conv = M * (a * b~) in natural order from a in LDS (natural order):
-// decimation-in-frequency passes down to 16-point blocks; per block the last
-// forward pass, the product with B and the first inverse pass in registers;
-// decimation-in-time passes back up.  The twiddles of each radix-8 pass are
-// requested before the barrier that precedes it.
-template <int P>
-__device__ __forceinline__ void convolve(cplx *x, const cplx *B, const cplx *tw)
+template <int R, bool INV>
+__device__ __forceinline__ void run_pass(cplx *x, const fft::TwSplit &tw, int M, int S)
 {
-    constexpr int M = Cfg<P>::M, T = Cfg<P>::T, NB = Cfg<P>::NB8;
-    constexpr int F = fft::plan_first(P), N8 = (P == 3) ? 0 : fft::plan_n8(P);
-    constexpr int LAST = (P == 3) ? 8 : 16;  // radix of the fused middle pass
-    const int t = threadIdx.x;
-    constexpr int S8 = (F ? M / F : M);      // block size of the first radix-8 pass
+    const fft::PassGeom g = fft::pass_geom(M, S, R);
+    if (g.sh >= 0)
+        run_pass_g<R, INV, true>(x, tw, M, g);
+    else
+        run_pass_g<R, INV, false>(x, tw, M, g);
+}
 
-    // forward
-    fft::Tw8 nxt[NB];
-#pragma unroll
-    for (int q = 0; q < NB; q++)
-        if (N8 > 0 && t + q * T < M / 8) nxt[q] = tw8_for(tw, M, S8, t + q * T);
-    if (F) {
-        for (int u = t; u < M / F; u += T) {
-            if (F == 2)
-                fft::dif_butterfly<2>(x, tw, M, M, u);
-            else
-                fft::dif_butterfly<4>(x, tw, M, M, u);
-        }
+template <bool INV>
+__device__ __forceinline__ void pass(cplx *x, const fft::TwSplit &tw, int M, int S, int R)
+{
+    switch (R) {
+    case 2: run_pass<2, INV>(x, tw, M, S); break;
+    case 3: run_pass<3, INV>(x, tw, M, S); break;
+    case 4: run_pass<4, INV>(x, tw, M, S); break;
+    case 5: run_pass<5, INV>(x, tw, M, S); break;
+    case 7: run_pass<7, INV>(x, tw, M, S); break;
+    case 8: run_pass<8, INV>(x, tw, M, S); break;
+    default: run_pass<9, INV>(x, tw, M, S); break;
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void run_last(cplx *x, int M)
+{
+    for (int u = threadIdx.x; u < M / R; u += blockDim.x) fft::last_block<R>(x, u);
+}
+
+template <int R>
+__device__ __forceinline__ void run_middle(cplx *x, const cplx *B, int M)
+{
+    for (int u = threadIdx.x; u < M / R; u += blockDim.x) fft::middle_block<R>(x, B, u);
+}
+
+// Forward passes 0..np-2 (twiddled), leaving blocks of R[np-1] points
+__device__ __forceinline__ void forward_head(cplx *x, const fft::TwSplit &tw, const fft::MixPlan &p)
+{
+    int S = p.M;
+    for (int i = 0; i + 1 < p.np; i++) {
+        pass<false>(x, tw, p.M, S, p.R[i]);
+        S /= p.R[i];
         __syncthreads();
     }
-    int S = S8;
-#pragma unroll
-    for (int i = 0; i < N8; i++) {
-        fft::Tw8 cur[NB];
-#pragma unroll
-        for (int q = 0; q < NB; q++) {
-            const int u = t + q * T;
-            cur[q] = nxt[q];
-            if (i + 1 < N8 && u < M / 8) nxt[q] = tw8_for(tw, M, S / 8, u);
-        }
-#pragma unroll
-        for (int q = 0; q < NB; q++) {
-            const int u = t + q * T;
-            if (u < M / 8) {
-                const int st = S / 8, b = u / st, j = u - b * st;
-                fft::dif_butterfly8(x, cur[q], S, b, j);
-            }
-        }
-        S /= 8;
-        __syncthreads();
+}
+
+// The full forward DFT (direct plans): output k at position rev[k]
+__device__ __forceinline__ void forward_dft(cplx *x, const fft::TwSplit &tw, const fft::MixPlan &p)
+{
+    if (p.np == 0) return;
+    forward_head(x, tw, p);
+    switch (p.R[p.np - 1]) {
+    case 2: run_last<2>(x, p.M); break;
+    case 3: run_last<3>(x, p.M); break;
+    case 4: run_last<4>(x, p.M); break;
+    case 5: run_last<5>(x, p.M); break;
+    case 7: run_last<7>(x, p.M); break;
+    case 8: run_last<8>(x, p.M); break;
+    case 9: run_last<9>(x, p.M); break;
+    default: run_last<16>(x, p.M); break;
     }
-    // middle: contiguous blocks of LAST points (the twiddles of the first
-    // inverse pass are requested after it: nothing else stays live across it)
-    for (int u = t; u < M / LAST; u += T) {
-        cplx v[LAST];
-#pragma unroll
-        for (int r = 0; r < LAST; r++) v[r] = x[fft::slot(u * LAST + r)];
-        fft::dft<LAST, -1>(v);
-#pragma unroll
-        for (int r = 0; r < LAST; r++) v[r] = fft::mul(v[r], B[u * LAST + r]);
-        fft::dft<LAST, +1>(v);
-#pragma unroll
-        for (int r = 0; r < LAST; r++) x[fft::slot(u * LAST + r)] = v[r];
-    }
-#pragma unroll
-    for (int q = 0; q < NB; q++)
-        if (N8 > 0 && t + q * T < M / 8) nxt[q] = tw8_for(tw, M, LAST * 8, t + q * T);
     __syncthreads();
-    // inverse
-    S = LAST * 8;
-#pragma unroll
-    for (int i = N8 - 1; i >= 0; i--) {
-        fft::Tw8 cur[NB];
-#pragma unroll
-        for (int q = 0; q < NB; q++) {
-            const int u = t + q * T;
-            cur[q] = nxt[q];
-            if (i > 0 && u < M / 8) nxt[q] = tw8_for(tw, M, S * 8, u);
-        }
-#pragma unroll
-        for (int q = 0; q < NB; q++) {
-            const int u = t + q * T;
-            if (u < M / 8) {
-                const int st = S / 8, b = u / st, j = u - b * st;
-                fft::dit_butterfly8(x, cur[q], S, b, j);
-            }
-        }
-        S *= 8;
+}
+
+// conv = M (a * b~) in natural order from a in natural order (Bluestein)
+__device__ __forceinline__ void convolve(cplx *x, const cplx *B, const fft::TwSplit &tw,
+                                         const fft::MixPlan &p)
+{
+    forward_head(x, tw, p);
+    switch (p.R[p.np - 1]) {
+    case 2: run_middle<2>(x, B, p.M); break;
+    case 3: run_middle<3>(x, B, p.M); break;
+    case 4: run_middle<4>(x, B, p.M); break;
+    case 5: run_middle<5>(x, B, p.M); break;
+    case 7: run_middle<7>(x, B, p.M); break;
+    case 8: run_middle<8>(x, B, p.M); break;
+    case 9: run_middle<9>(x, B, p.M); break;
+    default: run_middle<16>(x, B, p.M); break;
+    }
+    __syncthreads();
+    // inverse passes np-2..0 on blocks of S_i = M / (R[0] ... R[i-1])
+    int S = p.R[p.np - 1];
+    for (int i = p.np - 2; i >= 0; i--) {
+        S *= p.R[i];
+        pass<true>(x, tw, p.M, S, p.R[i]);
         __syncthreads();
     }
-    if (F) {
-        for (int u = t; u < M / F; u += T) {
-            if (F == 2)
-                fft::dit_butterfly<2>(x, tw, M, M, u);
-            else
-                fft::dit_butterfly<4>(x, tw, M, M, u);
-        }
-        __syncthreads();
+}
+
+extern __shared__ cplx lds_rows[];
+
+constexpr int NWSUM = 16;  // per-wave partial sums (Rader), after the tables
+
+// The row's slots, then the twiddle tables, in LDS; the tables are filled
+// from the global table (visible after the caller's first barrier)
+__device__ __forceinline__ fft::TwSplit lds_twiddles(const cplx *tw, int M)
+{
+    cplx *hi = lds_rows + fft::slots(M);
+    const int nh = fft::tw_hi_count(M);
+    cplx *lo = hi + nh;
+    for (int i = threadIdx.x; i < nh + fft::TW_LO; i += blockDim.x) {
+        if (i < nh)
+            hi[i] = tw[i * fft::TW_LO];
+        else if (i - nh < M)
+            lo[i - nh] = tw[i - nh];
     }
+    fft::TwSplit t;
+    t.hi = hi;
+    t.lo = lo;
+    return t;
+}
+
+__device__ __forceinline__ cplx* wave_sums(int M)
+{
+    return lds_rows + fft::slots(M) + fft::tw_hi_count(M) + fft::TW_LO;
+}
+
+__device__ __forceinline__ cplx wave_reduce(cplx v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v.x += __shfl_xor(v.x, o);
+        v.y += __shfl_xor(v.y, o);
+    }
+    return v;
+}
+
+// Loads the L inputs v(n) of a row's DFT into LDS in the order MODE needs
+// (Rader: point n >= 1 at slot(log_g n), the sum of all points per wave
+// beside the tables) and runs the transform.  get(n) gives input n.
+template <int MODE, class Get>
+__device__ __forceinline__ void row_dft(const RowArgs &a, cplx *x, const fft::TwSplit &tw, Get get)
+{
+    const fft::MixPlan &p = a.plan;
+    const int L = a.L, M = p.M;
+    if (MODE == fft::DFT_RADER) {
+        cplx acc = fft::mk(0.0, 0.0);
+        for (int n = threadIdx.x; n < L; n += blockDim.x) {
+            const cplx v = get(n);
+            acc = fft::add(acc, v);
+            if (n >= 1) x[fft::slot(a.lg[n])] = v;
+        }
+        acc = wave_reduce(acc);
+        if ((threadIdx.x & 63) == 0) wave_sums(M)[threadIdx.x >> 6] = acc;
+    } else {
+        for (int n = threadIdx.x; n < M; n += blockDim.x) {
+            cplx v = fft::mk(0.0, 0.0);
+            if (n < L) v = (MODE == fft::DFT_BLUESTEIN) ? fft::mul(get(n), a.w[n]) : get(n);
+            x[fft::slot(n)] = v;
+        }
+    }
+    __syncthreads();
+    if (MODE == fft::DFT_DIRECT)
+        forward_dft(x, tw, p);
+    else
+        convolve(x, a.B, tw, p);
+}
+
+// Output k of the row's DFT after row_dft (v0 = input 0, used by Rader)
+template <int MODE>
+__device__ __forceinline__ cplx row_out(const RowArgs &a, const cplx *x, int k, cplx v0)
+{
+    const int M = a.plan.M;
+    if (MODE == fft::DFT_DIRECT) return x[fft::slot(a.rev[k])];
+    if (MODE == fft::DFT_BLUESTEIN)
+        return fft::scale(fft::mul(a.w[k], x[fft::slot(k)]), 1.0 / (double)M);
+    // Rader: Z_0 = sum of the inputs; Z_{g^-p} = v0 + conv_p / M
+    if (k == 0) {
+        const cplx *ws = wave_sums(M);
+        cplx s = fft::mk(0.0, 0.0);
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; i++) s = fft::add(s, ws[i]);
+        return s;
+    }
+    const int q = a.lg[k];
+    return fft::axpy(1.0 / (double)M, x[fft::slot(q == 0 ? 0 : M - q)], v0);
 }
 
 // f (2L reals per row) -> spec (L+1 complex per row), F_k = sum_j f_j e^{-2 pi i jk/2L}
-template <int P>
-__global__ __launch_bounds__(Cfg<P>::T) void r2c_kernel(RowArgs a)
+template <int MODE>
+__global__ __launch_bounds__(1024) void r2c_kernel(RowArgs a)
 {
-    constexpr int M = Cfg<P>::M, T = Cfg<P>::T;
-    __shared__ cplx x[Cfg<P>::SLOTS];
+    cplx *x = lds_rows;
     const int L = a.L;
     const size_t row = blockIdx.x;
     const cplx *z = (const cplx *)(a.f + row * a.ldf);
-    for (int n = threadIdx.x; n < M; n += T)
-        x[fft::slot(n)] = (n < L) ? fft::mul(z[n], a.w[n]) : fft::mk(0.0, 0.0);
-    __syncthreads();
-    convolve<P>(x, a.B, a.tw);
-    // Z_k = w_k conv_k / M; F_k = E_k + e^{-i pi k/L} O_k with
-    // E_k = (Z_k + conj Z_{L-k}) / 2, O_k = (Z_k - conj Z_{L-k}) / (2i)
-    const double h = 0.5 / (double)M;
+    const fft::TwSplit tw = lds_twiddles(a.tw, a.plan.M);
+    row_dft<MODE>(a, x, tw, [&](int n) { return z[n]; });
+    const cplx z0 = (MODE == fft::DFT_RADER) ? z[0] : fft::mk(0.0, 0.0);
+    // Z_k; F_k = E_k + e^{-i pi k/L} O_k with E_k = (Z_k + conj Z_{L-k}) / 2,
+    // O_k = (Z_k - conj Z_{L-k}) / (2i)
     cplx *out = a.spec + row * a.lds;
-    for (int k = threadIdx.x; k <= L; k += T) {
+    for (int k = threadIdx.x; k <= L; k += blockDim.x) {
         const int kk = (k == L) ? 0 : k, kr = (k == 0) ? 0 : L - k;
-        const cplx zk = fft::mul(a.w[kk], x[fft::slot(kk)]);
-        const cplx zr = fft::conj(fft::mul(a.w[kr], x[fft::slot(kr)]));
-        const cplx e = fft::scale(fft::add(zk, zr), h);
-        const cplx o = fft::rot90<-1>(fft::scale(fft::sub(zk, zr), h));
+        const cplx zk = row_out<MODE>(a, x, kk, z0);
+        const cplx zr = fft::conj(row_out<MODE>(a, x, kr, z0));
+        const cplx e = fft::scale(fft::add(zk, zr), 0.5);
+        const cplx o = fft::rot90<-1>(fft::scale(fft::sub(zk, zr), 0.5));
         out[k] = fft::add(e, fft::mul(a.pt[k], o));
     }
 }
 
+// Input n of the inverse transform of a spectrum row: conj Z_n with
+// E = (X_n + conj X_{L-n}) / 2, O = (X_n - conj X_{L-n}) e^{i pi n/L} / 2,
+// Z = E + i O (imaginary parts of X_0 and X_L ignored); the inverse DFT of
+// Z is conj(DFT(conj Z))
+__device__ __forceinline__ cplx c2r_input(const RowArgs &a, const cplx *X, int n)
+{
+    const int L = a.L;
+    cplx xk = X[n], xr = X[L - n];
+    if (n == 0) {
+        xk.y = 0.0;
+        xr.y = 0.0;
+    }
+    const cplx e = fft::scale(fft::add(xk, fft::conj(xr)), 0.5);
+    const cplx o = fft::mul(fft::scale(fft::sub(xk, fft::conj(xr)), 0.5), fft::conj(a.pt[n]));
+    return fft::conj(fft::add(e, fft::rot90<+1>(o)));
+}
+
 // spec (L+1 complex per row) -> f (2L reals per row),
 // f_j = sum_{m=0}^{2L-1} X_m e^{+2 pi i jm/2L} over the Hermitian extension
-// (imaginary parts of X_0 and X_L ignored)
-template <int P>
-__global__ __launch_bounds__(Cfg<P>::T) void c2r_kernel(RowArgs a)
+template <int MODE>
+__global__ __launch_bounds__(1024) void c2r_kernel(RowArgs a)
 {
-    constexpr int M = Cfg<P>::M, T = Cfg<P>::T;
-    __shared__ cplx x[Cfg<P>::SLOTS];
+    cplx *x = lds_rows;
     const int L = a.L;
     const size_t row = blockIdx.x;
     const cplx *X = a.spec + row * a.lds;
-    for (int n = threadIdx.x; n < M; n += T) {
-        cplx v = fft::mk(0.0, 0.0);
-        if (n < L) {
-            cplx xk = X[n], xr = X[L - n];
-            if (n == 0) {
-                xk.y = 0.0;
-                xr.y = 0.0;
-            }
-            // E = (X_n + conj X_{L-n}) / 2, O = (X_n - conj X_{L-n}) e^{i pi n/L} / 2,
-            // Z = E + i O; the inverse DFT of Z is conj(DFT(conj Z))
-            const cplx e = fft::scale(fft::add(xk, fft::conj(xr)), 0.5);
-            const cplx o = fft::mul(fft::scale(fft::sub(xk, fft::conj(xr)), 0.5), fft::conj(a.pt[n]));
-            const cplx zz = fft::add(e, fft::rot90<+1>(o));
-            v = fft::mul(fft::conj(zz), a.w[n]);
-        }
-        x[fft::slot(n)] = v;
-    }
-    __syncthreads();
-    convolve<P>(x, a.B, a.tw);
-    // z_n = conj(w_n conv_n / M); f_2n = 2 Re z_n, f_2n+1 = 2 Im z_n
-    const double s2 = 2.0 / (double)M;
+    const fft::TwSplit tw = lds_twiddles(a.tw, a.plan.M);
+    row_dft<MODE>(a, x, tw, [&](int n) { return c2r_input(a, X, n); });
+    const cplx v0 = (MODE == fft::DFT_RADER) ? c2r_input(a, X, 0) : fft::mk(0.0, 0.0);
+    // z_n = conj(DFT_n); f_2n = 2 Re z_n, f_2n+1 = 2 Im z_n
     cplx *f = (cplx *)(a.f + row * a.ldf);
-    for (int n = threadIdx.x; n < L; n += T) {
-        const cplx y = fft::mul(a.w[n], x[fft::slot(n)]);
-        f[n] = fft::mk(s2 * y.x, -s2 * y.y);
+    for (int n = threadIdx.x; n < L; n += blockDim.x) {
+        const cplx y = row_out<MODE>(a, x, n, v0);
+        f[n] = fft::mk(2.0 * y.x, -2.0 * y.y);
     }
 }
 
-// Spectrum of the chirp filter b~ (b~_n = conj w_n for |n| < L, cyclic) in the
-// order the forward passes of convolve() leave data in
-template <int P>
-__global__ __launch_bounds__(Cfg<P>::T) void filter_kernel(int L, const cplx *w, const cplx *tw,
-                                                           cplx *B)
+// Spectrum of the convolution filter in the order the forward passes of
+// convolve() leave data in: src (Rader's b), or the chirp b~ (b~_n = conj w_n
+// for |n| < L, cyclic) when src is null
+__global__ __launch_bounds__(1024) void filter_kernel(RowArgs a, const cplx *src, cplx *B)
 {
-    constexpr int M = Cfg<P>::M, T = Cfg<P>::T;
-    constexpr int F = fft::plan_first(P), N8 = (P == 3) ? 0 : fft::plan_n8(P);
-    constexpr int LAST = (P == 3) ? 8 : 16;
-    __shared__ cplx x[Cfg<P>::SLOTS];
-    for (int n = threadIdx.x; n < M; n += T) {
+    cplx *x = lds_rows;
+    const fft::MixPlan &p = a.plan;
+    const int L = a.L, M = p.M;
+    const fft::TwSplit tw = lds_twiddles(a.tw, M);
+    for (int n = threadIdx.x; n < M; n += blockDim.x) {
         cplx v = fft::mk(0.0, 0.0);
-        if (n < L)
-            v = fft::conj(w[n]);
+        if (src)
+            v = src[n];
+        else if (n < L)
+            v = fft::conj(a.w[n]);
         else if (M - n < L)
-            v = fft::conj(w[M - n]);
+            v = fft::conj(a.w[M - n]);
         x[fft::slot(n)] = v;
     }
     __syncthreads();
-    const int t = threadIdx.x;
-    if (F) {
-        for (int u = t; u < M / F; u += T) {
-            if (F == 2)
-                fft::dif_butterfly<2>(x, tw, M, M, u);
-            else
-                fft::dif_butterfly<4>(x, tw, M, M, u);
-        }
-        __syncthreads();
-    }
-    int S = F ? M / F : M;
-    for (int i = 0; i < N8; i++) {
-        for (int u = t; u < M / 8; u += T) fft::dif_butterfly<8>(x, tw, M, S, u);
-        S /= 8;
-        __syncthreads();
-    }
-    for (int u = t; u < M / LAST; u += T) {
-        cplx v[LAST];
-#pragma unroll
-        for (int r = 0; r < LAST; r++) v[r] = x[fft::slot(u * LAST + r)];
-        fft::dft<LAST, -1>(v);
-#pragma unroll
-        for (int r = 0; r < LAST; r++) B[u * LAST + r] = v[r];
-    }
+    forward_dft(x, tw, p);
+    for (int e = threadIdx.x; e < M; e += blockDim.x) B[e] = x[fft::slot(e)];
 }
 
 __global__ void chirp_table_kernel(int L, cplx *w, cplx *pt)
@@ -275,6 +329,7 @@ __global__ void chirp_table_kernel(int L, cplx *w, cplx *pt)
     }
 }
 
+// tw[q] = exp(-2 pi i q / M), exact at the quarter points
 __global__ void twiddle_table_kernel(int M, cplx *tw)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -295,51 +350,70 @@ __global__ void twiddle_table_kernel(int M, cplx *tw)
 }
 
 struct Plan {
-    int L = 0, P = 0;
+    fft::MixPlan mp;
+    int L = 0;
+    int threads = 64;
     cplx *w = nullptr, *pt = nullptr, *B = nullptr;
+    int *rev = nullptr, *lg = nullptr;
     const cplx *tw = nullptr;
 };
 
 std::mutex g_fft_mu;
 std::map<int, cplx *> g_tw;   // M -> twiddles
 std::map<int, Plan> g_plans;  // L -> plan
+bool g_attr_set = false;
 
-int log2_ceil(long v)
+size_t lds_bytes(int M)
 {
-    int p = 0;
-    while ((1L << p) < v) p++;
-    return p;
+    return (size_t)(fft::slots(M) + fft::tw_hi_count(M) + fft::TW_LO + NWSUM) * sizeof(cplx);
 }
 
-template <int P>
-void launch_rows(bool c2r, const RowArgs &a, int nrow, hipStream_t s)
+// Up to one radix-8 butterfly per thread, in whole waves, and no more waves
+// than let every block that fits in LDS stay resident at 16 waves per CU
+// (4 per SIMD: the row kernels use up to 128 VGPRs)
+int threads_for(const fft::MixPlan &p)
 {
-    if (c2r)
-        hipLaunchKernelGGL(c2r_kernel<P>, dim3(nrow), dim3(Cfg<P>::T), 0, s, a);
-    else
-        hipLaunchKernelGGL(r2c_kernel<P>, dim3(nrow), dim3(Cfg<P>::T), 0, s, a);
+    int most = 1;
+    for (int i = 0; i < p.np; i++) most = std::max(most, p.M / p.R[i]);
+    most = std::min(most, std::max(1, p.M / 8));
+    int t = std::min(1024, std::max(64, (most + 63) / 64 * 64));
+    const int per_cu = std::max(1, (int)(163840 / lds_bytes(p.M)));
+    const int cap = std::max(64, (16 / std::min(per_cu, 16)) * 64);
+    return std::min(t, cap);
 }
 
-template <int P>
-void launch_filter(int L, const cplx *w, const cplx *tw, cplx *B, hipStream_t s)
+hipError_t set_lds_limits()
 {
-    hipLaunchKernelGGL(filter_kernel<P>, dim3(1), dim3(Cfg<P>::T), 0, s, L, w, tw, B);
-}
-
-#define FFT_DISPATCH(P, CALL)                                                                   \
-    switch (P) {                                                                                \
-    case 3: CALL(3); break;                                                                     \
-    case 4: CALL(4); break;                                                                     \
-    case 5: CALL(5); break;                                                                     \
-    case 6: CALL(6); break;                                                                     \
-    case 7: CALL(7); break;                                                                     \
-    case 8: CALL(8); break;                                                                     \
-    case 9: CALL(9); break;                                                                     \
-    case 10: CALL(10); break;                                                                   \
-    case 11: CALL(11); break;                                                                   \
-    case 12: CALL(12); break;                                                                   \
-    default: CALL(13); break;                                                                   \
+    if (g_attr_set) return hipSuccess;
+    const int bytes = (int)lds_bytes(fft::MMAX);
+    const void *fns[] = {(const void *)r2c_kernel<fft::DFT_DIRECT>,
+                         (const void *)r2c_kernel<fft::DFT_BLUESTEIN>,
+                         (const void *)r2c_kernel<fft::DFT_RADER>,
+                         (const void *)c2r_kernel<fft::DFT_DIRECT>,
+                         (const void *)c2r_kernel<fft::DFT_BLUESTEIN>,
+                         (const void *)c2r_kernel<fft::DFT_RADER>,
+                         (const void *)filter_kernel};
+    for (const void *fn : fns) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e != hipSuccess) return e;
     }
+    g_attr_set = true;
+    return hipSuccess;
+}
+
+RowArgs row_args(const Plan &p)
+{
+    RowArgs a{};
+    a.plan = p.mp;
+    a.L = p.L;
+    a.w = p.w;
+    a.pt = p.pt;
+    a.B = p.B;
+    a.rev = p.rev;
+    a.lg = p.lg;
+    a.tw = p.tw;
+    return a;
+}
 
 // Plan of length L (created once per process; the tables stay resident)
 int get_plan(int L, hipStream_t s, Plan *out)
@@ -352,39 +426,73 @@ int get_plan(int L, hipStream_t s, Plan *out)
     }
     Plan p;
     p.L = L;
-    p.P = std::max(3, log2_ceil(2L * L - 1));
-    const int M = 1 << p.P;
-    hipError_t e = hipSuccess;
+    if (!fft::choose_plan(L, p.mp)) return (int)hipErrorInvalidValue;
+    p.threads = threads_for(p.mp);
+    const int M = p.mp.M;
+    hipError_t e = set_lds_limits();
+    if (e != hipSuccess) return (int)e;
     cplx *tw = nullptr;
     auto tit = g_tw.find(M);
     if (tit == g_tw.end()) {
-        e = hipMalloc((void **)&tw, M * sizeof(cplx));
-        if (e != hipSuccess) return (int)e;
+        if ((e = hipMalloc((void **)&tw, M * sizeof(cplx))) != hipSuccess) return (int)e;
         hipLaunchKernelGGL(twiddle_table_kernel, dim3((M + 255) / 256), dim3(256), 0, s, M, tw);
         g_tw[M] = tw;
     } else {
         tw = tit->second;
     }
     p.tw = tw;
-    if ((e = hipMalloc((void **)&p.w, L * sizeof(cplx))) != hipSuccess) return (int)e;
     if ((e = hipMalloc((void **)&p.pt, (L + 1) * sizeof(cplx))) != hipSuccess) return (int)e;
-    if ((e = hipMalloc((void **)&p.B, M * sizeof(cplx))) != hipSuccess) return (int)e;
+    if ((e = hipMalloc((void **)&p.w, L * sizeof(cplx))) != hipSuccess) return (int)e;
+    // host tables go up through pageable copies, waited for below
+    std::vector<int> itab;
+    std::vector<cplx> btab;
+    cplx *bsrc = nullptr;
+    if (p.mp.mode != fft::DFT_DIRECT) {
+        if ((e = hipMalloc((void **)&p.B, M * sizeof(cplx))) != hipSuccess) return (int)e;
+    }
+    if (p.mp.mode == fft::DFT_DIRECT) {
+        itab = fft::digit_reversal(p.mp);
+        if ((e = hipMalloc((void **)&p.rev, L * sizeof(int))) != hipSuccess) return (int)e;
+        if ((e = hipMemcpyAsync(p.rev, itab.data(), L * sizeof(int), hipMemcpyHostToDevice, s)) !=
+            hipSuccess)
+            return (int)e;
+    } else if (p.mp.mode == fft::DFT_RADER) {
+        itab = fft::rader_logs(L, p.mp.g);
+        btab = fft::rader_filter(L, p.mp.g);
+        if ((e = hipMalloc((void **)&p.lg, L * sizeof(int))) != hipSuccess) return (int)e;
+        if ((e = hipMalloc((void **)&bsrc, M * sizeof(cplx))) != hipSuccess) return (int)e;
+        if ((e = hipMemcpyAsync(p.lg, itab.data(), L * sizeof(int), hipMemcpyHostToDevice, s)) !=
+            hipSuccess)
+            return (int)e;
+        if ((e = hipMemcpyAsync(bsrc, btab.data(), M * sizeof(cplx), hipMemcpyHostToDevice, s)) !=
+            hipSuccess)
+            return (int)e;
+    }
     hipLaunchKernelGGL(chirp_table_kernel, dim3((L + 1 + 255) / 256), dim3(256), 0, s, L, p.w, p.pt);
-#define FFT_FILTER(PP) launch_filter<PP>(L, p.w, p.tw, p.B, s)
-    FFT_DISPATCH(p.P, FFT_FILTER)
-#undef FFT_FILTER
+    if (p.mp.mode != fft::DFT_DIRECT) {
+        RowArgs a = row_args(p);
+        hipLaunchKernelGGL(filter_kernel, dim3(1), dim3(p.threads), lds_bytes(M), s, a, bsrc, p.B);
+    }
     if ((e = hipGetLastError()) != hipSuccess) return (int)e;
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return (int)e;
+    if (bsrc) (void)hipFree(bsrc);
     g_plans[L] = p;
     *out = p;
     return 0;
+}
+
+bool plan_exists(int L)
+{
+    fft::MixPlan mp;
+    return fft::choose_plan(L, mp);
 }
 
 }  // namespace
 
 bool row_fft_supported(size_t nlon)
 {
-    return nlon >= 2 && nlon % 2 == 0 && nlon / 2 <= 4096;
+    if (nlon < 2 || nlon % 2 != 0 || nlon / 2 > (size_t)fft::MMAX) return false;
+    return plan_exists((int)(nlon / 2));
 }
 
 int launch_row_fft(int c2r, size_t nlon, size_t nrow, double *f, size_t ldf, void *spec, size_t lds,
@@ -396,19 +504,29 @@ int launch_row_fft(int c2r, size_t nlon, size_t nrow, double *f, size_t ldf, voi
     Plan p;
     const int L = (int)(nlon / 2);
     if (int e = get_plan(L, s, &p)) return e;
-    RowArgs a;
-    a.L = L;
-    a.w = p.w;
-    a.pt = p.pt;
-    a.B = p.B;
-    a.tw = p.tw;
+    RowArgs a = row_args(p);
     a.f = f;
     a.ldf = ldf;
     a.spec = (cplx *)spec;
     a.lds = lds;
-#define FFT_ROWS(PP) launch_rows<PP>(c2r != 0, a, (int)nrow, s)
-    FFT_DISPATCH(p.P, FFT_ROWS)
-#undef FFT_ROWS
+    const size_t sh = lds_bytes(p.mp.M);
+    const dim3 grid((unsigned)nrow), block((unsigned)p.threads);
+#define ROW_LAUNCH(KERNEL)                                                                         \
+    switch (p.mp.mode) {                                                                           \
+    case fft::DFT_DIRECT:                                                                          \
+        hipLaunchKernelGGL(KERNEL<fft::DFT_DIRECT>, grid, block, sh, s, a);                        \
+        break;                                                                                     \
+    case fft::DFT_BLUESTEIN:                                                                       \
+        hipLaunchKernelGGL(KERNEL<fft::DFT_BLUESTEIN>, grid, block, sh, s, a);                     \
+        break;                                                                                     \
+    default: hipLaunchKernelGGL(KERNEL<fft::DFT_RADER>, grid, block, sh, s, a); break;             \
+    }
+    if (c2r) {
+        ROW_LAUNCH(c2r_kernel)
+    } else {
+        ROW_LAUNCH(r2c_kernel)
+    }
+#undef ROW_LAUNCH
     return (int)hipGetLastError();
 }
 

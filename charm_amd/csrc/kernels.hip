@@ -110,6 +110,43 @@ __device__ __forceinline__ bool lane_deep(double x, double y, int e)
     return e != 0 && e + max(ex, ey) < -1180;
 }
 
+// lane_renorm that also returns whether the next 8-degree tile is deep
+// (lane_deep of the renormalised state: rescaling keeps the true values, so
+// the exponents read here decide it)
+__device__ __forceinline__ void lane_renorm_deep(double &x, double &y, int &e, bool &deep)
+{
+    deep = false;
+    if (e != 0) {
+        int ex, ey;
+        (void)frexp(x, &ex);
+        (void)frexp(y, &ey);
+        deep = e + max(ex, ey) < -1180;
+        if (e + ex > -480 && e + ey > -480) {
+            x = ldexp(x, e);
+            y = ldexp(y, e);
+            e = 0;
+        } else if (ex > 256 || ex < -256) {
+            x = ldexp(x, -ex);
+            y = ldexp(y, -ex);
+            e += ex;
+        }
+    }
+}
+
+// The 8 alpha values of a tile from the LDS window, read before any other
+// LDS access of the tile: LDS requests of a wave complete in order, so a
+// wait for a value read later would also wait for everything issued before.
+__device__ __forceinline__ void load_alpha8(const double *al, double a[8])
+{
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const double2 w = *(const double2 *)(al + 2 * i);
+        a[2 * i] = w.x;
+        a[2 * i + 1] = w.y;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+}
+
 __device__ __forceinline__ int uniform(int v)
 {
     return __builtin_amdgcn_readfirstlane(v);
@@ -630,17 +667,18 @@ struct WaveReducer {
     double r[16];     // pending values read back from LDS
     int lane;
 
+    // The reads are issued whether or not a tile is pending (the buffer is
+    // always this wave's own LDS; unused values are dropped by finish), so
+    // that no branch makes the compiler wait for them early.
     __device__ __forceinline__ void begin()
     {
-        if (dst) {
-            const int j = lane >> 2, q = lane & 3;
-            const double *row = red + j * RED_LD + 2 * q;
+        const int j = lane >> 2, q = lane & 3;
+        const double *row = red + j * RED_LD + 2 * q;
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const double2 w = *(const double2 *)(row + 8 * i);
-                r[2 * i] = w.x;
-                r[2 * i + 1] = w.y;
-            }
+        for (int i = 0; i < 8; i++) {
+            const double2 w = *(const double2 *)(row + 8 * i);
+            r[2 * i] = w.x;
+            r[2 * i + 1] = w.y;
         }
     }
     __device__ __forceinline__ void finish()
@@ -805,6 +843,9 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
     st.lane = lane;
     st.start(m + 1);
     int n = m + 1;
+    bool dk[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) dk[k] = !valid[k] || lane_deep(x[k], y[k], e[k]);
     // phase 1: tiles while a lane of the wave still carries an exponent
     for (; n + 7 <= nmax; n += 8) {
         if (n - st.n0 == 64) st.next();
@@ -813,18 +854,19 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
 #pragma unroll
         for (int k = 0; k < K; k++) {
             anyx = anyx || (e[k] != 0);
-            deep = deep && (!valid[k] || lane_deep(x[k], y[k], e[k]));
+            deep = deep && dk[k];
         }
         if (!wave_any(anyx)) break;
+        double a[8];
+        load_alpha8(st.al + o, a);
         if (PIPE) rd.begin();
         if (wave_all(deep)) {
             // every term of the tile is exactly zero
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                const double al = st.al[o + j];
 #pragma unroll
                 for (int k = 0; k < K; k++) {
-                    const double z = fma(al * x[k], t[k], -y[k]);
+                    const double z = fma(a[j] * x[k], t[k], -y[k]);
                     y[k] = x[k];
                     x[k] = z;
                 }
@@ -834,7 +876,7 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
         } else {
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                const double al = st.al[o + j];
+                const double al = a[j];
                 double c = 0.0, sv = 0.0;
 #pragma unroll
                 for (int k = 0; k < K; k++) {
@@ -855,23 +897,27 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
             }
         }
 #pragma unroll
-        for (int k = 0; k < K; k++) lane_renorm(x[k], y[k], e[k]);
+        for (int k = 0; k < K; k++) {
+            lane_renorm_deep(x[k], y[k], e[k], dk[k]);
+            dk[k] = dk[k] || !valid[k];
+        }
     }
     // phase 2: every lane in plain doubles
     for (; n + 7 <= nmax; n += 8) {
         if (n - st.n0 == 64) st.next();
         const int o = n - st.n0;
+        double a[8];
+        load_alpha8(st.al + o, a);
         if (PIPE) rd.begin();
 #pragma unroll
         for (int j = 0; j < 8; j += 2) {
-            const double2 a2 = *(const double2 *)(st.al + o + j);
             double c0 = 0.0, s0 = 0.0, c1 = 0.0, s1 = 0.0;
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                const double z = fma(a2.x * x[k], t[k], -y[k]);
+                const double z = fma(a[j] * x[k], t[k], -y[k]);
                 c0 = fma(z, ammC[k], c0);
                 s0 = fma(z, ammS[k], s0);
-                const double w = fma(a2.y * z, t[k], -x[k]);
+                const double w = fma(a[j + 1] * z, t[k], -x[k]);
                 c1 = fma(w, ampC[k], c1);
                 s1 = fma(w, ampS[k], s1);
                 y[k] = z;

@@ -1,7 +1,8 @@
 // Host build of the longitude-transform building blocks (charm_amd/csrc/
-// fft_core.h), driven serially in the same pass order as the HIP kernels, so
-// that the CPU test suite checks the butterflies, twiddles, pass plan and
-// the real/complex packing against numpy.fft.
+// fft_core.h), driven serially with the same plans and in the same pass order
+// as the HIP kernels, so that the CPU test suite checks the butterflies,
+// twiddles, plan choice, digit reversal and the real/complex packing against
+// numpy.fft.
 #include <cstddef>
 #include <vector>
 
@@ -11,45 +12,85 @@ using namespace charm_amd::fft;
 
 namespace {
 
-int log2_ceil(long v)
-{
-    int p = 0;
-    while ((1L << p) < v) p++;
-    return p;
-}
-
 struct HostPlan {
-    int L, P, M;
-    std::vector<cplx> w, pt, B, tw;
+    MixPlan mp;
+    bool ok = false;
+    int L = 0;
+    std::vector<cplx> w, pt, B, hi, lo;
+    std::vector<int> rev, lg;
+    TwSplit tw() const
+    {
+        TwSplit t;
+        t.hi = hi.data();
+        t.lo = lo.data();
+        return t;
+    }
 };
 
-void dif(std::vector<cplx> &x, const HostPlan &p)
+void pass(std::vector<cplx> &x, const HostPlan &p, int S, int R, bool inv)
 {
-    int S = p.M;
-    for (int i = 0; i < pass_count(p.P); i++) {
-        const int R = pass_radix(p.P, i);
-        for (int t = 0; t < p.M / R; t++) {
-            if (R == 16) dif_butterfly<16>(x.data(), p.tw.data(), p.M, S, t);
-            if (R == 8) dif_butterfly<8>(x.data(), p.tw.data(), p.M, S, t);
-            if (R == 4) dif_butterfly<4>(x.data(), p.tw.data(), p.M, S, t);
-            if (R == 2) dif_butterfly<2>(x.data(), p.tw.data(), p.M, S, t);
-        }
-        S /= R;
+    const int M = p.mp.M;
+    const TwSplit tw = p.tw();
+    const PassGeom g = pass_geom(M, S, R);
+    for (int u = 0; u < M / R; u++) {
+        cplx *d = x.data();
+#define HOST_PASS(RR)                                                                              \
+    if (R == RR) {                                                                                 \
+        if (inv) {                                                                                 \
+            if (g.sh >= 0)                                                                         \
+                dit_butterfly<RR, true>(d, tw, g, u);                                              \
+            else                                                                                   \
+                dit_butterfly<RR, false>(d, tw, g, u);                                             \
+        } else {                                                                                   \
+            if (g.sh >= 0)                                                                         \
+                dif_butterfly<RR, true>(d, tw, g, u);                                              \
+            else                                                                                   \
+                dif_butterfly<RR, false>(d, tw, g, u);                                             \
+        }                                                                                          \
+    }
+        HOST_PASS(2) HOST_PASS(3) HOST_PASS(4) HOST_PASS(5) HOST_PASS(7) HOST_PASS(8) HOST_PASS(9)
+#undef HOST_PASS
     }
 }
 
-void dit(std::vector<cplx> &x, const HostPlan &p)
+void forward_head(std::vector<cplx> &x, const HostPlan &p)
 {
-    for (int i = pass_count(p.P) - 1; i >= 0; i--) {
-        int S = p.M;
-        for (int q = 0; q < i; q++) S /= pass_radix(p.P, q);
-        const int R = pass_radix(p.P, i);
-        for (int t = 0; t < p.M / R; t++) {
-            if (R == 16) dit_butterfly<16>(x.data(), p.tw.data(), p.M, S, t);
-            if (R == 8) dit_butterfly<8>(x.data(), p.tw.data(), p.M, S, t);
-            if (R == 4) dit_butterfly<4>(x.data(), p.tw.data(), p.M, S, t);
-            if (R == 2) dit_butterfly<2>(x.data(), p.tw.data(), p.M, S, t);
-        }
+    int S = p.mp.M;
+    for (int i = 0; i + 1 < p.mp.np; i++) {
+        pass(x, p, S, p.mp.R[i], false);
+        S /= p.mp.R[i];
+    }
+}
+
+void forward_dft(std::vector<cplx> &x, const HostPlan &p)
+{
+    if (p.mp.np == 0) return;
+    forward_head(x, p);
+    const int R = p.mp.R[p.mp.np - 1];
+    for (int u = 0; u < p.mp.M / R; u++) {
+#define HOST_LAST(RR)                                                                              \
+    if (R == RR) last_block<RR>(x.data(), u);
+        HOST_LAST(2) HOST_LAST(3) HOST_LAST(4) HOST_LAST(5) HOST_LAST(7) HOST_LAST(8) HOST_LAST(9)
+        HOST_LAST(16)
+#undef HOST_LAST
+    }
+}
+
+void convolve(std::vector<cplx> &x, const HostPlan &p)
+{
+    forward_head(x, p);
+    const int R = p.mp.R[p.mp.np - 1];
+    for (int u = 0; u < p.mp.M / R; u++) {
+#define HOST_MID(RR)                                                                               \
+    if (R == RR) middle_block<RR>(x.data(), p.B.data(), u);
+        HOST_MID(2) HOST_MID(3) HOST_MID(4) HOST_MID(5) HOST_MID(7) HOST_MID(8) HOST_MID(9)
+        HOST_MID(16)
+#undef HOST_MID
+    }
+    int S = R;
+    for (int i = p.mp.np - 2; i >= 0; i--) {
+        S *= p.mp.R[i];
+        pass(x, p, S, p.mp.R[i], true);
     }
 }
 
@@ -57,14 +98,16 @@ HostPlan make_plan(int L)
 {
     HostPlan p;
     p.L = L;
-    p.P = log2_ceil(2L * L - 1);
-    if (p.P < 3) p.P = 3;
-    p.M = 1 << p.P;
-    p.tw.resize(p.M);
-    for (int q = 0; q < p.M; q++) {
-        const double a = 2.0 * 3.14159265358979323846 * q / p.M;
-        p.tw[q] = mk(std::cos(a), -std::sin(a));
+    if (!choose_plan(L, p.mp)) return p;
+    p.ok = true;
+    const int M = p.mp.M;
+    std::vector<cplx> tw(M);
+    for (int q = 0; q < M; q++) {
+        const double a = 2.0 * 3.14159265358979323846 * q / M;
+        tw[q] = mk(std::cos(a), -std::sin(a));
     }
+    for (int i = 0; i < tw_hi_count(M); i++) p.hi.push_back(tw[i * TW_LO]);
+    for (int i = 0; i < TW_LO; i++) p.lo.push_back(i < M ? tw[i] : mk(0, 0));
     p.w.resize(L);
     p.pt.resize(L + 1);
     for (int n = 0; n < L; n++) p.w[n] = chirp(n, L, -1);
@@ -72,48 +115,92 @@ HostPlan make_plan(int L)
         const double a = 3.14159265358979323846 * n / L;
         p.pt[n] = mk(std::cos(a), -std::sin(a));
     }
-    std::vector<cplx> x(slots(p.M), mk(0, 0));
-    for (int n = 0; n < p.M; n++) {
-        cplx v = mk(0, 0);
-        if (n < L)
-            v = conj(p.w[n]);
-        else if (p.M - n < L)
-            v = conj(p.w[p.M - n]);
-        x[slot(n)] = v;
+    if (p.mp.mode == DFT_DIRECT) {
+        p.rev = digit_reversal(p.mp);
+        return p;
     }
-    dif(x, p);
-    p.B.resize(p.M);
-    for (int e = 0; e < p.M; e++) p.B[e] = x[slot(e)];
+    std::vector<cplx> x(slots(M), mk(0, 0));
+    if (p.mp.mode == DFT_RADER) {
+        p.lg = rader_logs(L, p.mp.g);
+        const std::vector<cplx> b = rader_filter(L, p.mp.g);
+        for (int n = 0; n < M; n++) x[slot(n)] = b[n];
+    } else {
+        for (int n = 0; n < M; n++) {
+            cplx v = mk(0, 0);
+            if (n < L)
+                v = conj(p.w[n]);
+            else if (M - n < L)
+                v = conj(p.w[M - n]);
+            x[slot(n)] = v;
+        }
+    }
+    forward_dft(x, p);
+    p.B.resize(M);
+    for (int e = 0; e < M; e++) p.B[e] = x[slot(e)];
     return p;
 }
 
-void convolve(std::vector<cplx> &x, const HostPlan &p)
+// Z_k = sum_n z_n exp(-2 pi i nk / L), k = 0..L-1, through the plan, with the
+// kernels' input placement and output reads
+std::vector<cplx> row_dft(const HostPlan &p, const std::vector<cplx> &z)
 {
-    dif(x, p);
-    for (int e = 0; e < p.M; e++) x[slot(e)] = mul(x[slot(e)], p.B[e]);
-    dit(x, p);
+    const int L = p.L, M = p.mp.M;
+    std::vector<cplx> x(slots(M), mk(0, 0)), Z(L);
+    cplx sum = mk(0, 0);
+    if (p.mp.mode == DFT_RADER) {
+        for (int n = 0; n < L; n++) {
+            sum = add(sum, z[n]);
+            if (n >= 1) x[slot(p.lg[n])] = z[n];
+        }
+        convolve(x, p);
+        Z[0] = sum;
+        for (int k = 1; k < L; k++) {
+            const int q = p.lg[k];
+            Z[k] = axpy(1.0 / M, x[slot(q == 0 ? 0 : M - q)], z[0]);
+        }
+    } else if (p.mp.mode == DFT_BLUESTEIN) {
+        for (int n = 0; n < L; n++) x[slot(n)] = mul(z[n], p.w[n]);
+        convolve(x, p);
+        for (int k = 0; k < L; k++) Z[k] = scale(mul(p.w[k], x[slot(k)]), 1.0 / M);
+    } else {
+        for (int n = 0; n < L; n++) x[slot(n)] = z[n];
+        forward_dft(x, p);
+        for (int k = 0; k < L; k++) Z[k] = x[slot(p.rev[k])];
+    }
+    return Z;
 }
 
 }  // namespace
 
 extern "C" {
 
+// plan of length L: M, pass count, radices, mode; returns 0 if none
+int host_plan(int L, int *M, int *np, int *R, int *mode)
+{
+    MixPlan mp;
+    if (!choose_plan(L, mp)) return 0;
+    *M = mp.M;
+    *np = mp.np;
+    for (int i = 0; i < mp.np; i++) R[i] = mp.R[i];
+    *mode = mp.mode;
+    return 1;
+}
+
 // f: nrow x 2L reals -> spec: nrow x (L+1) complex (interleaved)
 void host_r2c(int L, int nrow, const double *f, double *spec)
 {
     HostPlan p = make_plan(L);
+    if (!p.ok) return;
     for (int row = 0; row < nrow; row++) {
-        std::vector<cplx> x(slots(p.M), mk(0, 0));
-        const double *z = f + (size_t)row * 2 * L;
-        for (int n = 0; n < L; n++) x[slot(n)] = mul(mk(z[2 * n], z[2 * n + 1]), p.w[n]);
-        convolve(x, p);
-        const double h = 0.5 / p.M;
+        const double *zr0 = f + (size_t)row * 2 * L;
+        std::vector<cplx> z(L);
+        for (int n = 0; n < L; n++) z[n] = mk(zr0[2 * n], zr0[2 * n + 1]);
+        const std::vector<cplx> Z = row_dft(p, z);
         for (int k = 0; k <= L; k++) {
             const int kk = (k == L) ? 0 : k, kr = (k == 0) ? 0 : L - k;
-            const cplx zk = mul(p.w[kk], x[slot(kk)]);
-            const cplx zr = conj(mul(p.w[kr], x[slot(kr)]));
-            const cplx e = scale(add(zk, zr), h);
-            const cplx o = rot90<-1>(scale(sub(zk, zr), h));
+            const cplx zk = Z[kk], zr = conj(Z[kr]);
+            const cplx e = scale(add(zk, zr), 0.5);
+            const cplx o = rot90<-1>(scale(sub(zk, zr), 0.5));
             const cplx F = add(e, mul(p.pt[k], o));
             spec[((size_t)row * (L + 1) + k) * 2] = F.x;
             spec[((size_t)row * (L + 1) + k) * 2 + 1] = F.y;
@@ -125,9 +212,10 @@ void host_r2c(int L, int nrow, const double *f, double *spec)
 void host_c2r(int L, int nrow, const double *spec, double *f)
 {
     HostPlan p = make_plan(L);
+    if (!p.ok) return;
     for (int row = 0; row < nrow; row++) {
         const double *X = spec + (size_t)row * (L + 1) * 2;
-        std::vector<cplx> x(slots(p.M), mk(0, 0));
+        std::vector<cplx> v(L);
         for (int n = 0; n < L; n++) {
             cplx xk = mk(X[2 * n], X[2 * n + 1]), xr = mk(X[2 * (L - n)], X[2 * (L - n) + 1]);
             if (n == 0) {
@@ -136,15 +224,12 @@ void host_c2r(int L, int nrow, const double *spec, double *f)
             }
             const cplx e = scale(add(xk, conj(xr)), 0.5);
             const cplx o = mul(scale(sub(xk, conj(xr)), 0.5), conj(p.pt[n]));
-            const cplx zz = add(e, rot90<+1>(o));
-            x[slot(n)] = mul(conj(zz), p.w[n]);
+            v[n] = conj(add(e, rot90<+1>(o)));
         }
-        convolve(x, p);
-        const double s2 = 2.0 / p.M;
+        const std::vector<cplx> Y = row_dft(p, v);
         for (int n = 0; n < L; n++) {
-            const cplx y = mul(p.w[n], x[slot(n)]);
-            f[(size_t)row * 2 * L + 2 * n] = s2 * y.x;
-            f[(size_t)row * 2 * L + 2 * n + 1] = -s2 * y.y;
+            f[(size_t)row * 2 * L + 2 * n] = 2.0 * Y[n].x;
+            f[(size_t)row * 2 * L + 2 * n + 1] = -2.0 * Y[n].y;
         }
     }
 }

@@ -25,16 +25,62 @@ def fftlib(tmp_path_factory):
     dp = ct.POINTER(ct.c_double)
     lib.host_r2c.argtypes = [ct.c_int, ct.c_int, dp, dp]
     lib.host_c2r.argtypes = [ct.c_int, ct.c_int, dp, dp]
+    ip = ct.POINTER(ct.c_int)
+    lib.host_plan.argtypes = [ct.c_int, ip, ip, ip, ip]
     return lib
+
+
+DIRECT, BLUESTEIN, RADER = 0, 1, 2
+
+
+def plan(lib, L):
+    M, np_, mode = ct.c_int(), ct.c_int(), ct.c_int()
+    R = (ct.c_int * 16)()
+    if not lib.host_plan(L, ct.byref(M), ct.byref(np_), R, ct.byref(mode)):
+        return None
+    return M.value, [R[i] for i in range(np_.value)], mode.value
+
+
+def is_prime(n):
+    return n > 1 and all(n % d for d in range(2, int(n ** 0.5) + 1))
+
+
+def test_plans(fftlib):
+    """Plans multiply out to M, put 16 last when 16 | M, run smooth lengths
+    directly, primes with a smooth L - 1 by Rader, and the rest by Bluestein
+    over a smooth length >= 2L - 1."""
+    assert plan(fftlib, 1) == (1, [], DIRECT)
+    assert plan(fftlib, 360) == (360, [5, 9, 8], DIRECT)
+    assert plan(fftlib, 8192) == (8192, [8, 8, 8, 16], DIRECT)
+    assert plan(fftlib, 2161) == (2160, [5, 9, 3, 16], RADER)   # nmax 2160
+    for L in range(1, 600):
+        M, R, mode = plan(fftlib, L)
+        assert int(np.prod(R)) == M
+        assert all(r in (2, 3, 4, 5, 7, 8, 9, 16) for r in R)
+        assert all(r != 16 for r in R[:-1])
+        if mode == DIRECT:
+            assert M == L
+        elif mode == RADER:
+            assert is_prime(L) and M == L - 1
+        else:
+            assert 2 * L - 1 <= M
+    assert plan(fftlib, 9601) is None    # prime beyond the largest LDS row
+    assert plan(fftlib, 4603) == (9216, [9, 8, 8, 16], BLUESTEIN)   # 4602 = 2 3 13 59
 
 
 def _p(a):
     return a.ctypes.data_as(ct.POINTER(ct.c_double))
 
 
-# L = nlon / 2: tiny, prime (2161 = nmax 2160 + 1), composite, powers of two,
-# and the largest length the LDS kernels take
-@pytest.mark.parametrize("L", [1, 2, 3, 5, 8, 91, 361, 1000, 2161, 4096])
+# L = nlon / 2: tiny; every radix as a direct plan (3, 5, 7, 9, 16, 360 =
+# 5 9 8, 1000, 4096, 2^4 3^2 5 7 = 5040, 9216 near the largest row);
+# Bluestein over a smooth M (91, 361, 2161 = nmax 2160 + 1, 4603 with M = 9216)
+# Rader on primes with smooth L - 1 (11, 97, 1801, 2161, 4801)
+LENGTHS = [1, 2, 3, 5, 7, 8, 9, 11, 16, 91, 97, 360, 361, 1000, 1801, 2161, 4096, 4603, 4801,
+           5040, 9216]
+
+
+@pytest.mark.parametrize("L", LENGTHS)
 def test_row_r2c_matches_numpy(fftlib, L):
     rng = np.random.default_rng(L)
     f = rng.standard_normal((3, 2 * L))
@@ -45,7 +91,7 @@ def test_row_r2c_matches_numpy(fftlib, L):
     assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("L", [1, 2, 3, 5, 8, 91, 361, 1000, 2161, 4096])
+@pytest.mark.parametrize("L", LENGTHS)
 def test_row_c2r_matches_numpy(fftlib, L):
     rng = np.random.default_rng(10 + L)
     X = rng.standard_normal((3, L + 1)) + 1j * rng.standard_normal((3, L + 1))
