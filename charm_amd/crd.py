@@ -101,6 +101,47 @@ class PointGrid(_PointBase):
         self._from_arrays(lat, lon, r)
 
 
+class _PointGridDH(_PointBase):
+    _fn = None
+
+    def __init__(self, nmax, r=1.0):
+        f = getattr(lib(), "charm_crd_point_%s" % self._fn)
+        f.restype = _ct.POINTER(_Point)
+        f.argtypes = [_ct.c_ulong, _ct.c_double]
+        p = f(int(nmax), float(r))
+        if not p:
+            raise ValueError("charm_crd_point_%s failed" % self._fn)
+        self._Point = p
+        self._arrays = None
+
+    @staticmethod
+    def get_nmax(nlat):
+        return nlat // 2 - 1
+
+    @classmethod
+    def shape(cls, nmax):
+        f = getattr(lib(), "charm_crd_point_%s_shape" % cls._fn)
+        f.restype = None
+        f.argtypes = [_ct.c_ulong, _ct.POINTER(_ct.c_size_t), _ct.POINTER(_ct.c_size_t)]
+        a, b = _ct.c_size_t(), _ct.c_size_t()
+        f(int(nmax), _ct.byref(a), _ct.byref(b))
+        return a.value, b.value
+
+
+class PointGridDH1(_PointGridDH):
+    """Non-equiangular Driscoll-Healy grid for degree nmax on a sphere of
+    radius r (charm_crd_point_dh1): 2(nmax+1) x 2(nmax+1) points."""
+    _type = POINT_GRID_DH1
+    _fn = "dh1"
+
+
+class PointGridDH2(_PointGridDH):
+    """Equiangular Driscoll-Healy grid (charm_crd_point_dh2): 2(nmax+1) x
+    4(nmax+1) points."""
+    _type = POINT_GRID_DH2
+    _fn = "dh2"
+
+
 class PointGridGL(_PointBase):
     """Gauss-Legendre grid for degree nmax on a sphere of radius r
     (charm_crd_point_gl)."""

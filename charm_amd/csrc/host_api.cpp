@@ -414,4 +414,101 @@ charm_point *charm_crd_point_gl(unsigned long nmax, double r)
     return glg;
 }
 
+// ---------------------------------------------------------------------------
+// Driscoll--Healy grids (src/crd/crd_point_dh1.c, crd_point_dh2.c,
+// crd_point_dh_chunk.c:19-150, crd_point_dh_lat_w_chunk.c:16-100,
+// crd_point_dh{1,2}_shape.c): 2L latitudes from the north pole (co-latitude
+// pi i / 2L, no south pole), 2L (DH1) or 4L (DH2) longitudes, L = nmax + 1
+// ---------------------------------------------------------------------------
+void charm_crd_point_dh1_shape(unsigned long nmax, size_t *nlat, size_t *nlon)
+{
+    const size_t L = (size_t)nmax + 1;
+    *nlat = 2 * L;
+    *nlon = 2 * L;
+}
+
+void charm_crd_point_dh2_shape(unsigned long nmax, size_t *nlat, size_t *nlon)
+{
+    const size_t L = (size_t)nmax + 1;
+    *nlat = 2 * L;
+    *nlon = 4 * L;
+}
+
+static charm_point *dh_grid(unsigned long nmax, double r, int type)
+{
+    if (r <= 0.0) return nullptr;
+    size_t nlat, nlon;
+    if (type == CHARM_CRD_POINT_GRID_DH1)
+        charm_crd_point_dh1_shape(nmax, &nlat, &nlon);
+    else
+        charm_crd_point_dh2_shape(nmax, &nlat, &nlon);
+    charm_point *dhg = charm_crd_point_calloc(type, nlat, nlon);
+    if (!dhg) return nullptr;
+    const unsigned long L = nmax + 1;
+    const double pi = 3.14159265358979323846;
+    const double c = 2.0 / (double)L, pi_2L = pi / (2.0 * (double)L);
+    const double thold = charm_glob_threshold;
+    // rings 0 (north pole) .. L (equator); ring i > 0 and its mirror 2L - i
+    // share the weight w = c sin(th) sum_{k<L} sin((2k+1) th)/(2k+1), the
+    // sines by the Chebyshev recurrence
+    auto work = [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; i++) {
+            const double clti = pi_2L * (double)i;
+            dhg->lat[i] = pi / 2.0 - clti;
+            const double sclti = std::sin(clti);
+            double w;
+            if (nmax == 0) {
+                w = c * sclti;
+            } else {
+                double s0 = 0.0, s1 = sclti, s2, acc = s1;
+                const double cc2 = 2.0 * std::cos(clti);
+                for (unsigned long k = 1; k < L; k++) {
+                    s2 = cc2 * s1 - s0;
+                    s0 = s1;
+                    s1 = s2;
+                    s2 = cc2 * s1 - s0;
+                    acc += s2 / (double)(2 * k + 1);
+                    s0 = s1;
+                    s1 = s2;
+                }
+                w = c * sclti * acc;
+            }
+            dhg->w[i] = w;
+            if (i == 0 || charm_amd::nearly_equal(dhg->lat[i], 0.0, thold)) continue;
+            dhg->w[nlat - i] = w;
+            dhg->lat[nlat - i] = -dhg->lat[i];
+        }
+    };
+    const size_t nnorth = L + 1;
+    size_t nthr = std::thread::hardware_concurrency();
+    if (nthr == 0) nthr = 1;
+    if (nthr > 16) nthr = 16;
+    if (nnorth < 64 || nthr == 1) {
+        work(0, nnorth);
+    } else {
+        // later rings cost the same (L terms each): contiguous ranges
+        std::vector<std::thread> th;
+        const size_t chunk = (nnorth + nthr - 1) / nthr;
+        for (size_t t = 0; t < nthr; t++) {
+            const size_t i0 = t * chunk, i1 = std::min(nnorth, i0 + chunk);
+            if (i0 < i1) th.emplace_back(work, i0, i1);
+        }
+        for (auto &x : th) x.join();
+    }
+    for (size_t i = 0; i < nlat; i++) dhg->r[i] = r;
+    const double cl = (type == CHARM_CRD_POINT_GRID_DH1) ? pi / (double)L : pi / (2.0 * (double)L);
+    for (size_t j = 0; j < nlon; j++) dhg->lon[j] = cl * (double)j;
+    return dhg;
+}
+
+charm_point *charm_crd_point_dh1(unsigned long nmax, double r)
+{
+    return dh_grid(nmax, r, CHARM_CRD_POINT_GRID_DH1);
+}
+
+charm_point *charm_crd_point_dh2(unsigned long nmax, double r)
+{
+    return dh_grid(nmax, r, CHARM_CRD_POINT_GRID_DH2);
+}
+
 }  // extern "C"

@@ -110,27 +110,23 @@ __device__ __forceinline__ bool lane_deep(double x, double y, int e)
     return e != 0 && e + max(ex, ey) < -1180;
 }
 
-// lane_renorm that also returns whether the next 8-degree tile is deep
-// (lane_deep of the renormalised state: rescaling keeps the true values, so
-// the exponents read here decide it)
+// lane_renorm without branches, which also returns whether the next 8-degree
+// tile is deep (lane_deep of the renormalised state: rescaling keeps the true
+// values, so the exponents read here decide it).  The same power-of-two
+// scalings as lane_renorm, selected instead of branched: shift by e (both
+// values back in plain range), by -ex (mantissa drifted past 2^+-256) or 0.
 __device__ __forceinline__ void lane_renorm_deep(double &x, double &y, int &e, bool &deep)
 {
-    deep = false;
-    if (e != 0) {
-        int ex, ey;
-        (void)frexp(x, &ex);
-        (void)frexp(y, &ey);
-        deep = e + max(ex, ey) < -1180;
-        if (e + ex > -480 && e + ey > -480) {
-            x = ldexp(x, e);
-            y = ldexp(y, e);
-            e = 0;
-        } else if (ex > 256 || ex < -256) {
-            x = ldexp(x, -ex);
-            y = ldexp(y, -ex);
-            e += ex;
-        }
-    }
+    const int ex = __builtin_amdgcn_frexp_exp(x);
+    const int ey = __builtin_amdgcn_frexp_exp(y);
+    const bool xe = e != 0;
+    const bool plain = xe && e + min(ex, ey) > -480;
+    const bool drift = xe && (ex > 256 || ex < -256);
+    const int sh = plain ? e : (drift ? -ex : 0);
+    deep = xe && e + max(ex, ey) < -1180;
+    x = ldexp(x, sh);
+    y = ldexp(y, sh);
+    e -= sh;
 }
 
 // The 8 alpha values of a tile from the LDS window, read before any other

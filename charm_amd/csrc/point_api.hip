@@ -312,6 +312,10 @@ void h2d(T *dst, const T *src, size_t n, hipStream_t s)
 // Grid geometry shared by the synthesis and analysis paths
 struct GridGeom {
     size_t nlat = 0, nlon = 0, nring = 0;
+    // latitude count the mirror rows are taken from: ring i mirrors to row
+    // mir_nlat - 1 - i (nlat, or nlat + 1 on Driscoll--Healy grids, which
+    // lack the south pole)
+    size_t mir_nlat = 0;
     bool symm = false;
     std::vector<double> t, u;             // [nring]
     std::vector<unsigned char> mirror;    // [nring]
@@ -319,13 +323,32 @@ struct GridGeom {
 
 // Symmetry of a grid's latitudes (src/shs/shs_point_grd.c:133-197,
 // src/crd/crd_grd_check_symm.c:21-72, crd_point_quad_equator.c:22-53,
-// src/misc/misc_arr_chck_symm.c).
+// src/misc/misc_arr_chck_symm.c).  Driscoll--Healy grids count one more
+// latitude, the missing south pole (src/shs/shs_point_grd.c:116-130,
+// src/sha/sha_point.c:190-192): rings 0 (north pole, no mirror) .. L
+// (equator), ring 0 < i < L mirrored to row 2L - i.
 GridGeom grid_geometry(const charm_point *pnt)
 {
     GridGeom g;
     g.nlat = pnt->nlat;
     g.nlon = pnt->nlon;
+    g.mir_nlat = g.nlat;
     const size_t nlat = g.nlat;
+    if (pnt->type == CHARM_CRD_POINT_GRID_DH1 || pnt->type == CHARM_CRD_POINT_GRID_DH2) {
+        const size_t L = nlat / 2;
+        g.symm = true;
+        g.mir_nlat = nlat + 1;
+        g.nring = L + 1;
+        g.t.resize(g.nring);
+        g.u.resize(g.nring);
+        g.mirror.resize(g.nring);
+        for (size_t i = 0; i < g.nring; i++) {
+            g.t[i] = std::sin(pnt->lat[i]);
+            g.u[i] = std::cos(pnt->lat[i]);
+            g.mirror[i] = (i > 0 && i < L) ? 1 : 0;
+        }
+        return g;
+    }
     const bool even = !(nlat % 2);
     const bool gl = pnt->type == CHARM_CRD_POINT_GRID_GL;
     if (nlat == 1) {
@@ -474,7 +497,7 @@ void run_shs_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev, bo
     ShsGridArgs a{};
     a.nmax = (int)nmax;
     a.nring = (int)g.nring;
-    a.nlat = (int)nlat;
+    a.nlat = (int)g.mir_nlat;
     a.unit = d.ratio == nullptr;
     a.t = d.t;
     a.sx = d.sx;
@@ -539,11 +562,6 @@ void shs_grid_host(const charm_point *pnt, const charm_shc *shcs, unsigned long 
                    charm_err *err)
 {
     const size_t nlat = pnt->nlat, nlon = pnt->nlon;
-    if (pnt->type != CHARM_CRD_POINT_GRID && pnt->type != CHARM_CRD_POINT_GRID_GL) {
-        CHARM_AMD_ERR(err, CHARM_EFUNCARG,
-                      "Driscoll--Healy grids are not supported by this implementation.");
-        return;
-    }
     // custom grids: longitudes must be linearly increasing
     // (src/shs/shs_point_grd.c:205-236)
     if (pnt->type == CHARM_CRD_POINT_GRID && nlon > 1) {
@@ -735,9 +753,13 @@ void shs_sctr_host(const charm_point *pnt, const charm_shc *shcs, unsigned long 
 // Analysis
 // ---------------------------------------------------------------------------
 // c = pi/(nmax_grd+1) * 1/(4 pi) * r0/mu (src/sha/sha_point.c:313-322)
-double sha_const(size_t nlat, double r0, double mu)
+// c = pi / (nmax_grd + 1) (Gauss--Legendre, DH1) or pi / (2 nmax_grd + 2)
+// (DH2), times 1/(4 pi) r0/mu (src/sha/sha_point.c:309-322)
+double sha_const(int type, size_t nlat, double r0, double mu)
 {
-    double c = PI / (double)((nlat - 1) + 1);
+    const bool dh = type == CHARM_CRD_POINT_GRID_DH1 || type == CHARM_CRD_POINT_GRID_DH2;
+    const double nmax_grd = dh ? (double)(nlat / 2 - 1) : (double)(nlat - 1);
+    double c = (type == CHARM_CRD_POINT_GRID_DH2) ? PI / (2.0 * nmax_grd + 2.0) : PI / (nmax_grd + 1.0);
     c *= 1.0 / (4.0 * PI) * (r0 / mu);
     return c;
 }
@@ -770,7 +792,7 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
     ShaArgs a{};
     a.nmax = (int)nmax;
     a.nring = (int)g.nring;
-    a.nlat = (int)nlat;
+    a.nlat = (int)g.mir_nlat;
     a.t = d.t;
     a.sx = d.sx;
     a.six = d.six;
@@ -831,16 +853,11 @@ void sha_host(const charm_point *pnt, const double *f, unsigned long nmax, charm
                       "the analysis.");
         return;
     }
-    if (type != CHARM_CRD_POINT_GRID_GL) {
-        CHARM_AMD_ERR(err, CHARM_EFUNCARG,
-                      "Driscoll--Healy grids are not supported by this implementation.");
-        return;
-    }
     hipStream_t s = thread_stream();
     GridGeom g = grid_geometry(pnt);
     Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
     const size_t nr = g.nring;
-    const double c = sha_const(nlat, r0, shcs->mu);
+    const double c = sha_const(type, nlat, r0, shcs->mu);
     std::vector<double> cw(nlat);
     for (size_t i = 0; i < nlat; i++) cw[i] = c * pnt->w[i];
     DevBuf dt(nr * sizeof(double), s), du(nr * sizeof(double), s), dmir(nr, s);
@@ -1108,7 +1125,7 @@ void charm_amd_sha_grid_dev(const charm_amd_grid *grd, unsigned long nmax, doubl
         hipStream_t s = (hipStream_t)stream;
         Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
         const size_t nlat = grd->g.nlat;
-        const double c = sha_const(nlat, grd->r0, mu);
+        const double c = sha_const(grd->type, nlat, grd->r0, mu);
         std::vector<double> cw(nlat);
         for (size_t i = 0; i < nlat; i++) cw[i] = c * grd->wh[i];
         DevBuf dcw(nlat * sizeof(double), s);
@@ -1155,7 +1172,7 @@ void charm_amd_shs_grid_multi_dev(const charm_amd_grid *grd, unsigned long nmax,
         MultiShsArgs a{};
         a.nmax = (int)nmax;
         a.nring = (int)grd->g.nring;
-        a.nlat = (int)nlat;
+        a.nlat = (int)grd->g.mir_nlat;
         a.nset = (int)nset;
         a.mcount = (int)nfc;
         a.nrt = (int)((grd->g.nring + 63) / 64);
@@ -1415,7 +1432,7 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
                 ShsGridArgs a{};
                 a.nmax = nmax;
                 a.nring = (int)g->g.nring;
-                a.nlat = (int)nlat;
+                a.nlat = (int)g->g.mir_nlat;
                 a.k = K;
                 a.ntile = ring_tiles(g->g.nring, K);
                 a.t = g->t;
@@ -1461,7 +1478,7 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
                 ShaArgs a{};
                 a.nmax = nmax;
                 a.nring = (int)g->g.nring;
-                a.nlat = (int)nlat;
+                a.nlat = (int)g->g.mir_nlat;
                 a.k = K;
                 a.ntile = ring_tiles(g->g.nring, K);
                 a.t = g->t;
@@ -1469,7 +1486,7 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
                 a.six = g->six;
                 a.sld = g->sld;
                 a.mirror = g->mirror;
-                const double c = sha_const(nlat, g->r0, q.mu);
+                const double c = sha_const(g->type, nlat, g->r0, q.mu);
                 std::vector<double> cw(nlat);
                 for (size_t r = 0; r < nlat; r++) cw[r] = c * g->wh[r];
                 a.cw = dupload(b, cw, s);

@@ -139,6 +139,14 @@ void charm_crd_point_free(charm_point *pnt);
 charm_point *charm_crd_point_gl(unsigned long nmax, double r);
 /* src/crd/crd.h.in:626-628 */
 void charm_crd_point_gl_shape(unsigned long nmax, size_t *nlat, size_t *nlon);
+/* src/crd/crd.h.in:650-653 (non-equiangular Driscoll--Healy grid) */
+charm_point *charm_crd_point_dh1(unsigned long nmax, double r);
+/* src/crd/crd.h.in:662-664 */
+void charm_crd_point_dh1_shape(unsigned long nmax, size_t *nlat, size_t *nlon);
+/* src/crd/crd.h.in:674-677 (equiangular Driscoll--Healy grid) */
+charm_point *charm_crd_point_dh2(unsigned long nmax, double r);
+/* src/crd/crd.h.in:686-688 */
+void charm_crd_point_dh2_shape(unsigned long nmax, size_t *nlat, size_t *nlon);
 
 /* ------------------------------------------------------------------------ */
 /* Global tunables: src/glob/glob.h.in:66-149, src/glob/glob.c:16-23        */

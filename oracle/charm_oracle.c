@@ -467,6 +467,54 @@ int orc_gl_grid(unsigned long nmax, double *lat_o, double *w_o, double *lon_o)
     return err;
 }
 
+/* Driscoll--Healy grids (crd_point_dh_chunk.c:19-150,
+ * crd_point_dh_lat_w_chunk.c:16-100, crd_point_dh{1,2}_shape.c): type 4 = DH1
+ * (nlon = 2L), 5 = DH2 (nlon = 4L); nlat = 2L, L = nmax+1, co-latitudes
+ * pi i/(2L) from the north pole (no south pole).  Weights
+ * w = (2/L) sin(th) sum_{k<L} sin((2k+1) th)/(2k+1), the sines by the
+ * Chebyshev recurrence in CHarm's order; rows i and 2L-i share them.
+ * Returns 0 on success, 1 for an unknown type. */
+int orc_dh_grid(unsigned long nmax, int type, double *lat_o, double *w_o, double *lon_o)
+{
+    init_consts();
+    if (type != 4 && type != 5) return 1;
+    const unsigned long L = nmax + 1;
+    const size_t nlat = 2 * L, nlon = (type == 4) ? 2 * L : 4 * L;
+    const real Lf = (real)L, c = (real)2 / Lf, pi_2L = RPI / ((real)2 * Lf);
+    real *lat = (real *)calloc(nlat, sizeof(real));
+    real *w = (real *)calloc(nlat, sizeof(real));
+    for (size_t i = 0; i <= L; i++) {
+        const real clti = pi_2L * (real)i;
+        lat[i] = RPI / 2 - clti;
+        const real sclti = RSIN(clti);
+        if (nmax == 0) {
+            w[i] = c * sclti;
+        } else {
+            real s0 = 0, s1 = sclti, s2, cc2 = (real)2 * RCOS(clti), acc = s1;
+            for (unsigned long k = 1; k < L; k++) {
+                s2 = cc2 * s1 - s0;
+                s0 = s1;
+                s1 = s2;
+                s2 = cc2 * s1 - s0;
+                acc += s2 / (real)(2 * k + 1);
+                s0 = s1;
+                s1 = s2;
+            }
+            w[i] = c * sclti * acc;
+        }
+        if (i == 0) continue;                                /* no south pole */
+        if (nearly_equal(lat[i], 0, GLOB_THRESHOLD)) continue; /* equator */
+        w[nlat - i] = w[i];
+        lat[nlat - i] = -lat[i];
+    }
+    for (size_t i = 0; i < nlat; i++) { st(lat_o, nlat, i, lat[i]); st(w_o, nlat, i, w[i]); }
+    const real cl = (type == 4) ? RPI / Lf : RPI / ((real)2 * Lf);
+    for (size_t j = 0; j < nlon; j++) st(lon_o, nlon, j, cl * (real)j);
+    free(lat);
+    free(w);
+    return 0;
+}
+
 /* Point synthesis on a grid (shs_point_grd.c:97-1076).
  *   gl      : 1 if the grid is a Gauss-Legendre grid (then it is symmetric
  *             by definition and the southern rings reuse the northern
@@ -661,28 +709,61 @@ int orc_shs_sctr(unsigned long nmax, unsigned long nmax_shc, const double *C_i,
  *   C, S     : outputs, m-major triangles of degree nmax_shc >= nmax; every
  *              entry is written (zeros above nmax, sha_point.c:381).
  * Returns 0 on success, 1 on invalid input. */
+static int orc_sha_quad(int type, unsigned long nmax, unsigned long nmax_shc, size_t nlat,
+                        size_t nlon, const double *f_i, const double *lat_i, const double *w_i,
+                        const double *r0_i, const double *mu_i, const double *R_i,
+                        double *C_o, double *S_o);
+
 int orc_sha_gl(unsigned long nmax, unsigned long nmax_shc, size_t nlat, size_t nlon,
                const double *f_i, const double *lat_i, const double *w_i,
                const double *r0_i, const double *mu_i, const double *R_i,
                double *C_o, double *S_o)
 {
+    return orc_sha_quad(3, nmax, nmax_shc, nlat, nlon, f_i, lat_i, w_i, r0_i, mu_i, R_i, C_o, S_o);
+}
+
+/* The same on a Driscoll--Healy grid (type 4 = DH1, 5 = DH2): nmax_grd =
+ * nlat/2 - 1, c = pi/(nmax_grd+1) (DH1) or pi/(2 nmax_grd+2) (DH2)
+ * (sha_point.c:313-317); the grid counts one more latitude (the missing
+ * south pole, sha_point.c:190-192), the north pole has no mirror, ring i
+ * < nlat/2 mirrors to row nlat - i (crd_grd_check_symm.c:40-52). */
+int orc_sha_dh(int type, unsigned long nmax, unsigned long nmax_shc, size_t nlat, size_t nlon,
+               const double *f_i, const double *lat_i, const double *w_i,
+               const double *r0_i, const double *mu_i, const double *R_i,
+               double *C_o, double *S_o)
+{
+    if (type != 4 && type != 5) return 1;
+    return orc_sha_quad(type, nmax, nmax_shc, nlat, nlon, f_i, lat_i, w_i, r0_i, mu_i, R_i, C_o,
+                        S_o);
+}
+
+static int orc_sha_quad(int type, unsigned long nmax, unsigned long nmax_shc, size_t nlat,
+                        size_t nlon, const double *f_i, const double *lat_i, const double *w_i,
+                        const double *r0_i, const double *mu_i, const double *R_i,
+                        double *C_o, double *S_o)
+{
     init_consts();
-    if (nlat == 0 || nmax > nlat - 1 || nmax > nmax_shc) return 1;
-    const unsigned long nmax_grd = nlat - 1;
+    const int dh = type != 3;
+    if (nlat == 0 || (dh && nlat % 2)) return 1;
+    const unsigned long nmax_grd = dh ? nlat / 2 - 1 : nlat - 1;
+    if (nmax > nmax_grd || nmax > nmax_shc) return 1;
     real *f = ld_all(f_i, nlat * nlon), *lat = ld_all(lat_i, nlat), *w = ld_all(w_i, nlat);
     real r0 = ld(r0_i, 1, 0), mu = ld(mu_i, 1, 0), R = ld(R_i, 1, 0);
     const size_t ncs = tri_len(nmax_shc);
     real *C = (real *)calloc(ncs, sizeof(real)), *S = (real *)calloc(ncs, sizeof(real));
 
-    /* c = pi/(nmax_grd+1) * 1/(4 pi) * r0/mu (sha_point.c:313-322) */
-    real c = RPI / (real)(nmax_grd + 1);
+    /* c = pi/(nmax_grd+1) (GL, DH1) or pi/(2 nmax_grd + 2) (DH2), times
+     * 1/(4 pi) * r0/mu (sha_point.c:313-322) */
+    real c = (type == 5) ? RPI / (real)(2 * nmax_grd + 2) : RPI / (real)(nmax_grd + 1);
     c *= ((real)1) / ((real)4 * RPI) * (r0 / mu);
 
     real *ct, *stb;
     twiddles(nlon, &ct, &stb);
-    const int even = !(nlat % 2);
-    const size_t equator = (!(nmax_grd % 2)) ? nlat / 2 : (size_t)-1;
-    const size_t nlatdo = (nlat > 1) ? (nlat + 1 - even) / 2 : nlat;
+    /* DH grids count the missing south pole (sha_point.c:190-192) */
+    const size_t nlat_s = dh ? nlat + 1 : nlat;
+    const int even = !(nlat_s % 2);
+    const size_t equator = dh ? nlat / 2 : ((!(nmax_grd % 2)) ? nlat / 2 : (size_t)-1);
+    const size_t nlatdo = (nlat_s > 1) ? (nlat_s + 1 - even) / 2 : nlat_s;
     /* lumped sums per ring pair: A, B for the ring and A2, B2 for its mirror */
     real *re = (real *)malloc((nmax + 1) * sizeof(real)), *mim = (real *)malloc((nmax + 1) * sizeof(real));
     real *amp = (real *)malloc((nmax + 1) * nlatdo * sizeof(real));
@@ -694,13 +775,13 @@ int orc_sha_gl(unsigned long nmax, unsigned long nmax_shc, size_t nlat, size_t n
     for (size_t i = 0; i < nlatdo; i++) {
         tt[i] = RSIN(lat[i]);
         uu[i] = RCOS(lat[i]);
-        int symm = (nlat > 1) && (i < equator);
+        int symm = (nlat > 1) && (i < equator) && !(dh && i == 0);
         dft_r2c_direct(nlon, f + i * nlon, nmax, ct, stb, re, mim);
         real cw = c * w[i];
         for (unsigned long k = 0; k <= nmax; k++) { A[k] = cw * re[k]; B[k] = cw * mim[k]; }
         real *A2 = (real *)calloc(nmax + 1, sizeof(real)), *B2 = (real *)calloc(nmax + 1, sizeof(real));
         if (symm) {
-            size_t is = nlat - 1 - i;
+            size_t is = nlat_s - 1 - i;
             dft_r2c_direct(nlon, f + is * nlon, nmax, ct, stb, re, mim);
             real cw2 = c * w[is];
             for (unsigned long k = 0; k <= nmax; k++) { A2[k] = cw2 * re[k]; B2[k] = cw2 * mim[k]; }

@@ -51,6 +51,9 @@ class Oracle:
         L.orc_lumped.argtypes = [_ul, _ul, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_shs_sctr.argtypes = [_ul, _ul, _dp, _dp, _dp, _dp, _sz, _dp, _dp, _dp, _dp]
         L.orc_sha_gl.argtypes = [_ul, _ul, _sz, _sz, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_dh_grid.argtypes = [_ul, ct.c_int, _dp, _dp, _dp]
+        L.orc_sha_dh.argtypes = [ct.c_int, _ul, _ul, _sz, _sz, _dp, _dp, _dp, _dp, _dp, _dp,
+                                 _dp, _dp]
         L.orc_leg_pnm.argtypes = [_ul, _dp, _dp]
         L.orc_set_polar.argtypes = [_ul, ct.c_double]
 
@@ -112,6 +115,17 @@ class Oracle:
         return (self.unpack(lat, L, exact), self.unpack(w, L, exact),
                 self.unpack(lon, 2 * L, exact))
 
+    def dh_grid(self, nmax, dh, exact=False):
+        """Driscoll-Healy grid dh = 1 or 2: (lat, w) of 2L rings, lon."""
+        L = nmax + 1
+        nlat, nlon = 2 * L, (2 if dh == 1 else 4) * L
+        lat, w, lon = self.empty(nlat), self.empty(nlat), self.empty(nlon)
+        rc = self.lib.orc_dh_grid(nmax, 3 + dh, self._p(lat), self._p(w), self._p(lon))
+        if rc:
+            raise ValueError("dh must be 1 or 2")
+        return (self.unpack(lat, nlat, exact), self.unpack(w, nlat, exact),
+                self.unpack(lon, nlon, exact))
+
     def shs_grid(self, nmax, C, S, mu, R, lat, r, lon, gl=True, rings=None, exact=False):
         nmax_shc = _nmax_from_len(len(C))
         nlat, nlon = len(lat), len(lon)
@@ -157,6 +171,20 @@ class Oracle:
         args = [self.pack(v) for v in (f, lat, w, [r0], [mu], [R])]
         rc = self.lib.orc_sha_gl(nmax, nmax_shc, nlat, nlon, *(self._p(a) for a in args),
                                  self._p(C), self._p(S))
+        if rc:
+            raise ValueError("invalid SHA input")
+        return self.unpack(C, ncs, exact), self.unpack(S, ncs, exact)
+
+    def sha_dh(self, dh, nmax, f, lat, w, r0, mu, R, nmax_shc=None, exact=False):
+        """Analysis on a Driscoll-Healy grid (dh = 1 or 2)."""
+        f = np.asarray(f)
+        nlat, nlon = f.shape
+        nmax_shc = nmax if nmax_shc is None else nmax_shc
+        ncs = (nmax_shc + 1) * (nmax_shc + 2) // 2
+        C, S = self.empty(ncs), self.empty(ncs)
+        args = [self.pack(v) for v in (f, lat, w, [r0], [mu], [R])]
+        rc = self.lib.orc_sha_dh(3 + dh, nmax, nmax_shc, nlat, nlon,
+                                 *(self._p(a) for a in args), self._p(C), self._p(S))
         if rc:
             raise ValueError("invalid SHA input")
         return self.unpack(C, ncs, exact), self.unpack(S, ncs, exact)

@@ -82,6 +82,21 @@ def test_gl_grid_matches_fixtures(nmax):
         assert (g.r == 1.0 + 1000.0 * d).all()
 
 
+@pytest.mark.parametrize("nmax", range(0, 11))
+@pytest.mark.parametrize("dh", (1, 2))
+def test_dh_grid_matches_fixtures(nmax, dh):
+    """charm_crd_point_dh1/dh2 against CHarm's crd_nx*_dr*_dh{1,2} files at
+    CHarm's tolerance (tests/check_crd_point_quad.c:59-76)."""
+    for d in (0, 1):
+        g = (crd.PointGridDH1 if dh == 1 else crd.PointGridDH2)(nmax, 1.0 + 1000.0 * d)
+        assert (g.lat.size, g.lon.size) == (crd.PointGridDH1 if dh == 1 else
+                                            crd.PointGridDH2).shape(nmax)
+        name = "crd_nx%d_dr%d_dh%d_%%s.txt" % (nmax, d, dh)
+        for key, got in (("lat", g.lat), ("lon", g.lon), ("w", g.w), ("r", g.r)):
+            ref = read_fixture("double", name % key)
+            assert nearly_equal(got, ref, 10 * THRESH["double"]).all(), key
+
+
 def test_gl_grid_matches_oracle_large():
     from oracle.oracle import Oracle
 

@@ -53,6 +53,56 @@ def test_gl_grid_fixtures(orc, prec, p, nmax):
 
 
 @pytest.mark.parametrize("prec,p", PRECS)
+@pytest.mark.parametrize("nmax", range(0, 11))
+@pytest.mark.parametrize("dh", (1, 2))
+def test_dh_grid_fixtures(orc, prec, p, nmax, dh):
+    exact = p == "q"
+    lat, w, lon = orc[p].dh_grid(nmax, dh, exact=exact)
+    eps = 10 * THRESH[prec]
+    for d in (0, 1):
+        name = "crd_nx%d_dr%d_dh%d_%%s.txt" % (nmax, d, dh)
+        _check(prec, lat, read_fixture(prec, name % "lat", exact), eps)
+        _check(prec, w, read_fixture(prec, name % "w", exact), eps)
+        _check(prec, lon, read_fixture(prec, name % "lon", exact), eps)
+        assert all(float(v) == 1.0 + DELTAR * d for v in read_fixture(prec, name % "r", exact))
+
+
+@pytest.mark.parametrize("prec,p", PRECS)
+@pytest.mark.parametrize("nmax", range(0, 5))
+@pytest.mark.parametrize("deltar", (0, 1))
+@pytest.mark.parametrize("dh", (1, 2))
+def test_shs_dh_fixtures(orc, prec, p, nmax, deltar, dh):
+    """Synthesis on Driscoll-Healy grids (CHarm's shs_p_*_fft1_dh{1,2} files);
+    the oracle evaluates every ring directly (no south pole, so no mirror)."""
+    exact = p == "q"
+    mu, R, C, S = egm96_test_model(exact)
+    lat, w, lon = orc[p].dh_grid(nmax, dh, exact=exact)
+    rr = _radius(prec, R, deltar)
+    r = np.array([rr] * len(lat), dtype=object if exact else np.float64)
+    f = orc[p].shs_grid(nmax, C, S, mu, R, lat, r, lon, gl=False, exact=exact)
+    ref = read_fixture(prec, "shs_p_nx%d_dr%d_fft1_dh%d.txt" % (nmax, deltar, dh), exact)
+    _check(prec, f.ravel(), ref, 10 * THRESH[prec])
+
+
+@pytest.mark.parametrize("nmax", (0, 1, 6, 25))
+@pytest.mark.parametrize("dh", (1, 2))
+def test_sha_dh_roundtrip(orc, nmax, dh):
+    """Driscoll-Healy quadrature is exact to degree nmax: the analysis of a
+    synthesis returns the coefficients (as CHarm's check_sha_point.c checks)."""
+    rng = np.random.default_rng(100 + nmax)
+    ncs = (nmax + 1) * (nmax + 2) // 2
+    C, S = rng.standard_normal(ncs), rng.standard_normal(ncs)
+    S[:nmax + 1] = 0.0   # S_n0
+    mu, R = 3.986004415e14, 6378136.3
+    lat, w, lon = orc["d"].dh_grid(nmax, dh)
+    r = np.full(len(lat), R)
+    f = orc["d"].shs_grid(nmax, C, S, mu, R, lat, r, lon, gl=False)
+    Co, So = orc["d"].sha_dh(dh, nmax, f, lat, w, R, mu, R)
+    assert np.abs(Co - C).max() <= 1e-12 * np.abs(C).max()
+    assert np.abs(So - S).max() <= 1e-12 * np.abs(C).max()
+
+
+@pytest.mark.parametrize("prec,p", PRECS)
 @pytest.mark.parametrize("nmax", range(0, 5))
 @pytest.mark.parametrize("deltar", (0, 1))
 def test_shs_gl_fixtures(orc, prec, p, nmax, deltar):
