@@ -151,3 +151,38 @@ def nearly_equal_exact(x, ref, eps):
             else:
                 ok.append(d / max(abs(a), abs(b)) <= eps)
     return np.array(ok)
+
+
+# tests/check_shs_point_all.c:537-600: custom grids (nlat, nlon) of CHarm's
+# point-synthesis test, fft = 0 (longitudes over pi, no FFT) or 1 (over 2 pi)
+CUSTOM_GRIDS = [(1, 1), (1, 2), (3, 8), (10, 22)]
+
+
+def custom_grid(i, fft, symm, prec="double"):
+    """lat, lon of custom grid i as tests/generate_point.c:19-53 makes it
+    (lat from pi/2 down over pi, lon = l/nlon * range), the first latitude
+    moved by BREAK_SYMM (tests/check_shs_point_all.c:589-595) when symm is
+    false; arithmetic in the precision of the build.  None for the
+    combinations the reference test skips."""
+    nlat, nlon = CUSTOM_GRIDS[i]
+    if nlat == 1 and symm:
+        return None
+    nlat2 = nlat - 1 if nlat > 1 else nlat
+    if prec == "double":
+        lat = np.array([np.pi / 2 - (float(l) / float(nlat2)) * np.pi for l in range(nlat)])
+        lon_rng = 2.0 * np.pi if fft else np.pi
+        lon = np.array([(float(l) / float(nlon)) * lon_rng for l in range(nlon)])
+        if not symm:
+            lat[0] -= BREAK_SYMM
+        return lat, lon
+    import mpmath
+
+    with mpmath.workprec(113):
+        PI = +mpmath.pi
+        lat = np.array([PI / 2 - (mpmath.mpf(l) / nlat2) * PI for l in range(nlat)], dtype=object)
+        lon_rng = 2 * PI if fft else PI
+        lon = np.array([(mpmath.mpf(l) / nlon) * lon_rng for l in range(nlon)], dtype=object)
+        if not symm:
+            lat[0] = lat[0] - mpmath.mpf(BREAK_SYMM)
+    return lat, lon
+

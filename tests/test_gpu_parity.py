@@ -14,8 +14,8 @@ import pytest
 import charm_amd
 from charm_amd import crd, sha, shc, shs
 from charm_amd.workloads import GM, R_EARTH, kaula_coeffs, scattered_points
-from charm_fixtures import (BREAK_SYMM, DELTAR, THRESH, THRESH2, egm96_test_model, nearly_equal,
-                            read_fixture)
+from charm_fixtures import (BREAK_SYMM, CUSTOM_GRIDS, DATA, DELTAR, THRESH, THRESH2, custom_grid,
+                            egm96_test_model, nearly_equal, read_fixture)
 from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -59,6 +59,35 @@ def test_shs_gl_fixtures(nmax, deltar):
     # error against CHarm built in quad precision
     refq = read_fixture("quad", "shs_p_nx%d_dr%d_fft1_gl.txt" % (nmax, deltar))
     assert relmax(f, refq) <= TOL
+
+
+def _custom_cases():
+    import os
+    out = []
+    for nmax in range(0, 5):
+        for i in range(len(CUSTOM_GRIDS)):
+            for fft in (0, 1):
+                for s in (0, 1):
+                    name = "shs_p_nx%d_n%d_dr%%d_fft%d_s%d.txt" % (nmax, i, fft, s)
+                    if os.path.exists(os.path.join(DATA, "double", name % 0)):
+                        out.append((nmax, i, fft, s, name))
+    return out
+
+
+@pytest.mark.parametrize("nmax,i,fft,s,name", _custom_cases())
+def test_shs_custom_grid_fixtures(nmax, i, fft, s, name):
+    """Custom grids of CHarm's point-synthesis test (tests/check_shs_point_all.c:
+    537-620): with the FFT along parallels (fft1) and with direct longitude
+    sums (fft0, half the circle), symmetric and not, at R and R + 1 km."""
+    mu, R, C, S = egm96_test_model()
+    m = model(10, C, S, mu, R)
+    lat, lon = custom_grid(i, fft, s == 1)
+    for deltar in (0, 1):
+        g = crd.PointGrid(lat, lon, np.full(lat.size, R + DELTAR * deltar))
+        f = shs.point(g, m, nmax).ravel()
+        ref = read_fixture("double", name % deltar)
+        assert nearly_equal(f, ref, 100 * THRESH["double"]).all(), relmax(f, ref)
+        assert relmax(f, read_fixture("quad", name % deltar)) <= TOL
 
 
 @pytest.mark.parametrize("nmax", range(0, 5))

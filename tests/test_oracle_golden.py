@@ -8,8 +8,8 @@ Tolerances are the ones CHarm's own test-suite applies to the same files
 import numpy as np
 import pytest
 
-from charm_fixtures import (BREAK_SYMM, DELTAR, THRESH, THRESH2, egm96_test_model,
-                            nearly_equal, nearly_equal_exact, read_fixture,
+from charm_fixtures import (BREAK_SYMM, DELTAR, THRESH, THRESH2, custom_grid,
+                            egm96_test_model, nearly_equal, nearly_equal_exact, read_fixture,
                             scattered_points, truncate)
 from oracle.oracle import Oracle
 
@@ -82,6 +82,37 @@ def test_shs_dh_fixtures(orc, prec, p, nmax, deltar, dh):
     f = orc[p].shs_grid(nmax, C, S, mu, R, lat, r, lon, gl=False, exact=exact)
     ref = read_fixture(prec, "shs_p_nx%d_dr%d_fft1_dh%d.txt" % (nmax, deltar, dh), exact)
     _check(prec, f.ravel(), ref, 10 * THRESH[prec])
+
+
+def _custom_cases():
+    out = []
+    for nmax in range(0, 5):
+        for i in range(4):
+            for fft in (0, 1):
+                for s in (0, 1):
+                    for deltar in (0, 1):
+                        name = "shs_p_nx%d_n%d_dr%d_fft%d_s%d.txt" % (nmax, i, deltar, fft, s)
+                        import os
+                        from charm_fixtures import DATA
+                        if os.path.exists(os.path.join(DATA, "double", name)):
+                            out.append((nmax, i, fft, s, deltar, name))
+    return out
+
+
+@pytest.mark.parametrize("prec,p", PRECS)
+@pytest.mark.parametrize("nmax,i,fft,s,deltar,name", _custom_cases())
+def test_shs_custom_grid_fixtures(orc, prec, p, nmax, i, fft, s, deltar, name):
+    """Custom grids of CHarm's point-synthesis test, with (fft1) and without
+    (fft0, longitudes over half the circle) the FFT along parallels,
+    symmetric (s1) and not (s0)."""
+    exact = p == "q"
+    mu, R, C, S = egm96_test_model(exact)
+    lat, lon = custom_grid(i, fft, s == 1, prec)
+    rr = _radius(prec, R, deltar)
+    r = np.array([rr] * len(lat), dtype=object if exact else np.float64)
+    f = orc[p].shs_grid(nmax, C, S, mu, R, lat, r, lon, gl=False, exact=exact)
+    ref = read_fixture(prec, name, exact)
+    _check(prec, f.ravel(), ref, 100 * THRESH[prec])
 
 
 @pytest.mark.parametrize("nmax", (0, 1, 6, 25))
