@@ -39,7 +39,7 @@ UNIT = "(nmax+1)^2*nlat/s"
 # K = 2 for synthesis), measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE in separate runs and scaled by a 256 MiB device copy measured in
 # the same runs (HBM bytes = 2.0 FETCH_SIZE + 1.0 WRITE_SIZE).
-TRAFFIC_GL2160 = {"sha_kernel": 408.4e6, "shs_grid_kernel": 166.1e6}
+TRAFFIC_GL2160 = {"sha_kernel": 405.6e6, "shs_grid_kernel": 165.4e6}
 
 
 def kernel_ms(lib, name):

@@ -76,8 +76,9 @@ struct ShaArgs {
     size_t lds;
     const double *al;  // alpha_nm, m-major, degree al_nt
     int al_nt;
-    void *part;        // [tile][tri_len(nmax)] double2 partial sums (before s_nm)
+    void *part;        // [slot][tri_len(nmax)] double2 partial sums (before s_nm)
     int k;             // ring pairs per lane (2..4)
+    int tpg;           // ring tiles per block, added in tile order into slot tile / tpg
     const double *u;   // [nring] cos(lat)
     double polar;      // polar optimisation threshold (< 0: off)
 };

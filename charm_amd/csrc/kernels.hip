@@ -662,6 +662,7 @@ struct WaveReducer {
     int cnt;          // degrees in the pending tile
     double r[16];     // pending values read back from LDS
     int lane;
+    bool acc;         // add to dst (a later ring tile of the same slot)
 
     // The reads are issued whether or not a tile is pending (the buffer is
     // always this wave's own LDS; unused values are dropped by finish), so
@@ -686,7 +687,9 @@ struct WaveReducer {
             s += dpp_quad_xor1(s);
             s += dpp_quad_xor2(s);
             const int j = lane >> 2;
-            if ((lane & 3) == 0 && j < 2 * cnt) dst[j] = s;
+            // the same lane owns output j for every tile of the slot, so the
+            // earlier tile's value is its own store
+            if ((lane & 3) == 0 && j < 2 * cnt) dst[j] = acc ? dst[j] + s : s;
             dst = nullptr;
         }
     }
@@ -719,8 +722,11 @@ struct ShaSmem {
     double al[4][64];
 };
 
+// Ring tile `tile` of order quad `quad`; its sums go to partial slot `slot`,
+// written (acc false: the slot's first tile) or added to (acc true).
 template <int K>
-__device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, ShaSmem<K> &sm)
+__device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, int slot, bool acc,
+                                         ShaSmem<K> &sm)
 {
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
@@ -776,7 +782,7 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
     __syncthreads();
     if (m > nmax) return;
 
-    double *dst = (double *)((double2 *)A.part + (size_t)tile * tri_len_dev(nmax) + dcol_off(nmax, m) - m);
+    double *dst = (double *)((double2 *)A.part + (size_t)slot * tri_len_dev(nmax) + dcol_off(nmax, m) - m);
     // polar optimisation: order m is dropped at rings with m - nmax u > polar
     // (src/misc/misc_polar_optimization_apply.c:16-41, tested per ring here)
     bool skip[K], allskip = true;
@@ -786,10 +792,11 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
         allskip = allskip && skip[k];
     }
     if (wave_all(allskip)) {
-        for (int n = m + lane; n <= nmax; n += 64) {
-            dst[2 * n] = 0.0;
-            dst[2 * n + 1] = 0.0;
-        }
+        if (!acc)
+            for (int n = m + lane; n <= nmax; n += 64) {
+                dst[2 * n] = 0.0;
+                dst[2 * n + 1] = 0.0;
+            }
         return;
     }
     WaveReducer rd;
@@ -797,6 +804,7 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
     rd.dst = nullptr;
     rd.cnt = 0;
     rd.lane = lane;
+    rd.acc = acc;
 
     double x[K], y[K];
     int e[K];
@@ -868,7 +876,7 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, S
                 }
             }
             if (PIPE) rd.finish();
-            if (lane < 16) dst[2 * n + lane] = 0.0;
+            if (lane < 16 && !acc) dst[2 * n + lane] = 0.0;
         } else {
 #pragma unroll
             for (int j = 0; j < 8; j++) {
@@ -969,8 +977,14 @@ void sha_kernel(ShaArgs A)
 {
     __shared__ __align__(16) ShaSmem<K> sm;
     const int b = blockIdx.x;
-    const int s = b >> 3, h = s & 1, tile = (s >> 1) % A.ntile, g = (s >> 1) / A.ntile;
-    sha_item<K>(A, tile, 2 * (8 * g + (b & 7)) + h, sm);
+    const int ngrp = (A.ntile + A.tpg - 1) / A.tpg;
+    const int s = b >> 3, h = s & 1, grp = (s >> 1) % ngrp, g = (s >> 1) / ngrp;
+    const int quad = 2 * (8 * g + (b & 7)) + h;
+    const int t1 = min(A.ntile, (grp + 1) * A.tpg);
+    for (int tile = grp * A.tpg; tile < t1; tile++) {
+        if (tile > grp * A.tpg) __syncthreads();  // the block's LDS is reused
+        sha_item<K>(A, tile, quad, grp, tile > grp * A.tpg, sm);
+    }
 }
 
 template <int K>
@@ -989,7 +1003,7 @@ __global__ __launch_bounds__(256) void sha_batch_kernel(const ShaArgs *__restric
         const int2 w = items[it];
         const ShaArgs A = reqs[uniform(w.x)];
         const int wy = uniform(w.y);
-        sha_item<K>(A, wy >> 16, wy & 0xFFFF, sm);
+        sha_item<K>(A, wy >> 16, wy & 0xFFFF, wy >> 16, false, sm);
     }
 }
 
@@ -1430,7 +1444,8 @@ int launch_sha(const ShaArgs &a, void *stream)
 {
     const int nquad = (a.nmax + 1 + 3) / 4;
     const int nqg = (nquad + 15) / 16;  // groups of 8 quad pairs
-    const unsigned nblk = 16u * (unsigned)nqg * (unsigned)a.ntile;
+    const int ngrp = (a.ntile + a.tpg - 1) / a.tpg;
+    const unsigned nblk = 16u * (unsigned)nqg * (unsigned)ngrp;
     switch (a.k) {
     case 2: hipLaunchKernelGGL((sha_kernel<2>), dim3(nblk), dim3(256), 0, S(stream), a); break;
     case 3: hipLaunchKernelGGL((sha_kernel<3>), dim3(nblk), dim3(256), 0, S(stream), a); break;

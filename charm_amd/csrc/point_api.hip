@@ -126,6 +126,28 @@ void *pinned_buffer(size_t bytes)
 }
 
 // stream-ordered device buffer
+// Per-call scratch comes from the device's default stream-ordered pool.  Its
+// release threshold is raised once per device so that freed scratch stays
+// in the pool across calls: with the default threshold of 0 every
+// synchronisation (hipFFT's, a caller's) hands the memory back to the driver
+// and the next call maps it again, which at nmax = 10800 (several GB of
+// spectra per call) costs more than the transforms.
+void keep_pool_memory()
+{
+    static std::mutex mu;
+    static std::vector<char> done;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((size_t)dev >= done.size()) done.resize(dev + 1, 0);
+    if (done[dev]) return;
+    done[dev] = 1;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) return;
+    uint64_t keep = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+}
+
 struct DevBuf {
     void *p = nullptr;
     hipStream_t s = nullptr;
@@ -134,6 +156,7 @@ struct DevBuf {
     void alloc(size_t bytes, hipStream_t st)
     {
         s = st;
+        keep_pool_memory();
         if (bytes) HIPCHK(hipMallocAsync(&p, bytes, st));
     }
     DevBuf(const DevBuf &) = delete;
@@ -779,6 +802,9 @@ bool rescale_table(double r0, double R, unsigned long nmax_out, std::vector<doub
     return true;
 }
 
+// blocks an analysis launch aims at (a few waves of resident blocks)
+constexpr int SHA_BLOCKS = 4096;
+
 void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const double *f_dev,
              unsigned long nmax, const Tables &tb, const double *scale_dev, unsigned long nmax_out,
              double *c_dev, double *s_dev, size_t stride, hipStream_t s)
@@ -805,23 +831,33 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
     a.al_nt = tb.nt;
     a.u = d.u;
     a.polar = polar_threshold(nmax);
-    // partial sums sized for the smallest tile width (most tiles)
-    DevBuf part((size_t)ring_tiles(g.nring, 2) * tri_len(nmax) * 2 * sizeof(double), s);
+    // Ring tiles are grouped so that the launch has about SHA_BLOCKS blocks:
+    // each block adds its group's tiles, in order, into one partial slot.
+    // Few slots keep the partial sums small at high degree (one per tile
+    // would be 20-40 GB at nmax = 10800).
+    const int nquad = (int)(nmax + 1 + 3) / 4;
+    const int nslot = std::max(1, std::min(ring_tiles(g.nring, 2), (SHA_BLOCKS + nquad - 1) / nquad));
+    auto group = [&](ShaArgs &y) {
+        y.ntile = ring_tiles(g.nring, y.k);
+        y.tpg = (y.ntile + nslot - 1) / nslot;
+    };
+    DevBuf part((size_t)nslot * tri_len(nmax) * 2 * sizeof(double), s);
     a.part = part.p;
     a.k = tuned_k(1, (int)g.nring, nmax, 2, 4, s, [&](int k) {
         ShaArgs y = a;
         y.k = k;
-        y.ntile = ring_tiles(g.nring, k);
+        group(y);
         HIPCHK((hipError_t)launch_sha(y, s));
     });
-    a.ntile = ring_tiles(g.nring, a.k);
+    group(a);
     {
         ProfScope ps("sha_kernel", s);
         HIPCHK((hipError_t)launch_sha(a, s));
     }
     {
         ProfScope ps("sha_reduce_kernel", s);
-        HIPCHK((hipError_t)launch_sha_reduce((int)nmax, a.ntile, part.p, tb.sn, tb.nt,
+        const int nused = (a.ntile + a.tpg - 1) / a.tpg;
+        HIPCHK((hipError_t)launch_sha_reduce((int)nmax, nused, part.p, tb.sn, tb.nt,
                                              (int)nmax_out, scale_dev, c_dev, s_dev, stride, s));
     }
 }
@@ -1481,6 +1517,7 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
                 a.nlat = (int)g->g.mir_nlat;
                 a.k = K;
                 a.ntile = ring_tiles(g->g.nring, K);
+                a.tpg = 1;
                 a.t = g->t;
                 a.sx = g->sx;
                 a.six = g->six;
