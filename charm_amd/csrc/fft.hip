@@ -481,10 +481,19 @@ int get_plan(int L, hipStream_t s, Plan *out)
     return 0;
 }
 
+// Whether an LDS plan exists for L; the search over Bluestein lengths costs
+// O(L) factorisations, so the answer is kept per L
 bool plan_exists(int L)
 {
+    static std::mutex mu;
+    static std::map<int, bool> known;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = known.find(L);
+    if (it != known.end()) return it->second;
     fft::MixPlan mp;
-    return fft::choose_plan(L, mp);
+    const bool ok = fft::choose_plan(L, mp);
+    known[L] = ok;
+    return ok;
 }
 
 }  // namespace

@@ -148,22 +148,101 @@ void keep_pool_memory()
     (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
 }
 
+// Scratch released by a call stays cached for later calls, by stream and
+// size (the stream-ordered allocator slows down badly with thousands of
+// allocations queued behind a deep stream).  A cached buffer carries an event
+// recorded after its last use; the next user waits for it on its own stream,
+// so reuse is ordered even across streams (or a recycled stream handle).
+// Calls under stream capture bypass the cache.  Cached bytes are capped; the
+// oldest buffers go first.
+struct CachedBuf {
+    void *p;
+    size_t bytes;
+    hipEvent_t done;
+    uint64_t stamp;
+};
+std::mutex g_sc_mu;
+std::map<hipStream_t, std::multimap<size_t, CachedBuf>> g_sc;
+size_t g_sc_bytes = 0;
+uint64_t g_sc_clock = 0;
+constexpr size_t SCRATCH_CACHE_CAP = size_t(32) << 30;
+
+bool capturing(hipStream_t st)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone;
+}
+
+// with g_sc_mu held
+void scratch_evict_oldest()
+{
+    auto best_s = g_sc.end();
+    std::multimap<size_t, CachedBuf>::iterator best;
+    for (auto si = g_sc.begin(); si != g_sc.end(); ++si)
+        for (auto it = si->second.begin(); it != si->second.end(); ++it)
+            if (best_s == g_sc.end() || it->second.stamp < best->second.stamp) {
+                best_s = si;
+                best = it;
+            }
+    if (best_s == g_sc.end()) return;
+    CachedBuf b = best->second;
+    best_s->second.erase(best);
+    g_sc_bytes -= b.bytes;
+    (void)hipEventSynchronize(b.done);
+    (void)hipEventDestroy(b.done);
+    (void)hipFree(b.p);
+}
+
 struct DevBuf {
     void *p = nullptr;
     hipStream_t s = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;  // non-null: the buffer goes back to the cache
     DevBuf() = default;
     DevBuf(size_t bytes, hipStream_t st) { alloc(bytes, st); }
     void alloc(size_t bytes, hipStream_t st)
     {
         s = st;
+        if (!bytes) return;
+        const bool cache = !capturing(st);
+        if (cache) {
+            std::lock_guard<std::mutex> lk(g_sc_mu);
+            auto sit = g_sc.find(st);
+            if (sit != g_sc.end()) {
+                auto it = sit->second.lower_bound(bytes);
+                if (it != sit->second.end() && it->first <= 2 * bytes) {
+                    const CachedBuf b = it->second;
+                    sit->second.erase(it);
+                    g_sc_bytes -= b.bytes;
+                    p = b.p;
+                    cap = b.bytes;
+                    done = b.done;
+                }
+            }
+        }
+        if (p) {
+            HIPCHK(hipStreamWaitEvent(st, done, 0));
+            return;
+        }
         keep_pool_memory();
-        if (bytes) HIPCHK(hipMallocAsync(&p, bytes, st));
+        cap = bytes;
+        HIPCHK(hipMallocAsync(&p, bytes, st));
+        if (cache && hipEventCreateWithFlags(&done, hipEventDisableTiming) != hipSuccess) done = nullptr;
     }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf()
     {
-        if (p) hipFreeAsync(p, s);
+        if (!p) return;
+        if (!done || hipEventRecord(done, s) != hipSuccess) {
+            if (done) (void)hipEventDestroy(done);
+            (void)hipFreeAsync(p, s);
+            return;
+        }
+        std::lock_guard<std::mutex> lk(g_sc_mu);
+        g_sc[s].emplace(cap, CachedBuf{p, cap, done, ++g_sc_clock});
+        g_sc_bytes += cap;
+        while (g_sc_bytes > SCRATCH_CACHE_CAP) scratch_evict_oldest();
     }
     template <class T>
     T *as() const
