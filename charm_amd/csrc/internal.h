@@ -131,6 +131,42 @@ int launch_shs_multi(const MultiShsArgs &a, void *stream);
 int launch_scale_cs_multi(int nmax, size_t nset, const void *cs_dev, const double *sn_dev, int nt,
                           void *out_dev, void *stream);
 
+// Derivatives of the synthesis (src/shs/shs.h.in:176-436).  Every derivative
+// is a sum of ordinary syntheses F[T] over transformed coefficient tables T,
+// each weighted per point by a monomial in t = sin(lat), 1/u = 1/cos(lat),
+// 1/r and q = R/r.
+//
+// A table maps (C, S) of degree n, order m to degree n - shift:
+//   (c0 + c1 n + c2 n^2) [e_nm if shift] (C_nm, S_nm)
+// followed by lon = 1: (C, S) -> m (S, -C), lon = 2: (C, S) -> -m^2 (C, S);
+// entries whose source degree exceeds nmax are zero.  e_nm is the factor of
+// P_{n-1,m} in cos(lat) dP_nm/dlat = -n t P_nm + e_nm P_{n-1,m}
+// (src/leg/leg_func_enm.c:40-41, src/leg/leg_func_xnum.h:35-37).
+struct GradTable {
+    double c0, c1, c2;
+    int shift, lon;
+};
+// coef t^pt u^-pu r^-pr q^pq F[table tab]
+struct GradTerm {
+    double coef;
+    int tab, pt, pu, pr, pq, pad;
+};
+constexpr int GRAD_MAXTERM = 24;
+constexpr int GRAD_MAXOUT = 6;
+struct GradCombine {
+    int nout;
+    int beg[GRAD_MAXOUT + 1];           // terms of output o: beg[o] .. beg[o+1]-1
+    GradTerm term[GRAD_MAXTERM];
+    double *out[GRAD_MAXOUT];           // [point] per output
+};
+int launch_grad_table(int nmax, const void *cs_dev, const GradTable &g, void *out_dev,
+                      void *stream);
+// Point p of nrow x ncol lies on row p / ncol; t, u, r per row.  With unit
+// set every radius is taken as R.  Table k's values start at tabs + k * tld.
+int launch_grad_combine(size_t nrow, size_t ncol, const double *t_dev, const double *u_dev,
+                        const double *r_dev, int unit, double R, const double *tabs_dev,
+                        size_t tld, const GradCombine &c, void *stream);
+
 // Kernel launchers (kernels.hip).  All return hipError_t as int.
 // alpha_nm and s_nm of degree nt; ab_tmp holds tri_len(nt) double2 of scratch
 int launch_leg_tables(int nt, const double *r_dev, const double *ri_dev, void *ab_tmp,

@@ -1579,6 +1579,82 @@ int launch_cs_truncate(const double *src, unsigned long ns, double *dst, unsigne
     return (int)hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Derivatives: transformed coefficient tables and their per-point combination
+// ---------------------------------------------------------------------------
+__global__ void grad_table_kernel(int nmax, const double2 *__restrict__ cs, GradTable g,
+                                  double2 *__restrict__ out)
+{
+    const int m = blockIdx.y;
+    const double2 *src = cs + dcol_off(nmax, m) - m;
+    double2 *dst = out + dcol_off(nmax, m) - m;
+    const double dm = (double)m;
+    for (int nd = m + blockIdx.x * blockDim.x + threadIdx.x; nd <= nmax;
+         nd += gridDim.x * blockDim.x) {
+        const int n = nd + g.shift;
+        double2 v = make_double2(0.0, 0.0);
+        if (n <= nmax) {
+            const double2 x = src[n];
+            const double dn = (double)n;
+            double f = g.c0 + dn * (g.c1 + dn * g.c2);
+            // e_nm = sqrt(2n+1) sqrt(n-m) sqrt(n+m) / sqrt(2n-1), n > m here
+            if (g.shift)
+                f *= sqrt(2.0 * dn + 1.0) * sqrt(dn - dm) * sqrt(dn + dm) *
+                     (1.0 / sqrt(2.0 * dn - 1.0));
+            if (g.lon == 1)
+                v = make_double2(f * (dm * x.y), f * (-dm * x.x));
+            else if (g.lon == 2)
+                v = make_double2(f * (-(dm * dm) * x.x), f * (-(dm * dm) * x.y));
+            else
+                v = make_double2(f * x.x, f * x.y);
+        }
+        dst[nd] = v;
+    }
+}
+
+__global__ void grad_combine_kernel(size_t nrow, size_t ncol, const double *__restrict__ t,
+                                    const double *__restrict__ u, const double *__restrict__ r,
+                                    int unit, double R, const double *__restrict__ tabs, size_t tld,
+                                    GradCombine c)
+{
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nrow * ncol) return;
+    const size_t row = (ncol == 1) ? p : p / ncol;
+    const double tt = t[row], ui = 1.0 / u[row];
+    const double rr = unit ? R : r[row];
+    const double ri = 1.0 / rr, q = unit ? 1.0 : R / rr;
+    const double tp[3] = {1.0, tt, tt * tt}, up[3] = {1.0, ui, ui * ui};
+    const double rp[3] = {1.0, ri, ri * ri}, qp[2] = {1.0, q};
+    for (int o = 0; o < c.nout; o++) {
+        double acc = 0.0;
+        for (int k = c.beg[o]; k < c.beg[o + 1]; k++) {
+            const GradTerm &g = c.term[k];
+            const double w = g.coef * ((tp[g.pt] * up[g.pu]) * (rp[g.pr] * qp[g.pq]));
+            acc += w * tabs[(size_t)g.tab * tld + p];
+        }
+        c.out[o][p] = acc;
+    }
+}
+
+int launch_grad_table(int nmax, const void *cs, const GradTable &g, void *out, void *stream)
+{
+    const unsigned gx = (unsigned)((nmax + 1 + 255) / 256);
+    hipLaunchKernelGGL(grad_table_kernel, dim3(gx, (unsigned)nmax + 1), dim3(256), 0, S(stream),
+                       nmax, (const double2 *)cs, g, (double2 *)out);
+    return (int)hipGetLastError();
+}
+
+int launch_grad_combine(size_t nrow, size_t ncol, const double *t, const double *u,
+                        const double *r, int unit, double R, const double *tabs, size_t tld,
+                        const GradCombine &c, void *stream)
+{
+    const size_t n = nrow * ncol;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(grad_combine_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       S(stream), nrow, ncol, t, u, r, unit, R, tabs, tld, c);
+    return (int)hipGetLastError();
+}
+
 __global__ void zero_ints_kernel(int *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -79,7 +79,7 @@ void init_pool()
     hipMemPool_t pool;
     if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
         uint64_t thr = UINT64_MAX;
-        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
     }
 }
 
@@ -91,8 +91,8 @@ struct ThreadState {
     ~ThreadState()
     {
         for (auto &kv : plans) hipfftDestroy(kv.second);
-        if (pinned) hipHostFree(pinned);
-        if (stream) hipStreamDestroy(stream);
+        if (pinned) (void)hipHostFree(pinned);
+        if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -660,68 +660,96 @@ void run_shs_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev, bo
 // ---------------------------------------------------------------------------
 // Plain per-call synthesis on a grid (host arrays)
 // ---------------------------------------------------------------------------
-void shs_grid_host(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax, double *f,
-                   charm_err *err)
+// Custom grids: longitudes must be linearly increasing
+// (src/shs/shs_point_grd.c:205-236)
+bool check_grid_lons(const charm_point *pnt, charm_err *err)
+{
+    const size_t nlon = pnt->nlon;
+    if (pnt->type != CHARM_CRD_POINT_GRID || nlon < 2) return true;
+    const double d0 = pnt->lon[1] - pnt->lon[0];
+    for (size_t j = 1; j < nlon; j++)
+        if (!nearly_equal(pnt->lon[j] - pnt->lon[j - 1], d0, charm_glob_threshold2)) {
+            CHARM_AMD_ERR(err, CHARM_EFUNCARG,
+                          "\"pnt->lon\" is not a linearly increasing array within the "
+                          "\"threshold2\".");
+            return false;
+        }
+    return true;
+}
+
+// Device copies of a grid's rings, sectorial seeds, radius powers and, for
+// the direct longitude sums, longitudes, for syntheses up to degree nmax.
+struct GridSetup {
+    GridGeom g;
+    bool use_fft = false, unit = true;
+    DevBuf dt, du, dmir, sx, six, ratio, rp, lon;
+    DevGrid d{};
+};
+
+void grid_setup(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax,
+                const Tables &tb, hipStream_t s, GridSetup &G)
 {
     const size_t nlat = pnt->nlat, nlon = pnt->nlon;
-    // custom grids: longitudes must be linearly increasing
-    // (src/shs/shs_point_grd.c:205-236)
-    if (pnt->type == CHARM_CRD_POINT_GRID && nlon > 1) {
-        const double d0 = pnt->lon[1] - pnt->lon[0];
-        for (size_t j = 1; j < nlon; j++)
-            if (!nearly_equal(pnt->lon[j] - pnt->lon[j - 1], d0, charm_glob_threshold2)) {
-                CHARM_AMD_ERR(err, CHARM_EFUNCARG,
-                              "\"pnt->lon\" is not a linearly increasing array within the "
-                              "\"threshold2\".");
-                return;
-            }
+    G.g = grid_geometry(pnt);
+    G.use_fft = fft_applicable(pnt, nmax);
+    G.unit = true;
+    for (size_t i = 0; i < nlat && G.unit; i++)
+        G.unit = nearly_equal(pnt->r[i], shcs->r, charm_glob_threshold);
+    const size_t nr = G.g.nring;
+    G.dt.alloc(nr * sizeof(double), s);
+    G.du.alloc(nr * sizeof(double), s);
+    G.dmir.alloc(nr, s);
+    h2d(G.dt.as<double>(), G.g.t.data(), nr, s);
+    h2d(G.du.as<double>(), G.g.u.data(), nr, s);
+    h2d(G.dmir.as<unsigned char>(), G.g.mirror.data(), nr, s);
+    const size_t sld = nr;
+    G.sx.alloc(std::max<size_t>(nmax, 1) * sld * sizeof(double), s);
+    G.six.alloc(std::max<size_t>(nmax, 1) * sld * sizeof(int), s);
+    HIPCHK((hipError_t)launch_seeds((int)nr, G.du.as<double>(), tb.dm, (int)nmax, sld,
+                                    G.sx.as<double>(), G.six.as<int>(), s));
+    G.d = DevGrid{G.dt.as<double>(), G.du.as<double>(), G.dmir.as<unsigned char>(),
+                  G.sx.as<double>(), G.six.as<int>(), sld, nullptr, nullptr, 0};
+    if (!G.unit) {
+        std::vector<double> q(nlat);
+        for (size_t i = 0; i < nlat; i++) q[i] = shcs->r / pnt->r[i];
+        G.ratio.alloc(nlat * sizeof(double), s);
+        G.rp.alloc((nmax + 2) * nlat * sizeof(double), s);
+        h2d(G.ratio.as<double>(), q.data(), nlat, s);
+        HIPCHK((hipError_t)launch_rpows((int)nlat, G.ratio.as<double>(), (int)nmax + 1, nlat,
+                                        G.rp.as<double>(), s));
+        G.d.ratio = G.ratio.as<double>();
+        G.d.rp = G.rp.as<double>();
+        G.d.rld = nlat;
     }
-    hipStream_t s = thread_stream();
-    GridGeom g = grid_geometry(pnt);
-    const bool use_fft = fft_applicable(pnt, nmax);
-    bool unit = true;
-    for (size_t i = 0; i < nlat && unit; i++)
-        unit = nearly_equal(pnt->r[i], shcs->r, charm_glob_threshold);
-    Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
+    if (!G.use_fft) {
+        G.lon.alloc(nlon * sizeof(double), s);
+        h2d(G.lon.as<double>(), pnt->lon, nlon, s);
+    }
+}
 
+// Coefficients of degree nmax as (C, S) pairs on the device
+void upload_coeffs(const charm_shc *shcs, unsigned long nmax, hipStream_t s, DevBuf &cs)
+{
     const size_t P = tri_len(nmax);
     double *hcs = (double *)pinned_buffer(P * 2 * sizeof(double));
     pack_coeffs(shcs, nmax, hcs);
-    DevBuf cs(P * 2 * sizeof(double), s);
+    cs.alloc(P * 2 * sizeof(double), s);
     h2d(cs.as<double>(), hcs, 2 * P, s);
+}
 
-    const size_t nr = g.nring;
-    DevBuf dt(nr * sizeof(double), s), du(nr * sizeof(double), s), dmir(nr, s);
-    h2d(dt.as<double>(), g.t.data(), nr, s);
-    h2d(du.as<double>(), g.u.data(), nr, s);
-    h2d(dmir.as<unsigned char>(), g.mirror.data(), nr, s);
-    const size_t sld = nr;
-    DevBuf sx(std::max<size_t>(nmax, 1) * sld * sizeof(double), s);
-    DevBuf six(std::max<size_t>(nmax, 1) * sld * sizeof(int), s);
-    HIPCHK((hipError_t)launch_seeds((int)nr, du.as<double>(), tb.dm, (int)nmax, sld,
-                                    sx.as<double>(), six.as<int>(), s));
-    DevBuf ratio, rp;
-    DevGrid d{dt.as<double>(), du.as<double>(), dmir.as<unsigned char>(), sx.as<double>(),
-              six.as<int>(), sld, nullptr, nullptr, 0};
-    if (!unit) {
-        std::vector<double> q(nlat);
-        for (size_t i = 0; i < nlat; i++) q[i] = shcs->r / pnt->r[i];
-        ratio.alloc(nlat * sizeof(double), s);
-        rp.alloc((nmax + 2) * nlat * sizeof(double), s);
-        h2d(ratio.as<double>(), q.data(), nlat, s);
-        HIPCHK((hipError_t)launch_rpows((int)nlat, ratio.as<double>(), (int)nmax + 1, nlat,
-                                        rp.as<double>(), s));
-        d.ratio = ratio.as<double>();
-        d.rp = rp.as<double>();
-        d.rld = nlat;
-    }
-    DevBuf lon;
-    if (!use_fft) {
-        lon.alloc(nlon * sizeof(double), s);
-        h2d(lon.as<double>(), pnt->lon, nlon, s);
-    }
+void shs_grid_host(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax, double *f,
+                   charm_err *err)
+{
+    if (!check_grid_lons(pnt, err)) return;
+    const size_t nlat = pnt->nlat, nlon = pnt->nlon;
+    hipStream_t s = thread_stream();
+    Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
+    DevBuf cs;
+    upload_coeffs(shcs, nmax, s, cs);
+    GridSetup G;
+    grid_setup(pnt, shcs, nmax, tb, s, G);
     DevBuf df(nlat * nlon * sizeof(double), s);
-    run_shs_grid(g, d, lon.as<double>(), use_fft, nmax, shcs->mu / shcs->r, cs.p, tb,
+    run_shs_grid(G.g, G.d, G.lon.as<double>(), G.use_fft, nmax, shcs->mu / shcs->r, cs.p, tb,
                  df.as<double>(), s);
     HIPCHK(hipMemcpyAsync(f, df.p, nlat * nlon * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -848,6 +876,220 @@ void shs_sctr_host(const charm_point *pnt, const charm_shc *shcs, unsigned long 
     run_sctr(npoint, dlat.as<double>(), dl.as<double>(), dr.as<double>(), unit, nmax, shcs->mu,
              shcs->r, cs.p, tb, df.as<double>(), s);
     HIPCHK(hipMemcpyAsync(f, df.p, npoint * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// Gradients and single derivatives (src/shs/shs.h.in:176-436)
+//
+// With g_ijk = 1/(r^(j+k) cos^k lat) d^(i+j+k) f / dr^i dlat^j dlon^k, the
+// radial derivatives multiply degree n by rho_1 = -(n+1), rho_2 = (n+1)(n+2)
+// and add 1/r^i; the longitude derivatives act on (C, S) per order; the
+// latitude derivatives follow from
+//   u dP_nm/dlat       = -n t P_nm + e_nm P_{n-1,m}
+//   d^2 P_nm / dlat^2  = (t/u) dP_nm/dlat + (m^2/u^2 - n(n+1)) P_nm
+// (src/leg/leg_func_xnum.h:35-44), where the P_{n-1,m} term is a synthesis
+// over the table e_{n+1,m} X_{n+1,m} carrying one more power of q = R/r.
+// Terms of one output with the same weight share one table.
+// ---------------------------------------------------------------------------
+struct GradPlan {
+    std::vector<GradTable> tabs;
+    std::vector<std::vector<GradTerm>> outs;
+    explicit GradPlan(int nout) : outs(nout) {}
+    void term(int o, double coef, GradTable t, int pt, int pu, int pr, int pq)
+    {
+        int k = 0;
+        for (; k < (int)tabs.size(); k++) {
+            const GradTable &x = tabs[k];
+            if (x.c0 == t.c0 && x.c1 == t.c1 && x.c2 == t.c2 && x.shift == t.shift &&
+                x.lon == t.lon)
+                break;
+        }
+        if (k == (int)tabs.size()) tabs.push_back(t);
+        outs[o].push_back(GradTerm{coef, k, pt, pu, pr, pq, 0});
+    }
+};
+
+//                                c0  c1  c2 shift lon
+constexpr GradTable TAB_ONE    = {1, 0, 0, 0, 0};
+constexpr GradTable TAB_RAD1   = {-1, -1, 0, 0, 0};   // -(n+1)
+constexpr GradTable TAB_RAD2   = {2, 3, 1, 0, 0};     // (n+1)(n+2)
+constexpr GradTable TAB_LON1   = {1, 0, 0, 0, 1};
+constexpr GradTable TAB_LON2   = {1, 0, 0, 0, 2};
+constexpr GradTable TAB_N      = {0, 1, 0, 0, 0};     // n
+constexpr GradTable TAB_NN1    = {0, 1, 1, 0, 0};     // n(n+1)
+constexpr GradTable TAB_SH     = {1, 0, 0, 1, 0};
+constexpr GradTable TAB_NLON1  = {0, 1, 0, 0, 1};
+constexpr GradTable TAB_SHLON1 = {1, 0, 0, 1, 1};
+constexpr GradTable TAB_NRAD1  = {0, -1, -1, 0, 0};   // -n(n+1)
+constexpr GradTable TAB_SHRAD1 = {-1, -1, 0, 1, 0};
+constexpr GradTable TAB_RAD1LON1 = {-1, -1, 0, 0, 1};
+
+// g_{dr dlat dlon} as one output
+GradPlan guru_plan(unsigned dr, unsigned dlat, unsigned dlon)
+{
+    GradPlan p(1);
+    const int key = 100 * (int)dr + 10 * (int)dlat + (int)dlon;
+    switch (key) {
+    case 0: p.term(0, 1, TAB_ONE, 0, 0, 0, 0); break;
+    case 100: p.term(0, 1, TAB_RAD1, 0, 0, 1, 0); break;
+    case 200: p.term(0, 1, TAB_RAD2, 0, 0, 2, 0); break;
+    case 1: p.term(0, 1, TAB_LON1, 0, 1, 1, 0); break;
+    case 2: p.term(0, 1, TAB_LON2, 0, 2, 2, 0); break;
+    case 10:
+        p.term(0, -1, TAB_N, 1, 1, 1, 0);
+        p.term(0, 1, TAB_SH, 0, 1, 1, 1);
+        break;
+    case 20:
+        p.term(0, -1, TAB_N, 2, 2, 2, 0);
+        p.term(0, 1, TAB_SH, 1, 2, 2, 1);
+        p.term(0, -1, TAB_NN1, 0, 0, 2, 0);
+        p.term(0, -1, TAB_LON2, 0, 2, 2, 0);
+        break;
+    case 110:
+        p.term(0, -1, TAB_NRAD1, 1, 1, 2, 0);
+        p.term(0, 1, TAB_SHRAD1, 0, 1, 2, 1);
+        break;
+    case 101: p.term(0, 1, TAB_RAD1LON1, 0, 1, 2, 0); break;
+    case 11:
+        p.term(0, -1, TAB_NLON1, 1, 2, 2, 0);
+        p.term(0, 1, TAB_SHLON1, 0, 2, 2, 1);
+        break;
+    default: break;
+    }
+    return p;
+}
+
+// f_x = g_010, f_y = -g_001, f_z = g_100 (src/shs/shs.h.in:176-197)
+GradPlan grad1_plan()
+{
+    GradPlan p(3);
+    p.term(0, -1, TAB_N, 1, 1, 1, 0);
+    p.term(0, 1, TAB_SH, 0, 1, 1, 1);
+    p.term(1, -1, TAB_LON1, 0, 1, 1, 0);
+    p.term(2, 1, TAB_RAD1, 0, 0, 1, 0);
+    return p;
+}
+
+// Reed (1973) after src/shs/shs.h.in:278-312:
+//   f_xx = g_020 + g_100/r          f_xy = -g_011 - (t/u) g_001/r
+//   f_xz = g_110 - g_010/r          f_yy = g_002 + g_100/r - (t/u) g_010/r
+//   f_yz = -g_101 + g_001/r         f_zz = g_200
+GradPlan grad2_plan()
+{
+    GradPlan p(6);
+    // xx: the r^-2 terms -n(n+1) - (n+1) = -(n+1)^2 share a table
+    p.term(0, -1, TAB_N, 2, 2, 2, 0);
+    p.term(0, 1, TAB_SH, 1, 2, 2, 1);
+    p.term(0, 1, GradTable{-1, -2, -1, 0, 0}, 0, 0, 2, 0);
+    p.term(0, -1, TAB_LON2, 0, 2, 2, 0);
+    // xy: n - 1 times the longitude table
+    p.term(1, 1, GradTable{-1, 1, 0, 0, 1}, 1, 2, 2, 0);
+    p.term(1, -1, TAB_SHLON1, 0, 2, 2, 1);
+    // xz: n(n+1) + n and -(n+1) - 1 on the shifted table
+    p.term(2, 1, GradTable{0, 2, 1, 0, 0}, 1, 1, 2, 0);
+    p.term(2, 1, GradTable{-2, -1, 0, 1, 0}, 0, 1, 2, 1);
+    // yy
+    p.term(3, 1, TAB_LON2, 0, 2, 2, 0);
+    p.term(3, 1, TAB_RAD1, 0, 0, 2, 0);
+    p.term(3, 1, TAB_N, 2, 2, 2, 0);
+    p.term(3, -1, TAB_SH, 1, 2, 2, 1);
+    // yz: 1 + (n+1) times the longitude table
+    p.term(4, 1, GradTable{2, 1, 0, 0, 1}, 0, 1, 2, 0);
+    // zz
+    p.term(5, 1, TAB_RAD2, 0, 0, 2, 0);
+    return p;
+}
+
+// Per-row sin and cos of a grid's latitudes, as the synthesis uses them:
+// mirrored rows take -t of their ring
+void grid_rows_tu(const GridGeom &g, std::vector<double> &t, std::vector<double> &u)
+{
+    t.assign(g.nlat, 0.0);
+    u.assign(g.nlat, 1.0);
+    for (size_t i = 0; i < g.nring; i++) {
+        if (i < g.nlat) {
+            t[i] = g.t[i];
+            u[i] = g.u[i];
+        }
+        if (g.mirror[i]) {
+            const size_t j = g.mir_nlat - 1 - i;
+            if (j < g.nlat) {
+                t[j] = -g.t[i];
+                u[j] = g.u[i];
+            }
+        }
+    }
+}
+
+void shs_grad_host(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax,
+                   const GradPlan &plan, double **f, charm_err *err)
+{
+    const bool sctr = pnt->type == CHARM_CRD_POINT_SCATTERED;
+    if (sctr && pnt->nlat != pnt->nlon) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG,
+                      "The number of latitudes and longitudes in the \"pnt\" structure must be "
+                      "the same to perform point-wise spherical harmonic synthesis.");
+        return;
+    }
+    if (!sctr && !check_grid_lons(pnt, err)) return;
+    const int nout = (int)plan.outs.size();
+    size_t nterm = 0;
+    for (const auto &o : plan.outs) nterm += o.size();
+    if (nout > GRAD_MAXOUT || nterm > (size_t)GRAD_MAXTERM)
+        throw HipFail{hipErrorInvalidValue, "derivative plan too large", __FILE__, __LINE__};
+    hipStream_t s = thread_stream();
+    Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
+    DevBuf cs;
+    upload_coeffs(shcs, nmax, s, cs);
+    const size_t P = tri_len(nmax), ntab = plan.tabs.size();
+    const size_t nrow = pnt->nlat, ncol = sctr ? 1 : pnt->nlon, npoint = nrow * ncol;
+    DevBuf ctab(P * 2 * sizeof(double), s);
+    DevBuf tabs(ntab * npoint * sizeof(double), s), outs(nout * npoint * sizeof(double), s);
+    DevBuf dt(nrow * sizeof(double), s), du(nrow * sizeof(double), s);
+    DevBuf dr(nrow * sizeof(double), s);
+    h2d(dr.as<double>(), pnt->r, nrow, s);
+    bool unit = true;
+    for (size_t i = 0; i < nrow && unit; i++)
+        unit = nearly_equal(pnt->r[i], shcs->r, charm_glob_threshold);
+    if (sctr) {
+        DevBuf dlat(npoint * sizeof(double), s), dlon(npoint * sizeof(double), s);
+        h2d(dlat.as<double>(), pnt->lat, npoint, s);
+        h2d(dlon.as<double>(), pnt->lon, npoint, s);
+        HIPCHK((hipError_t)launch_sincos(npoint, dlat.as<double>(), dt.as<double>(),
+                                         du.as<double>(), s));
+        for (size_t k = 0; k < ntab; k++) {
+            HIPCHK((hipError_t)launch_grad_table((int)nmax, cs.p, plan.tabs[k], ctab.p, s));
+            run_sctr(npoint, dlat.as<double>(), dlon.as<double>(), dr.as<double>(), unit, nmax,
+                     shcs->mu, shcs->r, ctab.p, tb, tabs.as<double>() + k * npoint, s);
+        }
+    } else {
+        GridSetup G;
+        grid_setup(pnt, shcs, nmax, tb, s, G);
+        std::vector<double> ht, hu;
+        grid_rows_tu(G.g, ht, hu);
+        h2d(dt.as<double>(), ht.data(), nrow, s);
+        h2d(du.as<double>(), hu.data(), nrow, s);
+        for (size_t k = 0; k < ntab; k++) {
+            HIPCHK((hipError_t)launch_grad_table((int)nmax, cs.p, plan.tabs[k], ctab.p, s));
+            run_shs_grid(G.g, G.d, G.lon.as<double>(), G.use_fft, nmax, shcs->mu / shcs->r,
+                         ctab.p, tb, tabs.as<double>() + k * npoint, s);
+        }
+    }
+    GradCombine c{};
+    c.nout = nout;
+    int q = 0;
+    for (int o = 0; o < nout; o++) {
+        c.beg[o] = q;
+        for (const GradTerm &x : plan.outs[o]) c.term[q++] = x;
+        c.out[o] = outs.as<double>() + (size_t)o * npoint;
+    }
+    c.beg[nout] = q;
+    HIPCHK((hipError_t)launch_grad_combine(nrow, ncol, dt.as<double>(), du.as<double>(),
+                                           dr.as<double>(), unit ? 1 : 0, shcs->r,
+                                           tabs.as<double>(), npoint, c, s));
+    for (int o = 0; o < nout; o++)
+        HIPCHK(hipMemcpyAsync(f[o], c.out[o], npoint * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
 }
 
@@ -1114,6 +1356,82 @@ void charm_shs_point(const charm_point *pnt, const charm_shc *shcs, unsigned lon
     if (!charm_err_isempty(err)) err_propagate(err, __FILE__, __LINE__, __func__);
 }
 
+// Shared front end of the derivative syntheses: the argument checks of
+// src/shs/shs_point_gradn.c:49-83 and src/shs/shs_point_guru.c:30-66
+static void shs_deriv_entry(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax,
+                            const GradPlan &plan, double **f, charm_err *err)
+{
+    if (nmax > shcs->nmax) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG,
+                      "Maximum harmonic degree of the synthesis (\"nmax\") cannot be larger than "
+                      "maximum harmonic degree of spherical harmonic coefficients "
+                      "(\"shcs->nmax\").");
+        return;
+    }
+    const int type = pnt->type;
+    const bool sctr = type == CHARM_CRD_POINT_SCATTERED;
+    const bool grid = type == CHARM_CRD_POINT_GRID || type == CHARM_CRD_POINT_GRID_GL ||
+                      type == CHARM_CRD_POINT_GRID_DH1 || type == CHARM_CRD_POINT_GRID_DH2;
+    if (!sctr && !grid) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG,
+                      "Unsupported \"pnt->type\" for spherical harmonic synthesis of point "
+                      "values.");
+        return;
+    }
+    if (pnt->npoint == 0) return;
+    try {
+        shs_grad_host(pnt, shcs, nmax, plan, f, err);
+    } catch (const HipFail &e) {
+        report(err, e, "shs_deriv_entry");
+    }
+}
+
+// src/shs/shs_point_gradn.c:31-173 (GRADN = GRAD_1); f[0..2] = x, y, z
+void charm_shs_point_grad1(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax,
+                           double **f, charm_err *err)
+{
+    shs_deriv_entry(pnt, shcs, nmax, grad1_plan(), f, err);
+    if (!charm_err_isempty(err)) err_propagate(err, __FILE__, __LINE__, __func__);
+}
+
+// src/shs/shs_point_gradn.c:31-173 (GRADN = GRAD_2); f[0..5] = xx, xy, xz, yy, yz, zz
+void charm_shs_point_grad2(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax,
+                           double **f, charm_err *err)
+{
+    shs_deriv_entry(pnt, shcs, nmax, grad2_plan(), f, err);
+    if (!charm_err_isempty(err)) err_propagate(err, __FILE__, __LINE__, __func__);
+}
+
+// src/shs/shs_point_guru.c:21-119, derivative checks of
+// src/shs/shs_check_single_derivative.c:17-64
+void charm_shs_point_guru(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax,
+                          unsigned dr, unsigned dlat, unsigned dlon, double *f, charm_err *err)
+{
+    if (nmax > shcs->nmax) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG,
+                      "Maximum harmonic degree of the synthesis (\"nmax\") cannot be larger than "
+                      "maximum harmonic degree of spherical harmonic coefficients "
+                      "(\"shcs->nmax\").");
+    } else if ((int)dr < 0 || (int)dlat < 0 || (int)dlon < 0) {
+        char msg[CHARM_ERR_MAX_MSG];
+        const char *which = (int)dr < 0 ? "dr" : ((int)dlat < 0 ? "dlat" : "dlon");
+        const int v = (int)dr < 0 ? (int)dr : ((int)dlat < 0 ? (int)dlat : (int)dlon);
+        std::snprintf(msg, sizeof msg, "\"%s\" is \"%d\", but it must be non-negative.", which, v);
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG, msg);
+    } else if ((int)dr + (int)dlat + (int)dlon > 2) {
+        char msg[CHARM_ERR_MAX_MSG];
+        std::snprintf(msg, sizeof msg,
+                      "The sum \"dr + dlat + dlon\" is \"%d\", but it cannot be larger than "
+                      "\"%d\".",
+                      (int)dr + (int)dlat + (int)dlon, 2);
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG, msg);
+    } else {
+        double *fp[1] = {f};
+        shs_deriv_entry(pnt, shcs, nmax, guru_plan(dr, dlat, dlon), fp, err);
+    }
+    if (!charm_err_isempty(err)) err_propagate(err, __FILE__, __LINE__, __func__);
+}
+
 // src/sha/sha_point.c:124-1125
 void charm_sha_point(const charm_point *pnt, const double *f, unsigned long nmax,
                      charm_shc *shcs, charm_err *err)
@@ -1183,14 +1501,14 @@ charm_amd_grid *charm_amd_grid_create(const charm_point *pnt, charm_err *err)
 void charm_amd_grid_free(charm_amd_grid *grd)
 {
     if (!grd) return;
-    hipDeviceSynchronize();
-    hipFree(grd->t);
-    hipFree(grd->u);
-    hipFree(grd->mirror);
-    hipFree(grd->w);
-    hipFree(grd->lon);
-    hipFree(grd->sx);
-    hipFree(grd->six);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(grd->t);
+    (void)hipFree(grd->u);
+    (void)hipFree(grd->mirror);
+    (void)hipFree(grd->w);
+    (void)hipFree(grd->lon);
+    (void)hipFree(grd->sx);
+    (void)hipFree(grd->six);
     delete grd;
 }
 

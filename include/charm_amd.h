@@ -164,6 +164,19 @@ extern double charm_glob_polar_optimization_a2;
  * values (row-major [nlat][nlon] for grids). */
 void charm_shs_point(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax,
                      double *f, charm_err *err);
+/* src/shs/shs.h.in:239-243.  First-order gradient in the local north-oriented
+ * frame: f[0], f[1], f[2] each receive pnt->npoint values of
+ * fx = (1/r) df/dlat, fy = -1/(r cos lat) df/dlon, fz = df/dr. */
+void charm_shs_point_grad1(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax,
+                           double **f, charm_err *err);
+/* src/shs/shs.h.in:344-348.  Second-order gradient (Reed 1973): f[0..5]
+ * receive fxx, fxy, fxz, fyy, fyz, fzz. */
+void charm_shs_point_grad2(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax,
+                           double **f, charm_err *err);
+/* src/shs/shs.h.in:428-435.  1/(r^(dlat+dlon) cos^dlon lat) times the
+ * derivative of order dr, dlat, dlon in r, lat, lon; dr + dlat + dlon <= 2. */
+void charm_shs_point_guru(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax,
+                          unsigned dr, unsigned dlat, unsigned dlon, double *f, charm_err *err);
 /* src/sha/sha.h.in:164-169.  Point analysis of f sampled on the quadrature
  * grid pnt up to degree nmax; overwrites all of shcs->c and shcs->s. */
 void charm_sha_point(const charm_point *pnt, const double *f, unsigned long nmax,

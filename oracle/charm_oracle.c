@@ -895,6 +895,201 @@ static int orc_sha_quad(int type, unsigned long nmax, unsigned long nmax_shc, si
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Derivatives of the synthesis                                             */
+/* ------------------------------------------------------------------------ */
+/* P_nm(t), n = m..nmax, of one order into p[n - m], by the recurrences of
+ * lumped_ring (the same X-number switch per lane). */
+static void leg_column(unsigned long nmax, unsigned long m, const real *anm,
+                       const real *bnm, real t, const real *ps, const int64_t *ips,
+                       real *p)
+{
+    if (m == 0) {
+        real p0 = 1, p1, p2;
+        p[0] = p0;
+        if (nmax >= 1) {
+            p1 = RSQRT((real)3) * t;
+            p[1] = p1;
+            for (unsigned long n = 2; n <= nmax; n++) {
+                p2 = (anm[n] * t) * p1 - bnm[n] * p0;
+                p[n] = p2;
+                p0 = p1;
+                p1 = p2;
+            }
+        }
+        return;
+    }
+    real x = ps[m - 1], y = 0, z;
+    int64_t ix = ips[m - 1], iy = 0, iz;
+    p[0] = x2f(x, ix);
+    if (m == nmax) return;
+    y = x; iy = ix;
+    x = (anm[m + 1] * t) * y;
+    ix = iy;
+    XNORM(x, ix);
+    p[1] = x2f(x, ix);
+    int ds = 0;
+    for (unsigned long n = m + 2; n <= nmax; n++) {
+        real v;
+        if (ds) {
+            v = (anm[n] * t) * x - bnm[n] * y;
+            y = x;
+            x = v;
+        } else {
+            int64_t ixy = ix - iy;
+            if (ixy == 0) { z = (anm[n] * t) * x - bnm[n] * y; iz = ix; }
+            else if (ixy == 1) { z = (anm[n] * t) * x - bnm[n] * (y * BIGI); iz = ix; }
+            else if (ixy == -1) { z = (anm[n] * t) * (x * BIGI) - bnm[n] * y; iz = iy; }
+            else if (ixy > 1) { z = (anm[n] * t) * x; iz = ix; }
+            else { z = -bnm[n] * y; iz = iy; }
+            XNORM(z, iz);
+            y = x; iy = ix;
+            x = z; ix = iz;
+            v = x2f(z, iz);
+            if (ix == 0 && iy == 0) ds = 1;
+        }
+        p[n - m] = v;
+    }
+}
+
+/* The ten g_ijk = 1/(r^(j+k) cos^k lat) d^(i+j+k) f / dr^i dlat^j dlon^k,
+ * i + j + k <= 2, at one point, in the order
+ *   000, 100, 010, 001, 200, 020, 002, 110, 101, 011.
+ * Per term, as CHarm's kernels evaluate them (shs_point_kernel.c:176-487,
+ * leg_func_xnum.h:35-44, leg_func_enm.c:40-41, shs_get_mur_dorder_npar.c):
+ *   dP_nm  = e_nm / u P_{n-1,m} - n t/u P_nm
+ *   ddP_nm = t/u dP_nm + (m^2/u^2 - n(n+1)) P_nm
+ * radial factors -(n+1) and (n+1)(n+2) times (R/r)^(n+1), longitude
+ * derivatives of a cos + b sin, and 1/r^(i+j+k) with r = R when every radius
+ * equals R. */
+static void guru_point(unsigned long nmax, unsigned long nmax_shc, const real *C,
+                       const real *S, real mu, real R, real lat, real lon, real r,
+                       int unit, const real *rt, const real *rit, const real *dm,
+                       real *anm, real *bnm, real *enm, real *ps, int64_t *ips, real *rp,
+                       real *p, real *g)
+{
+    const real t = RSIN(lat), u = RCOS(lat);
+    const real tu = t / u, ui = ((real)1) / u, ui2 = ui * ui;
+    leg_seed(u, dm, nmax, ps, ips);
+    rpows(R, r, unit, nmax + 1, rp);
+    real F[3][3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) F[i][j][k] = 0;
+    for (unsigned long m = 0; m <= nmax; m++) {
+        if (polar_skip(m, nmax, u)) continue;
+        leg_anm_bnm(nmax, m, rt, rit, anm, bnm);
+        enm[m] = 0;
+        for (unsigned long n = m + 1; n <= nmax; n++)
+            enm[n] = rt[2 * n + 1] * rt[n - m] * rt[n + m] * rit[2 * n - 1];
+        leg_column(nmax, m, anm, bnm, t, ps, ips, p);
+        const size_t off = col_off(nmax_shc, m);
+        real A[3][3], B[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) A[i][j] = B[i][j] = 0;
+        const real m2 = (real)m * (real)m;
+        for (unsigned long n = m; n <= nmax; n++) {
+            const real pn = p[n - m], pp = (n > m) ? p[n - m - 1] : 0;
+            const real dn = (real)n;
+            const real dp = (enm[n] * ui) * pp - (dn * tu) * pn;
+            const real ddp = tu * dp + (m2 * ui2 - dn * (dn + 1)) * pn;
+            const real L[3] = {pn, dp, ddp};
+            const real W[3] = {rp[n + 1], (-(dn + 1)) * rp[n + 1],
+                               ((dn + 1) * (dn + 2)) * rp[n + 1]};
+            const real c = C[off + (n - m)], sv = (m == 0) ? 0 : S[off + (n - m)];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; i + j < 3; j++) {
+                    const real wl = W[i] * L[j];
+                    A[i][j] = A[i][j] + wl * c;
+                    B[i][j] = B[i][j] + wl * sv;
+                }
+        }
+        const real ml = (real)m * lon, cm = RCOS(ml), sm = RSIN(ml);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; i + j < 3; j++) {
+                F[i][j][0] = F[i][j][0] + (A[i][j] * cm + B[i][j] * sm);
+                if (i + j < 2)
+                    F[i][j][1] = F[i][j][1] + (real)m * (B[i][j] * cm - A[i][j] * sm);
+                if (i + j == 0)
+                    F[i][j][2] = F[i][j][2] - m2 * (A[i][j] * cm + B[i][j] * sm);
+            }
+    }
+    const real rr = unit ? R : r, ri = ((real)1) / rr, mur = mu / R;
+    static const int IJK[10][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0},
+                                   {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+    for (int q = 0; q < 10; q++) {
+        const int i = IJK[q][0], j = IJK[q][1], k = IJK[q][2];
+        real v = mur * F[i][j][k];
+        for (int e = 0; e < i + j + k; e++) v = v * ri;
+        for (int e = 0; e < k; e++) v = v * ui;
+        g[q] = v;
+    }
+}
+
+/* Derivatives at scattered points (src/shs/shs_point_sctr.c with
+ * dr, dlat, dlon as in shs_point_guru.c and shs_point_gradn.c).
+ *   kind 0..9 : one g_ijk in the order of guru_point, 1 output;
+ *   kind 10   : grad1 (fx, fy, fz) = (g010, -g001, g100), 3 outputs;
+ *   kind 11   : grad2 after Reed (1973) (src/shs/shs.h.in:278-312),
+ *               (fxx, fxy, fxz, fyy, fyz, fzz), 6 outputs.
+ * f receives the outputs one after another, npoint values each. */
+int orc_shs_deriv_sctr(int kind, unsigned long nmax, unsigned long nmax_shc,
+                       const double *C_i, const double *S_i, const double *mu_i,
+                       const double *R_i, size_t npoint, const double *lat_i,
+                       const double *lon_i, const double *r_i, double *f_o)
+{
+    if (kind < 0 || kind > 11) return 1;
+    init_consts();
+    const size_t ncs = tri_len(nmax_shc);
+    real *C = ld_all(C_i, ncs), *S = ld_all(S_i, ncs);
+    real mu = ld(mu_i, 1, 0), R = ld(R_i, 1, 0);
+    real *lat = ld_all(lat_i, npoint), *lon = ld_all(lon_i, npoint), *rr = ld_all(r_i, npoint);
+    int unit = 1;
+    for (size_t i = 0; i < npoint; i++)
+        if (!nearly_equal(rr[i], R, GLOB_THRESHOLD)) { unit = 0; break; }
+    real *rt = (real *)malloc((2 * nmax + 4) * sizeof(real));
+    real *rit = (real *)malloc((2 * nmax + 4) * sizeof(real));
+    real *dm = (real *)malloc((nmax + 1) * sizeof(real));
+    leg_tables(nmax, rt, rit, dm);
+    const int nout = kind < 10 ? 1 : (kind == 10 ? 3 : 6);
+#pragma omp parallel
+    {
+    real *anm = (real *)malloc((nmax + 2) * sizeof(real));
+    real *bnm = (real *)malloc((nmax + 2) * sizeof(real));
+    real *enm = (real *)malloc((nmax + 2) * sizeof(real));
+    real *ps = (real *)malloc((nmax + 1) * sizeof(real));
+    int64_t *ips = (int64_t *)malloc((nmax + 1) * sizeof(int64_t));
+    real *rp = (real *)malloc((nmax + 3) * sizeof(real));
+    real *p = (real *)malloc((nmax + 1) * sizeof(real));
+#pragma omp for schedule(dynamic)
+    for (size_t i = 0; i < npoint; i++) {
+        real g[10], o[6];
+        guru_point(nmax, nmax_shc, C, S, mu, R, lat[i], lon[i], rr[i], unit, rt, rit, dm,
+                   anm, bnm, enm, ps, ips, rp, p, g);
+        const real r = unit ? R : rr[i], ri = ((real)1) / r;
+        const real tu = RSIN(lat[i]) / RCOS(lat[i]);
+        if (kind < 10) {
+            o[0] = g[kind];
+        } else if (kind == 10) {
+            o[0] = g[2];
+            o[1] = -g[3];
+            o[2] = g[1];
+        } else {
+            o[0] = g[5] + g[1] * ri;
+            o[1] = -g[9] - tu * g[3] * ri;
+            o[2] = g[7] - g[2] * ri;
+            o[3] = g[6] + g[1] * ri - tu * g[2] * ri;
+            o[4] = -g[8] + g[3] * ri;
+            o[5] = g[4];
+        }
+        for (int q = 0; q < nout; q++) st(f_o, nout * npoint, q * npoint + i, o[q]);
+    }
+    free(anm); free(bnm); free(enm); free(ps); free(ips); free(rp); free(p);
+    }
+    free(C); free(S); free(lat); free(lon); free(rr); free(rt); free(rit); free(dm);
+    return 0;
+}
+
 /* Fully-normalised P_nm(sin lat) for all 0 <= m <= n <= nmax at one latitude,
  * through the same recurrences (m-major triangle output, ncs values). */
 int orc_leg_pnm(unsigned long nmax, const double *lat_i, double *p_o)

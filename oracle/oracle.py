@@ -55,6 +55,8 @@ class Oracle:
         L.orc_sha_dh.argtypes = [ct.c_int, _ul, _ul, _sz, _sz, _dp, _dp, _dp, _dp, _dp, _dp,
                                  _dp, _dp]
         L.orc_leg_pnm.argtypes = [_ul, _dp, _dp]
+        L.orc_shs_deriv_sctr.argtypes = [ct.c_int, _ul, _ul, _dp, _dp, _dp, _dp, _sz, _dp, _dp,
+                                         _dp, _dp]
         L.orc_set_polar.argtypes = [_ul, ct.c_double]
 
     # -- layout helpers ------------------------------------------------------
@@ -188,6 +190,40 @@ class Oracle:
         if rc:
             raise ValueError("invalid SHA input")
         return self.unpack(C, ncs, exact), self.unpack(S, ncs, exact)
+
+    # g_ijk order of orc_shs_deriv_sctr
+    GURU = ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (2, 0, 0), (0, 2, 0), (0, 0, 2),
+            (1, 1, 0), (1, 0, 1), (0, 1, 1))
+
+    def shs_deriv_sctr(self, what, nmax, C, S, mu, R, lat, lon, r, exact=False):
+        """Derivatives at scattered points: what = (dr, dlat, dlon) for one
+        g_ijk, or "grad1" (fx, fy, fz), or "grad2" (fxx, fxy, fxz, fyy, fyz,
+        fzz).  Returns an array of shape (nout, npoint)."""
+        if what == "grad1":
+            kind, nout = 10, 3
+        elif what == "grad2":
+            kind, nout = 11, 6
+        else:
+            kind, nout = self.GURU.index(tuple(what)), 1
+        nmax_shc = _nmax_from_len(len(C))
+        n = len(lat)
+        f = self.empty(nout * n)
+        args = [self.pack(v) for v in (C, S, [mu], [R], lat, lon, r)]
+        rc = self.lib.orc_shs_deriv_sctr(kind, nmax, nmax_shc, *(self._p(a) for a in args[:4]),
+                                         n, *(self._p(a) for a in args[4:]), self._p(f))
+        if rc:
+            raise ValueError("invalid derivative kind")
+        return self.unpack(f, nout * n, exact).reshape(nout, n)
+
+    def shs_deriv_grid(self, what, nmax, C, S, mu, R, lat, lon, r, exact=False):
+        """shs_deriv_sctr on every point of a grid (rows lat with radii r,
+        columns lon); returns (nout, nlat, nlon)."""
+        nlat, nlon = len(lat), len(lon)
+        la = np.repeat(np.asarray(lat, dtype=object if exact else None), nlon)
+        lo = np.tile(np.asarray(lon, dtype=object if exact else None), nlat)
+        ra = np.repeat(np.asarray(r, dtype=object if exact else None), nlon)
+        out = self.shs_deriv_sctr(what, nmax, C, S, mu, R, la, lo, ra, exact)
+        return out.reshape(out.shape[0], nlat, nlon)
 
     def leg_pnm(self, nmax, lat, exact=False):
         ncs = (nmax + 1) * (nmax + 2) // 2

@@ -186,3 +186,59 @@ def custom_grid(i, fft, symm, prec="double"):
             lat[0] = lat[0] - mpmath.mpf(BREAK_SYMM)
     return lat, lon
 
+
+
+# --------------------------------------------------------------------------
+# Gradient and guru fixtures (tests/check_shs_point_all.c)
+# --------------------------------------------------------------------------
+GRAD_NSCTR = [1, 2, 3, 4, 5, 6, 7, 8, 9, 20, 30, 31]
+GURU_ORDER = ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (2, 0, 0), (0, 2, 0), (0, 0, 2),
+              (1, 1, 0), (1, 0, 1), (0, 1, 1))
+
+
+def elements(what):
+    """File suffixes of the outputs of "grad1", "grad2" or a guru (i, j, k)."""
+    if what == "grad1":
+        return ["x", "y", "z"]
+    if what == "grad2":
+        return ["xx", "xy", "xz", "yy", "yz", "zz"]
+    return ["guru%d%d%d" % tuple(what)]
+
+
+def singular(what, j):
+    """tests/check_shs_point_all.c:256-300: which outputs are zeroed near
+    the poles."""
+    if what == "grad1":
+        return j != 2
+    if what == "grad2":
+        return j != 5
+    _, dlat, dlon = what
+    return dlat > 0 or dlon > 0
+
+
+def zero_polar(prec, lat_rows, f, ncol):
+    """tests/check_shs_point_all.c:229-252 on one output (rows of ncol)."""
+    f = np.array(f, dtype=object if f.dtype == object else np.float64).reshape(-1, ncol)
+    if prec == "double":
+        lim = np.pi / 2 - 100 * THRESH[prec]
+        polar = [abs(float(la)) > lim for la in lat_rows]
+    else:
+        import mpmath
+
+        with mpmath.workprec(113):
+            lim = +(mpmath.pi / 2 - 100 * mpmath.mpf(THRESH[prec]))
+            polar = [abs(mpmath.mpf(la)) > lim for la in lat_rows]
+    for i, pl in enumerate(polar):
+        if pl:
+            f[i, :] = 0
+    return f.ravel()
+
+
+def dsun_rows(lat):
+    """Rows of a GL-typed grid whose first latitude was moved: the mirror of
+    ring i is the row nlat-1-i at -lat[i] (src/shs/shs_grd_fft.c:117-130)."""
+    rows = np.array(lat, copy=True)
+    n = len(rows)
+    for i in range(n // 2):
+        rows[n - 1 - i] = -rows[i]
+    return rows

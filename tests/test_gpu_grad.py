@@ -1,0 +1,262 @@
+"""GPU parity of charm_shs_point_grad1 / grad2 / guru.
+
+Against CHarm's golden fixtures with CHarm's own tolerances and polar
+zeroing (tests/check_shs_point_all.c:229-300, 420-900), against the same
+fixtures from CHarm built in quad precision (max error relative to max|f|
+at most 1e-12 per output), and against the oracle at larger degrees and on
+Driscoll-Healy grids, which the fixtures do not cover.  For gradients and
+tensors the error bound of 1e-12 is relative to their largest element.
+"""
+import numpy as np
+import pytest
+
+import charm_amd
+from charm_amd import crd, shc, shs
+from charm_fixtures import (BREAK_SYMM, DELTAR, GRAD_NSCTR, GURU_ORDER, THRESH, THRESH2,
+                            custom_grid, dsun_rows, egm96_test_model, elements, nearly_equal,
+                            read_fixture, scattered_points, singular, zero_polar)
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if charm_amd.lib().charm_amd_device_count() < 1:
+        pytest.fail("no HIP device visible: the GPU tests need an MI355X")
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle("d")
+
+
+def relmax(x, ref):
+    x, ref = np.asarray(x, dtype=float), np.asarray(ref, dtype=float)
+    return float(np.max(np.abs(x - ref)) / max(np.max(np.abs(ref)), 1e-300))
+
+
+def relmax_set(outs, refs):
+    """Largest error over all outputs relative to the largest value of any
+    output: for a gradient or tensor, relative to its largest element."""
+    err = max(float(np.max(np.abs(np.asarray(o, dtype=float) - np.asarray(r, dtype=float))))
+              for o, r in zip(outs, refs))
+    return err / max(max(float(np.max(np.abs(np.asarray(r, dtype=float)))) for r in refs), 1e-300)
+
+
+def model(nmax, C, S, mu, R):
+    return shc.Shc.from_arrays(nmax, np.array(C, dtype=float), np.array(S, dtype=float), mu, R)
+
+
+def run(what, pnt, m, nmax):
+    if what == "grad1":
+        return shs.point_grad1(pnt, m, nmax)
+    if what == "grad2":
+        return shs.point_grad2(pnt, m, nmax)
+    return (shs.point_guru(pnt, m, nmax, *what),)
+
+
+WHATS = ["grad1", "grad2"] + list(GURU_ORDER)
+
+
+def _compare(what, outs, lat_rows, ncol, name, tol):
+    for j, el in enumerate(elements(what)):
+        got = np.asarray(outs[j]).ravel()
+        if singular(what, j):
+            got = zero_polar("double", lat_rows, got, ncol)
+        ref = read_fixture("double", name % el)
+        assert nearly_equal(got, ref, tol).all(), (el, relmax(got, ref))
+        # Against quad: 1e-12, or within 4x of CHarm's own double build where
+        # that is worse.  Latitude derivatives carry 1/cos(lat): 1e-3 rad from
+        # a pole CHarm in double is 7e-11 off its quad build (custom grids
+        # with a moved first latitude).
+        refq = read_fixture("quad", name % el)
+        tolq = max(TOL, 4 * relmax(ref, refq))
+        assert relmax(got, refq) <= tolq, (el, relmax(got, refq), tolq)
+
+
+@pytest.mark.parametrize("nmax", range(0, 5))
+@pytest.mark.parametrize("deltar", (0, 1))
+@pytest.mark.parametrize("what", WHATS, ids=str)
+def test_gl_fixtures(nmax, deltar, what):
+    mu, R, C, S = egm96_test_model()
+    m = model(10, C, S, mu, R)
+    g = crd.PointGridGL(nmax, R + DELTAR * deltar)
+    outs = run(what, g, m, nmax)
+    _compare(what, outs, g.lat, g.lon.size,
+             "shs_p_nx%d_dr%d_fft1_gl_%%s.txt" % (nmax, deltar), 1000 * THRESH["double"])
+
+
+@pytest.mark.parametrize("nmax", range(0, 5))
+@pytest.mark.parametrize("what", ["grad1", "grad2"])
+def test_sctr_fixtures(nmax, what):
+    mu, R, C, S = egm96_test_model()
+    m = model(10, C, S, mu, R)
+    for i, npt in enumerate(GRAD_NSCTR):
+        lat, lon = scattered_points(npt, "double")
+        for deltar in (0, 1):
+            p = crd.PointSctr(lat, lon, np.full(npt, R + DELTAR * deltar))
+            outs = run(what, p, m, nmax)
+            _compare(what, outs, lat, 1, "shs_p_nx%d_n%d_dr%d_sctr_%%s.txt" % (nmax, i, deltar),
+                     1000 * THRESH["double"])
+
+
+@pytest.mark.parametrize("i", (8, 11))
+def test_sctr_guru_fixtures(i):
+    mu, R, C, S = egm96_test_model()
+    m = model(10, C, S, mu, R)
+    npt = GRAD_NSCTR[i]
+    lat, lon = scattered_points(npt, "double")
+    p = crd.PointSctr(lat, lon, np.full(npt, R + DELTAR))
+    for what in GURU_ORDER:
+        outs = run(what, p, m, 4)
+        _compare(what, outs, lat, 1, "shs_p_nx4_n%d_dr1_sctr_%%s.txt" % i,
+                 1000 * THRESH["double"])
+
+
+def _custom_cases():
+    import os
+    from charm_fixtures import DATA
+
+    out = []
+    for i in range(4):
+        for fft in (0, 1):
+            for s in (0, 1):
+                name = "shs_p_nx4_n%d_dr%%d_fft%d_s%d_%%%%s.txt" % (i, fft, s)
+                if os.path.exists(os.path.join(DATA, "double", (name % 0) % "x")):
+                    out.append((i, fft, s, name))
+    return out
+
+
+@pytest.mark.parametrize("i,fft,s,name", _custom_cases())
+@pytest.mark.parametrize("what", ["grad1", "grad2"])
+def test_custom_grid_fixtures(i, fft, s, name, what):
+    """Custom grids: FFT (fft1) and direct longitude sums (fft0), symmetric
+    (s1) and not (s0)."""
+    mu, R, C, S = egm96_test_model()
+    m = model(10, C, S, mu, R)
+    lat, lon = custom_grid(i, fft, s == 1)
+    for deltar in (0, 1):
+        g = crd.PointGrid(lat, lon, np.full(lat.size, R + DELTAR * deltar))
+        outs = run(what, g, m, 4)
+        _compare(what, outs, lat, lon.size, name % deltar, 1000 * THRESH2["double"])
+
+
+@pytest.mark.parametrize("nmax", (150, 154))
+@pytest.mark.parametrize("s", (0, 1))
+@pytest.mark.parametrize("what", ["grad1", "grad2"])
+def test_dsun_fixtures(nmax, s, what):
+    """GL latitudes of degree 150/154 with one longitude: the X-number range
+    of the recurrences, direct longitude sums."""
+    from charm_amd.workloads import degree_order
+
+    n_of, m_of = degree_order(nmax)
+    C = 1.0 / (n_of + 1.0)
+    S = np.where(m_of > 0, C, 0.0)
+    gl = crd.PointGridGL(nmax, 1.0)
+    lat = gl.lat.copy()
+    if s == 0:
+        lat[0] -= BREAK_SYMM
+    tol = (10 if what == "grad1" else 1000) * THRESH2["double"]
+    for deltar in (0, 1):
+        pnt = crd.PointGridGL(nmax, 1.0 + DELTAR * deltar)
+        pnt.lat[:] = lat
+        pnt._Point.contents.nlon = 1
+        pnt._Point.contents.npoint = len(lat)
+        m = model(nmax, C, S, 1.0, 1.0)
+        outs = run(what, pnt, m, nmax)
+        gots, refs = [], []
+        for j, el in enumerate(elements(what)):
+            got = np.asarray(outs[j]).ravel()[: len(lat)]
+            if singular(what, j):
+                got = zero_polar("double", dsun_rows(lat), got, 1)
+            ref = read_fixture("double", "shs_p_nx%d_n0_dr%d_fft0_s%d_dsun_%s.txt"
+                               % (nmax, deltar, s, el))
+            assert nearly_equal(got, ref, tol).all(), (el, relmax(got, ref))
+            gots.append(got)
+            refs.append(ref)
+        # fxy is ~1e-19 of the tensor here: the error is relative to the tensor
+        assert relmax_set(gots, refs) <= TOL, relmax_set(gots, refs)
+
+
+# --------------------------------------------------------------------------
+# Oracle comparisons beyond the fixtures
+# --------------------------------------------------------------------------
+def _kaula(nmax, seed=11):
+    from charm_amd.workloads import GM, R_EARTH, kaula_coeffs
+
+    C, S = kaula_coeffs(nmax, seed=seed)
+    C[0] = 1.0
+    return C, S, GM, R_EARTH
+
+
+@pytest.mark.parametrize("dh", (1, 2))
+@pytest.mark.parametrize("deltar", (0, 1))
+def test_dh_grid_vs_oracle(orc, dh, deltar):
+    nmax = 40
+    C, S, mu, R = _kaula(nmax)
+    m = model(nmax, C, S, mu, R)
+    g = (crd.PointGridDH1 if dh == 1 else crd.PointGridDH2)(nmax, R + 1e4 * deltar)
+    for what in ("grad1", "grad2") + GURU_ORDER:
+        outs = run(what, g, m, nmax)
+        ref = orc.shs_deriv_grid(what, nmax, C, S, mu, R, g.lat, g.lon, g.r)
+        for j in range(ref.shape[0]):
+            got = np.asarray(outs[j]).ravel()
+            want = ref[j].ravel()
+            if singular(what, j):
+                got = zero_polar("double", g.lat, got, g.lon.size)
+                want = zero_polar("double", g.lat, want, g.lon.size)
+            assert relmax(got, want) <= TOL, (what, j, relmax(got, want))
+
+
+@pytest.mark.parametrize("nmax", (360, 1000))
+def test_gl_grad2_sampled_rows(orc, nmax):
+    """Degree 360 and 1000 GL grids, rows sampled from pole to equator."""
+    C, S, mu, R = _kaula(nmax)
+    m = model(nmax, C, S, mu, R)
+    g = crd.PointGridGL(nmax, R)
+    outs = shs.point_grad2(g, m, nmax)
+    rows = np.unique(np.r_[0, 1, 5, g.lat.size // 3, g.lat.size // 2, g.lat.size - 1])
+    cols = np.arange(0, g.lon.size, 37)
+    ref = orc.shs_deriv_grid("grad2", nmax, C, S, mu, R, g.lat[rows], g.lon[cols], g.r[rows])
+    got = [np.asarray(outs[j])[np.ix_(rows, cols)] for j in range(6)]
+    assert relmax_set(got, ref) <= TOL, relmax_set(got, ref)
+
+
+def test_sctr_vs_oracle(orc):
+    nmax = 300
+    C, S, mu, R = _kaula(nmax)
+    m = model(nmax, C, S, mu, R)
+    rng = np.random.default_rng(2)
+    npt = 3000
+    # up to 0.07 rad from the poles; closer, 1/cos(lat) amplifies the
+    # rounding of both the product and the oracle
+    lat = rng.uniform(-1.5, 1.5, npt)
+    lon = rng.uniform(-np.pi, 3 * np.pi, npt)
+    r = R + rng.uniform(0, 4e5, npt)
+    p = crd.PointSctr(lat, lon, r)
+    for what in ("grad1", "grad2") + GURU_ORDER:
+        outs = run(what, p, m, nmax)
+        ref = orc.shs_deriv_sctr(what, nmax, C, S, mu, R, lat, lon, r)
+        assert relmax_set(outs, ref) <= TOL, (what, relmax_set(outs, ref))
+
+
+def test_guru000_is_the_synthesis():
+    C, S, mu, R = _kaula(200)
+    m = model(200, C, S, mu, R)
+    g = crd.PointGridGL(200, R + 100.0)
+    assert np.array_equal(shs.point_guru(g, m, 200, 0, 0, 0), shs.point(g, m, 200))
+
+
+def test_laplace_trace_2160():
+    """fxx + fyy + fzz = 0 at every node of the degree-2160 GL grid."""
+    nmax = 2160
+    C, S, mu, R = _kaula(nmax)
+    m = model(nmax, C, S, mu, R)
+    g = crd.PointGridGL(nmax, R)
+    t = shs.point_grad2(g, m, nmax)
+    scale = max(np.max(np.abs(t[j])) for j in range(6))
+    assert np.max(np.abs(t[0] + t[3] + t[5])) <= TOL * scale
+

@@ -150,6 +150,23 @@ def test_nmax_check_reports_efuncarg():
     charm_amd._err.free(err)
 
 
+def test_derivative_argument_errors():
+    """Checks made before any device work (src/shs/shs_point_guru.c:30-62,
+    src/shs/shs_check_single_derivative.c:17-64)."""
+    s = shc.Shc.from_zeros(3, 1.0, 1.0)
+    g = crd.PointGridGL(3, 1.0)
+    with pytest.raises(charm_amd.CHarmError, match="cannot be larger than \"2\""):
+        charm_amd.shs.point_guru(g, s, 3, 1, 1, 1)
+    with pytest.raises(ValueError):
+        charm_amd.shs.point_guru(g, s, 3, -1, 0, 0)
+    with pytest.raises(ValueError):
+        charm_amd.shs.point_grad1(g, s, 4)
+    # zero points: nothing to do
+    p = crd.PointSctr(np.zeros(0), np.zeros(0), np.zeros(0))
+    assert charm_amd.shs.point_guru(p, s, 3, 1, 0, 0).size == 0
+    assert all(x.size == 0 for x in charm_amd.shs.point_grad2(p, s, 3))
+
+
 def test_sha_rejects_scattered_points():
     s = shc.Shc.from_zeros(2)
     p = crd.PointSctr(np.zeros(3), np.zeros(3), np.ones(3))
