@@ -16,6 +16,11 @@ Workloads (--workload):
           U{90..2700}, 1e3..1e5 points each, per-call vs batched.
   multi   256 coefficient sets at nmax = 720 on one GL grid (721 x 1442):
           synthesis by the FP64 matrix-core contraction vs one call per set.
+  grad    second-order gradient (fxx..fzz, charm_shs_point_grad2) on the GL
+          grid of nmax = 2160, device-resident; one step = one grad2 call;
+          value = (nmax+1)^2 nlat / step time.  Beside it: grad1 and the
+          plain synthesis on the same grid, grad1 at 2e5 scattered points,
+          and the CPU baseline (the oracle's gradient at scattered points).
   serve   Poisson arrivals against a resident nmax = 2190 model (60% scattered
           SHS of 1e3..1e6 points, 30% GL SHS on grids nmax ~ U{180..5400}
           with the model truncated to the grid, 10% GL SHA nmax ~ U{180..2160});
@@ -193,6 +198,104 @@ def sctr_measure(N, cs, npoint, stream):
             "sctr_units_per_s": (N + 1) ** 2 * npoint / dt,
             "sctr_kernel_ms": kms,
             "sctr_kernel_frac_fp64_peak": 7.0 * P * npoint / (kms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
+
+
+def grad_bench(args):
+    import numpy as np
+
+    import charm_amd
+    from charm_amd.device import (DERIV_GRAD1, DERIV_GRAD2, DevArray, DeviceGrid, Stream,
+                                  shs_sctr_deriv)
+    from charm_amd.workloads import GM, R_EARTH, kaula_coeffs, scattered_points
+
+    lib = charm_amd.lib()
+    N = 2160
+    C, S = kaula_coeffs(N, N)
+    grid = DeviceGrid(N, R_EARTH)
+    nlat, nlon = grid.nlat, grid.nlon
+    cs = DevArray.from_numpy(np.stack([C, S], axis=1))
+    f6 = DevArray((6, nlat, nlon))
+    f1 = DevArray((nlat, nlon))
+    stream = Stream()
+    for _ in range(args.warmup):
+        grid.shs_deriv(cs, N, GM, R_EARTH, DERIV_GRAD2, f6, stream)
+    stream.synchronize()
+    t6 = f6.numpy()
+    trace_err = float(np.abs(t6[0] + t6[3] + t6[5]).max() / np.abs(t6).max())
+    print("# warm-up done, Laplace trace |fxx+fyy+fzz|/max %.2e" % trace_err, file=sys.stderr,
+          flush=True)
+
+    def timed(fn, steps):
+        lib.charm_amd_prof_reset()
+        lib.charm_amd_prof_enable(1)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        stream.synchronize()
+        dt = time.perf_counter() - t0
+        lib.charm_amd_prof_enable(0)
+        return dt / steps
+
+    g2 = timed(lambda: grid.shs_deriv(cs, N, GM, R_EARTH, DERIV_GRAD2, f6, stream), args.steps)
+    k_shs = kernel_ms(lib, "shs_grid_kernel")
+    k_fft = kernel_ms(lib, "fft_c2r")
+    k_cmb = kernel_ms(lib, "grad_combine_kernel")
+    ntab = k_shs[1] // args.steps
+    g1 = timed(lambda: grid.shs_deriv(cs, N, GM, R_EARTH, DERIV_GRAD1, f6, stream),
+               max(1, args.steps // 2))
+    s0 = timed(lambda: grid.shs(cs, N, GM, R_EARTH, f1, stream), max(1, args.steps // 2))
+    units = float(N + 1) ** 2 * nlat
+    P = (N + 1) * (N + 2) // 2
+    flops = 8.0 * P * ((nlat + 1) // 2)
+    shs_avg = k_shs[0] / k_shs[1]
+    achieved = flops / (shs_avg * 1e-3) / 1e12
+    cmb_avg = k_cmb[0] / max(k_cmb[1], 1)
+    cmb_bytes = (ntab + 6) * 8.0 * nlat * nlon + 3 * 8.0 * nlat
+    line = base_line("(nmax+1)^2*nlat/s FP64 second-order gradient (6 outputs, GL grid, "
+                     "device-resident)", units / g2, args.steps, args.warmup, 1e3 * g2,
+                     {"workload": "charm_shs_point_grad2 on the GL grid nmax=%d, %dx%d, r=R"
+                                  % (N, nlat, nlon),
+                      "nmax": N, "nlat": nlat, "nlon": nlon, "parallelism": "single GPU"})
+    line.update({
+        "grad2_ms": 1e3 * g2, "grad1_ms": 1e3 * g1, "shs_ms": 1e3 * s0,
+        "grad2_over_shs": g2 / s0, "syntheses_per_grad2": ntab,
+        "kernel_avg_ms": {"shs_grid_kernel": shs_avg, "fft_c2r": k_fft[0] / max(k_fft[1], 1),
+                          "grad_combine_kernel": cmb_avg},
+        "laplace_trace_rel": trace_err,
+        "combine_GBps": cmb_bytes / (cmb_avg * 1e-3) / 1e9,
+        "roofline": {"bound": "mfma", "compute": "vector FP64 (VALU)", "kernel": "shs_grid_kernel",
+                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": TRAFFIC_GL2160["shs_grid_kernel"], "flops_per_launch": flops},
+    })
+    del f6, f1
+    # grad1 at scattered points
+    npt = args.grad_points
+    lat, lon, r = scattered_points(npt, 777)
+    d = [DevArray.from_numpy(a) for a in (lat, lon, r)]
+    out = DevArray((3, npt))
+    shs_sctr_deriv(*d, cs, N, GM, R_EARTH, DERIV_GRAD1, out, stream)
+    stream.synchronize()
+    gs = timed(lambda: shs_sctr_deriv(*d, cs, N, GM, R_EARTH, DERIV_GRAD1, out, stream), 1)
+    line.update({"sctr_grad1_npoint": npt, "sctr_grad1_s": gs,
+                 "sctr_grad1_units_per_s": (N + 1) ** 2 * npt / gs})
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        from oracle.oracle import Oracle
+
+        orc = Oracle("d")
+        ns = args.cpu_grad_points
+        t0 = time.perf_counter()
+        orc.shs_deriv_sctr("grad1", N, C, S, GM, R_EARTH, lat[:ns], lon[:ns], r[:ns])
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {
+            "value": (N + 1) ** 2 * ns / dt, "unit": UNIT, "cores": threads, "kind": "port",
+            "sample": "grad1 at %d of the scattered points at nmax=%d (CHarm's per-term "
+                      "derivative recurrences), oracle in double, %.1f s wall; compare "
+                      "sctr_grad1_units_per_s" % (ns, N, dt)}
+    return line
 
 
 def batch_bench(args):
@@ -468,11 +571,13 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="gl2160",
-                    choices=("gl2160", "gl10800", "batch", "multi", "serve"))
+                    choices=("gl2160", "gl10800", "batch", "multi", "grad", "serve"))
     ap.add_argument("--cpu-rows", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sctr", action="store_true")
     ap.add_argument("--sctr-points", type=int, default=1000000)
+    ap.add_argument("--grad-points", type=int, default=200000)
+    ap.add_argument("--cpu-grad-points", type=int, default=3000)
     ap.add_argument("--multi-sets", type=int, default=256)
     ap.add_argument("--multi-nmax", type=int, default=720)
     ap.add_argument("--requests", type=int, default=40)
@@ -498,6 +603,8 @@ def main():
         line = batch_bench(args)
     elif args.workload == "multi":
         line = multi_bench(args)
+    elif args.workload == "grad":
+        line = grad_bench(args)
     else:
         line = serve_bench(args)
     print(json.dumps(line), flush=True)

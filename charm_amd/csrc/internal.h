@@ -45,7 +45,7 @@ struct ShsGridArgs {
     const double *sx;  // sectorial seeds, [(m-1)*sld + i]
     const int *six;
     size_t sld;
-    const double *rp;  // unit == 0: (R/r)^k, [k*rld + row], k = 0..nmax+1
+    const double *rp;  // unit == 0: (R/r)^k at [k*rld + row], k = 0..nmax+1; rld = 0: [k]
     const double *ratio;  // unit == 0: R/r per row
     size_t rld;
     const unsigned char *mirror;  // [nring] 1 if row nlat-1-i is the mirror ring

@@ -370,8 +370,10 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
                 const int rs = A.mirror[ring[k]] ? A.nlat - 1 - ring[k] : ring[k];
                 qN[k] = A.ratio[ring[k]];
                 qS[k] = A.ratio[rs];
-                rpN[k] = A.rp[(size_t)(m + 1) * A.rld + ring[k]];
-                rpS[k] = A.rp[(size_t)(m + 1) * A.rld + rs];
+                // rld = 0: one radius for every row, a single column of powers
+                const size_t col = A.rld ? (size_t)(m + 1) * A.rld : (size_t)(m + 1);
+                rpN[k] = A.rp[col + (A.rld ? ring[k] : 0)];
+                rpS[k] = A.rp[col + (A.rld ? rs : 0)];
             }
         }
         // accumulate term n with Legendre value p; par = (n + m) & 1

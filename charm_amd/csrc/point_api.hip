@@ -895,6 +895,7 @@ void shs_sctr_host(const charm_point *pnt, const charm_shc *shcs, unsigned long 
 struct GradPlan {
     std::vector<GradTable> tabs;
     std::vector<std::vector<GradTerm>> outs;
+    GradPlan() = default;
     explicit GradPlan(int nout) : outs(nout) {}
     void term(int o, double coef, GradTable t, int pt, int pu, int pr, int pq)
     {
@@ -1022,6 +1023,58 @@ void grid_rows_tu(const GridGeom &g, std::vector<double> &t, std::vector<double>
     }
 }
 
+// One synthesis per table of the plan on a grid: tabs[k * nlat * nlon + p]
+void grad_tables_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev, bool use_fft,
+                      unsigned long nmax, double mur, const void *cs_dev, const Tables &tb,
+                      const GradPlan &plan, double *tabs, hipStream_t s)
+{
+    const size_t npoint = g.nlat * g.nlon;
+    DevBuf ctab(tri_len(nmax) * 2 * sizeof(double), s);
+    for (size_t k = 0; k < plan.tabs.size(); k++) {
+        HIPCHK((hipError_t)launch_grad_table((int)nmax, cs_dev, plan.tabs[k], ctab.p, s));
+        run_shs_grid(g, d, lon_dev, use_fft, nmax, mur, ctab.p, tb, tabs + k * npoint, s);
+    }
+}
+
+// One synthesis per table of the plan at scattered points
+void grad_tables_sctr(size_t npoint, const double *lat_dev, const double *lon_dev,
+                      const double *r_dev, bool unit, unsigned long nmax, double mu, double R,
+                      const void *cs_dev, const Tables &tb, const GradPlan &plan, double *tabs,
+                      hipStream_t s)
+{
+    DevBuf ctab(tri_len(nmax) * 2 * sizeof(double), s);
+    for (size_t k = 0; k < plan.tabs.size(); k++) {
+        HIPCHK((hipError_t)launch_grad_table((int)nmax, cs_dev, plan.tabs[k], ctab.p, s));
+        run_sctr(npoint, lat_dev, lon_dev, r_dev, unit, nmax, mu, R, ctab.p, tb,
+                 tabs + k * npoint, s);
+    }
+}
+
+// Outputs o = 0..nout-1 into out + o * nrow * ncol
+void grad_combine(const GradPlan &plan, size_t nrow, size_t ncol, const double *t_dev,
+                  const double *u_dev, const double *r_dev, bool unit, double R,
+                  const double *tabs, double *out, hipStream_t s)
+{
+    const int nout = (int)plan.outs.size();
+    size_t nterm = 0;
+    for (const auto &o : plan.outs) nterm += o.size();
+    if (nout > GRAD_MAXOUT || nterm > (size_t)GRAD_MAXTERM)
+        throw HipFail{hipErrorInvalidValue, "derivative plan too large", __FILE__, __LINE__};
+    const size_t npoint = nrow * ncol;
+    GradCombine c{};
+    c.nout = nout;
+    int q = 0;
+    for (int o = 0; o < nout; o++) {
+        c.beg[o] = q;
+        for (const GradTerm &x : plan.outs[o]) c.term[q++] = x;
+        c.out[o] = out + (size_t)o * npoint;
+    }
+    c.beg[nout] = q;
+    ProfScope ps("grad_combine_kernel", s);
+    HIPCHK((hipError_t)launch_grad_combine(nrow, ncol, t_dev, u_dev, r_dev, unit ? 1 : 0, R, tabs,
+                                           npoint, c, s));
+}
+
 void shs_grad_host(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax,
                    const GradPlan &plan, double **f, charm_err *err)
 {
@@ -1034,17 +1087,12 @@ void shs_grad_host(const charm_point *pnt, const charm_shc *shcs, unsigned long 
     }
     if (!sctr && !check_grid_lons(pnt, err)) return;
     const int nout = (int)plan.outs.size();
-    size_t nterm = 0;
-    for (const auto &o : plan.outs) nterm += o.size();
-    if (nout > GRAD_MAXOUT || nterm > (size_t)GRAD_MAXTERM)
-        throw HipFail{hipErrorInvalidValue, "derivative plan too large", __FILE__, __LINE__};
     hipStream_t s = thread_stream();
     Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
     DevBuf cs;
     upload_coeffs(shcs, nmax, s, cs);
-    const size_t P = tri_len(nmax), ntab = plan.tabs.size();
+    const size_t ntab = plan.tabs.size();
     const size_t nrow = pnt->nlat, ncol = sctr ? 1 : pnt->nlon, npoint = nrow * ncol;
-    DevBuf ctab(P * 2 * sizeof(double), s);
     DevBuf tabs(ntab * npoint * sizeof(double), s), outs(nout * npoint * sizeof(double), s);
     DevBuf dt(nrow * sizeof(double), s), du(nrow * sizeof(double), s);
     DevBuf dr(nrow * sizeof(double), s);
@@ -1058,11 +1106,8 @@ void shs_grad_host(const charm_point *pnt, const charm_shc *shcs, unsigned long 
         h2d(dlon.as<double>(), pnt->lon, npoint, s);
         HIPCHK((hipError_t)launch_sincos(npoint, dlat.as<double>(), dt.as<double>(),
                                          du.as<double>(), s));
-        for (size_t k = 0; k < ntab; k++) {
-            HIPCHK((hipError_t)launch_grad_table((int)nmax, cs.p, plan.tabs[k], ctab.p, s));
-            run_sctr(npoint, dlat.as<double>(), dlon.as<double>(), dr.as<double>(), unit, nmax,
-                     shcs->mu, shcs->r, ctab.p, tb, tabs.as<double>() + k * npoint, s);
-        }
+        grad_tables_sctr(npoint, dlat.as<double>(), dlon.as<double>(), dr.as<double>(), unit,
+                         nmax, shcs->mu, shcs->r, cs.p, tb, plan, tabs.as<double>(), s);
     } else {
         GridSetup G;
         grid_setup(pnt, shcs, nmax, tb, s, G);
@@ -1070,27 +1115,34 @@ void shs_grad_host(const charm_point *pnt, const charm_shc *shcs, unsigned long 
         grid_rows_tu(G.g, ht, hu);
         h2d(dt.as<double>(), ht.data(), nrow, s);
         h2d(du.as<double>(), hu.data(), nrow, s);
-        for (size_t k = 0; k < ntab; k++) {
-            HIPCHK((hipError_t)launch_grad_table((int)nmax, cs.p, plan.tabs[k], ctab.p, s));
-            run_shs_grid(G.g, G.d, G.lon.as<double>(), G.use_fft, nmax, shcs->mu / shcs->r,
-                         ctab.p, tb, tabs.as<double>() + k * npoint, s);
-        }
+        grad_tables_grid(G.g, G.d, G.lon.as<double>(), G.use_fft, nmax, shcs->mu / shcs->r, cs.p,
+                         tb, plan, tabs.as<double>(), s);
     }
-    GradCombine c{};
-    c.nout = nout;
-    int q = 0;
-    for (int o = 0; o < nout; o++) {
-        c.beg[o] = q;
-        for (const GradTerm &x : plan.outs[o]) c.term[q++] = x;
-        c.out[o] = outs.as<double>() + (size_t)o * npoint;
-    }
-    c.beg[nout] = q;
-    HIPCHK((hipError_t)launch_grad_combine(nrow, ncol, dt.as<double>(), du.as<double>(),
-                                           dr.as<double>(), unit ? 1 : 0, shcs->r,
-                                           tabs.as<double>(), npoint, c, s));
+    grad_combine(plan, nrow, ncol, dt.as<double>(), du.as<double>(), dr.as<double>(), unit,
+                 shcs->r, tabs.as<double>(), outs.as<double>(), s);
     for (int o = 0; o < nout; o++)
-        HIPCHK(hipMemcpyAsync(f[o], c.out[o], npoint * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(f[o], outs.as<double>() + (size_t)o * npoint,
+                              npoint * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+}
+
+// Plan of a derivative request: CHARM_AMD_DERIV_GRAD1, CHARM_AMD_DERIV_GRAD2
+// or 100 dr + 10 dlat + dlon with dr + dlat + dlon <= 2; false if invalid
+bool deriv_plan(int what, GradPlan &p)
+{
+    if (what == CHARM_AMD_DERIV_GRAD1) {
+        p = grad1_plan();
+        return true;
+    }
+    if (what == CHARM_AMD_DERIV_GRAD2) {
+        p = grad2_plan();
+        return true;
+    }
+    if (what < 0 || what > 200) return false;
+    const int dr = what / 100, dlat = (what / 10) % 10, dlon = what % 10;
+    if (dlat > 2 || dlon > 2 || dr + dlat + dlon > 2) return false;
+    p = guru_plan((unsigned)dr, (unsigned)dlat, (unsigned)dlon);
+    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -1260,6 +1312,7 @@ struct charm_amd_grid {
     double r0 = 0.0;
     unsigned long nmax_grd = 0;
     double *t = nullptr, *u = nullptr, *w = nullptr, *lon = nullptr, *sx = nullptr;
+    double *trow = nullptr, *urow = nullptr, *rrow = nullptr;  // per row, for derivatives
     int *six = nullptr;
     unsigned char *mirror = nullptr;
     size_t sld = 0;
@@ -1479,6 +1532,15 @@ charm_amd_grid *charm_amd_grid_create(const charm_point *pnt, charm_err *err)
         HIPCHK(hipMalloc((void **)&grd->mirror, nr));
         HIPCHK(hipMalloc((void **)&grd->w, nlat * sizeof(double)));
         HIPCHK(hipMalloc((void **)&grd->lon, std::max<size_t>(pnt->nlon, 1) * sizeof(double)));
+        HIPCHK(hipMalloc((void **)&grd->trow, nlat * sizeof(double)));
+        HIPCHK(hipMalloc((void **)&grd->urow, nlat * sizeof(double)));
+        HIPCHK(hipMalloc((void **)&grd->rrow, nlat * sizeof(double)));
+        std::vector<double> ht, hu;
+        grid_rows_tu(grd->g, ht, hu);
+        const std::vector<double> hr(nlat, r0);
+        HIPCHK(hipMemcpy(grd->trow, ht.data(), nlat * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(grd->urow, hu.data(), nlat * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(grd->rrow, hr.data(), nlat * sizeof(double), hipMemcpyHostToDevice));
         grd->sld = nr;
         HIPCHK(hipMalloc((void **)&grd->sx, nm * nr * sizeof(double)));
         HIPCHK(hipMalloc((void **)&grd->six, nm * nr * sizeof(int)));
@@ -1509,6 +1571,9 @@ void charm_amd_grid_free(charm_amd_grid *grd)
     (void)hipFree(grd->lon);
     (void)hipFree(grd->sx);
     (void)hipFree(grd->six);
+    (void)hipFree(grd->trow);
+    (void)hipFree(grd->urow);
+    (void)hipFree(grd->rrow);
     delete grd;
 }
 
@@ -1664,6 +1729,82 @@ void charm_amd_shs_sctr_dev(size_t npoint, const double *lat_dev, const double *
         int unit = 0;
         HIPCHK((hipError_t)launch_all_near(npoint, r_dev, R, charm_glob_threshold, &unit, s));
         run_sctr(npoint, lat_dev, lon_dev, r_dev, unit != 0, nmax, mu, R, cs_dev, tb, f_dev, s);
+        HIPCHK(hipGetLastError());
+    } catch (const HipFail &e) {
+        report(err, e, __func__);
+    }
+}
+
+// Derivatives on a device-resident grid: outputs one after another in f_dev
+void charm_amd_shs_grid_deriv_dev(const charm_amd_grid *grd, unsigned long nmax, double mu,
+                                  double R, const double *cs_dev, int what, double *f_dev,
+                                  void *stream, charm_err *err)
+{
+    GradPlan plan;
+    if (!deriv_plan(what, plan)) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG, "Unsupported derivative request \"what\".");
+        return;
+    }
+    if (!grd || nmax > grd->nmax_grd) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG, "\"nmax\" exceeds the degree of the grid.");
+        return;
+    }
+    try {
+        hipStream_t s = (hipStream_t)stream;
+        Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
+        DevGrid d{grd->t, grd->u, grd->mirror, grd->sx, grd->six, grd->sld, nullptr, nullptr, 0};
+        DevBuf ratio, rp;
+        const bool unit = nearly_equal(grd->r0, R, charm_glob_threshold);
+        if (!unit) {
+            const double q = R / grd->r0;
+            std::vector<double> pw(nmax + 2);
+            pw[0] = 1.0;
+            for (unsigned long k = 1; k <= nmax + 1; k++) pw[k] = pw[k - 1] * q;
+            std::vector<double> qq(grd->g.nlat, q);
+            ratio.alloc(qq.size() * sizeof(double), s);
+            rp.alloc(pw.size() * sizeof(double), s);
+            h2d(ratio.as<double>(), qq.data(), qq.size(), s);
+            h2d(rp.as<double>(), pw.data(), pw.size(), s);
+            d.ratio = ratio.as<double>();
+            d.rp = rp.as<double>();
+            d.rld = 0;
+        }
+        const size_t nrow = grd->g.nlat, ncol = grd->g.nlon;
+        DevBuf tabs(plan.tabs.size() * nrow * ncol * sizeof(double), s);
+        grad_tables_grid(grd->g, d, grd->lon, true, nmax, mu / R, cs_dev, tb, plan,
+                         tabs.as<double>(), s);
+        grad_combine(plan, nrow, ncol, grd->trow, grd->urow, grd->rrow, unit, R, tabs.as<double>(),
+                     f_dev, s);
+        HIPCHK(hipGetLastError());
+    } catch (const HipFail &e) {
+        report(err, e, __func__);
+    }
+}
+
+// Derivatives at scattered points given by device arrays
+void charm_amd_shs_sctr_deriv_dev(size_t npoint, const double *lat_dev, const double *lon_dev,
+                                  const double *r_dev, unsigned long nmax, double mu, double R,
+                                  const double *cs_dev, int what, double *f_dev, void *stream,
+                                  charm_err *err)
+{
+    GradPlan plan;
+    if (!deriv_plan(what, plan)) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG, "Unsupported derivative request \"what\".");
+        return;
+    }
+    if (npoint == 0) return;
+    try {
+        hipStream_t s = (hipStream_t)stream;
+        Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
+        int unit = 0;
+        HIPCHK((hipError_t)launch_all_near(npoint, r_dev, R, charm_glob_threshold, &unit, s));
+        DevBuf tabs(plan.tabs.size() * npoint * sizeof(double), s);
+        DevBuf dt(npoint * sizeof(double), s), du(npoint * sizeof(double), s);
+        HIPCHK((hipError_t)launch_sincos(npoint, lat_dev, dt.as<double>(), du.as<double>(), s));
+        grad_tables_sctr(npoint, lat_dev, lon_dev, r_dev, unit != 0, nmax, mu, R, cs_dev, tb, plan,
+                         tabs.as<double>(), s);
+        grad_combine(plan, npoint, 1, dt.as<double>(), du.as<double>(), r_dev, unit != 0, R,
+                     tabs.as<double>(), f_dev, s);
         HIPCHK(hipGetLastError());
     } catch (const HipFail &e) {
         report(err, e, __func__);

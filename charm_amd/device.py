@@ -157,6 +157,14 @@ class DeviceGrid:
               self._h, int(nmax), int(nset), float(mu), float(R), cs.ptr, out.ptr, _s(stream))
         return out
 
+    def shs_deriv(self, cs, nmax, mu, R, what, out, stream=None):
+        """Gradient or guru derivative into ``out`` (nout, nlat, nlon):
+        ``what`` is DERIV_GRAD1, DERIV_GRAD2 or 100 dr + 10 dlat + dlon."""
+        _call("charm_amd_shs_grid_deriv_dev",
+              [_vp, _ct.c_ulong, _ct.c_double, _ct.c_double, _vp, _ct.c_int, _vp, _vp],
+              self._h, int(nmax), float(mu), float(R), cs.ptr, int(what), out.ptr, _s(stream))
+        return out
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
@@ -175,6 +183,21 @@ def shs_sctr(lat, lon, r, cs, nmax, mu, R, out, stream=None):
           _s(stream))
     return out
 
+
+def shs_sctr_deriv(lat, lon, r, cs, nmax, mu, R, what, out, stream=None):
+    """Gradient or guru derivative at scattered points into ``out``
+    (nout, npoint)."""
+    _call("charm_amd_shs_sctr_deriv_dev",
+          [_ct.c_size_t, _vp, _vp, _vp, _ct.c_ulong, _ct.c_double, _ct.c_double, _vp, _ct.c_int,
+           _vp, _vp],
+          lat.size, lat.ptr, lon.ptr, r.ptr, int(nmax), float(mu), float(R), cs.ptr, int(what),
+          out.ptr, _s(stream))
+    return out
+
+
+DERIV_GRAD1 = -1
+DERIV_GRAD2 = -2
+DERIV_NOUT = {DERIV_GRAD1: 3, DERIV_GRAD2: 6}
 
 SHS_GRID = 0
 SHA_GRID = 1
@@ -235,6 +258,14 @@ class Batch:
         f.restype = _ct.c_int
         f.argtypes = [_vp]
         return bool(f(self._h))
+
+    def shs_deriv(self, cs, nmax, mu, R, what, out, stream=None):
+        """Gradient or guru derivative into ``out`` (nout, nlat, nlon):
+        ``what`` is DERIV_GRAD1, DERIV_GRAD2 or 100 dr + 10 dlat + dlon."""
+        _call("charm_amd_shs_grid_deriv_dev",
+              [_vp, _ct.c_ulong, _ct.c_double, _ct.c_double, _vp, _ct.c_int, _vp, _vp],
+              self._h, int(nmax), float(mu), float(R), cs.ptr, int(what), out.ptr, _s(stream))
+        return out
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -211,6 +211,23 @@ void charm_amd_shs_sctr_dev(size_t npoint, const double *lat_dev, const double *
                             const double *r_dev, unsigned long nmax, double mu, double R,
                             const double *cs_dev, double *f_dev, void *stream,
                             charm_err *err);
+/* Derivative requests of the device-resident entry points below:
+ * CHARM_AMD_DERIV_GRAD1 (3 outputs, as charm_shs_point_grad1),
+ * CHARM_AMD_DERIV_GRAD2 (6 outputs, as charm_shs_point_grad2), or
+ * 100 dr + 10 dlat + dlon (1 output, as charm_shs_point_guru). */
+#define CHARM_AMD_DERIV_GRAD1 (-1)
+#define CHARM_AMD_DERIV_GRAD2 (-2)
+/* charm_shs_point_grad1/grad2/guru (src/shs/shs.h.in:239-243, 344-348,
+ * 428-435) on a device-resident GL grid; the outputs follow one another in
+ * f_dev, nlat * nlon values each. */
+void charm_amd_shs_grid_deriv_dev(const charm_amd_grid *grd, unsigned long nmax, double mu,
+                                  double R, const double *cs_dev, int what, double *f_dev,
+                                  void *stream, charm_err *err);
+/* The same at npoint scattered points given by device arrays. */
+void charm_amd_shs_sctr_deriv_dev(size_t npoint, const double *lat_dev, const double *lon_dev,
+                                  const double *r_dev, unsigned long nmax, double mu, double R,
+                                  const double *cs_dev, int what, double *f_dev, void *stream,
+                                  charm_err *err);
 
 /* Synthesis of nset coefficient sets on one device-resident GL grid (no
  * counterpart in CHarm, where each set is a separate charm_shs_point call,

@@ -260,3 +260,51 @@ def test_laplace_trace_2160():
     scale = max(np.max(np.abs(t[j])) for j in range(6))
     assert np.max(np.abs(t[0] + t[3] + t[5])) <= TOL * scale
 
+
+
+@pytest.mark.parametrize("radius", ("R", "R+5km"))
+def test_device_grid_deriv_matches_host_api(radius):
+    """charm_amd_shs_grid_deriv_dev gives the host entry points' values
+    bit for bit (same tables, same syntheses, same combination)."""
+    from charm_amd.device import DERIV_GRAD1, DERIV_GRAD2, DERIV_NOUT, DevArray, DeviceGrid, Stream
+
+    N = 200
+    C, S, mu, R = _kaula(N)
+    r0 = R if radius == "R" else R + 5000.0
+    g = crd.PointGridGL(N, r0)
+    m = model(N, C, S, mu, R)
+    dg = DeviceGrid(N, r0)
+    st = Stream()
+    cs = DevArray.from_numpy(np.stack([C, S], axis=1))
+    for what, host in ((DERIV_GRAD1, shs.point_grad1(g, m, N)),
+                       (DERIV_GRAD2, shs.point_grad2(g, m, N)),
+                       (110, (shs.point_guru(g, m, N, 1, 1, 0),)),
+                       (2, (shs.point_guru(g, m, N, 0, 0, 2),))):
+        nout = DERIV_NOUT.get(what, 1)
+        out = DevArray((nout, dg.nlat, dg.nlon))
+        dg.shs_deriv(cs, N, mu, R, what, out, st)
+        st.synchronize()
+        got = out.numpy()
+        for j in range(nout):
+            assert np.array_equal(got[j], host[j]), (what, j)
+
+
+def test_device_sctr_deriv_matches_host_api():
+    from charm_amd.device import DERIV_GRAD2, DevArray, shs_sctr_deriv, synchronize
+
+    N = 150
+    C, S, mu, R = _kaula(N)
+    rng = np.random.default_rng(4)
+    npt = 5000
+    lat = rng.uniform(-1.5, 1.5, npt)
+    lon = rng.uniform(0, 2 * np.pi, npt)
+    r = R + rng.uniform(0, 9000, npt)
+    host = shs.point_grad2(crd.PointSctr(lat, lon, r), model(N, C, S, mu, R), N)
+    cs = DevArray.from_numpy(np.stack([C, S], axis=1))
+    out = DevArray((6, npt))
+    shs_sctr_deriv(DevArray.from_numpy(lat), DevArray.from_numpy(lon), DevArray.from_numpy(r),
+                   cs, N, mu, R, DERIV_GRAD2, out)
+    synchronize()
+    got = out.numpy()
+    for j in range(6):
+        assert np.array_equal(got[j], host[j]), j

@@ -276,6 +276,25 @@ def test_device_grid_matches_host_api():
     assert np.abs(cs_np[:, 1] - S).max() <= TOL * np.abs(C).max()
 
 
+def test_device_grid_radius_differs_from_R():
+    """A device grid whose radius is not R: one column of (R/r)^k for every
+    row, values equal to the host entry point's."""
+    from charm_amd.device import DevArray, DeviceGrid, Stream
+
+    N, Ng = 150, 300
+    C, S = kaula_coeffs(N, 21)
+    r0 = R_EARTH + 7000.0
+    g = crd.PointGridGL(Ng, r0)
+    f_host = shs.point(g, model(N, C, S, GM, R_EARTH), N)
+    dg = DeviceGrid(Ng, r0)
+    st = Stream()
+    cs = DevArray.from_numpy(np.stack([C, S], axis=1))
+    f = DevArray((dg.nlat, dg.nlon))
+    dg.shs(cs, N, GM, R_EARTH, f, st)
+    st.synchronize()
+    assert np.array_equal(f.numpy(), f_host)
+
+
 def test_device_sctr_matches_host_api():
     from charm_amd.device import DevArray, shs_sctr, synchronize
 
