@@ -21,6 +21,9 @@ Workloads (--workload):
           value = (nmax+1)^2 nlat / step time.  Beside it: grad1 and the
           plain synthesis on the same grid, grad1 at 2e5 scattered points,
           and the CPU baseline (the oracle's gradient at scattered points).
+  io      model files of degree 2190 (the serving model): write and read
+          tbl, dov, mtx and bin (charm_shc_write_*/read_*), then upload to
+          HBM; coefficient pairs per second, host-side parsing threads.
   serve   Poisson arrivals against a resident nmax = 2190 model (60% scattered
           SHS of 1e3..1e6 points, 30% GL SHS on grids nmax ~ U{180..5400}
           with the model truncated to the grid, 10% GL SHA nmax ~ U{180..2160});
@@ -298,6 +301,52 @@ def grad_bench(args):
     return line
 
 
+def io_bench(args):
+    import tempfile
+
+    import numpy as np
+
+    from charm_amd import shc
+    from charm_amd.device import DevArray, synchronize
+    from charm_amd.workloads import GM, R_EARTH, kaula_coeffs
+
+    N = 2190
+    C, S = kaula_coeffs(N, 5)
+    model = shc.Shc.from_arrays(N, C, S, GM, R_EARTH)
+    pairs = (N + 1) * (N + 2) // 2
+    res = {}
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        for ft in ("tbl", "dov", "mtx", "bin"):
+            path = os.path.join(d, "model." + ft)
+            t0 = time.perf_counter()
+            model.to_file(ft, path)
+            tw = time.perf_counter() - t0
+            size = os.path.getsize(path)
+            t0 = time.perf_counter()
+            back = shc.Shc.from_file(ft, path)
+            tr = time.perf_counter() - t0
+            assert np.array_equal(back.c, C) and np.array_equal(back.s, S), ft
+            t0 = time.perf_counter()
+            dev = DevArray.from_numpy(np.stack([back.c, back.s], axis=1))
+            synchronize()
+            tu = time.perf_counter() - t0
+            res[ft] = {"bytes": size, "write_s": tw, "read_s": tr, "upload_s": tu,
+                       "read_MBps": size / tr / 1e6, "pairs_per_s": pairs / tr}
+            del dev, back
+            print("# %s: %.1f MB read in %.3f s" % (ft, size / 1e6, tr), file=sys.stderr,
+                  flush=True)
+    best = min(v["read_s"] + v["upload_s"] for k, v in res.items() if k != "bin")
+    line = base_line("coefficient pairs/s, text model file read + upload to HBM (best text "
+                     "format)", pairs / best, 1, 0, 1e3 * best,
+                     {"workload": "degree-%d model (%d coefficient pairs) through tbl/dov/mtx/bin"
+                                  % (N, pairs), "nmax": N, "parallelism": "host threads + 1 GPU"})
+    line["unit"] = "pairs/s"
+    line["dtype"] = "f64"
+    line.update({"formats": res, "host_threads": min(16, os.cpu_count() or 1),
+                 "note": "host-side parsing (strtod) over up to 16 threads; exact round trips"})
+    return line
+
+
 def batch_bench(args):
     import numpy as np
 
@@ -571,7 +620,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="gl2160",
-                    choices=("gl2160", "gl10800", "batch", "multi", "grad", "serve"))
+                    choices=("gl2160", "gl10800", "batch", "multi", "grad", "io", "serve"))
     ap.add_argument("--cpu-rows", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sctr", action="store_true")
@@ -605,6 +654,8 @@ def main():
         line = multi_bench(args)
     elif args.workload == "grad":
         line = grad_bench(args)
+    elif args.workload == "io":
+        line = io_bench(args)
     else:
         line = serve_bench(args)
     print(json.dumps(line), flush=True)

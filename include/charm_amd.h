@@ -17,6 +17,7 @@
 #ifndef CHARM_AMD_H
 #define CHARM_AMD_H
 
+#include <limits.h>
 #include <stddef.h>
 #include <stdbool.h>
 
@@ -97,6 +98,43 @@ charm_shc *charm_shc_calloc(unsigned long nmax, double mu, double r);
 charm_shc *charm_shc_init(unsigned long nmax, double mu, double r, double *c, double *s);
 /* src/shc/shc.h.in:602 */
 void charm_shc_free(charm_shc *shcs);
+
+/* Coefficient files (src/shc/shc.h.in:60-108, 618-1330).  The readers return
+ * the maximum degree stored in the file (CHARM_SHC_NMAX_ERROR on error); with
+ * nmax = CHARM_SHC_NMAX_MODEL and shcs = NULL they only read that degree.
+ * They set shcs->mu, shcs->r and every coefficient (zeros above nmax). */
+#define CHARM_SHC_NMAX_MODEL (ULONG_MAX - 1UL)
+#define CHARM_SHC_NMAX_ERROR (ULONG_MAX)
+/* ordering of charm_shc_write_tbl/dov: degree (N) or order (M) fastest */
+enum { CHARM_SHC_WRITE_N, CHARM_SHC_WRITE_M };
+/* src/shc/shc.h.in:721-724 (binary: nmax, mu, r, C and S by order) */
+unsigned long charm_shc_read_bin(const char *pathname, unsigned long nmax, charm_shc *shcs,
+                                 charm_err *err);
+/* src/shc/shc.h.in:789-793 (ICGEM gfc 1.0/2.0; epoch "yyyyMMdd[.hhmm]" for
+ * time variable models) */
+unsigned long charm_shc_read_gfc(const char *pathname, unsigned long nmax, const char *epoch,
+                                 charm_shc *shcs, charm_err *err);
+/* src/shc/shc.h.in:863-866 */
+unsigned long charm_shc_read_tbl(const char *pathname, unsigned long nmax, charm_shc *shcs,
+                                 charm_err *err);
+/* src/shc/shc.h.in:939-942 */
+unsigned long charm_shc_read_dov(const char *pathname, unsigned long nmax, charm_shc *shcs,
+                                 charm_err *err);
+/* src/shc/shc.h.in:1013-1016 */
+unsigned long charm_shc_read_mtx(const char *pathname, unsigned long nmax, charm_shc *shcs,
+                                 charm_err *err);
+/* src/shc/shc.h.in:1055-1058 */
+void charm_shc_write_bin(const charm_shc *shcs, unsigned long nmax, const char *pathname,
+                         charm_err *err);
+/* src/shc/shc.h.in:1140-1145 (formatting: a printf conversion for doubles) */
+void charm_shc_write_tbl(const charm_shc *shcs, unsigned long nmax, const char *formatting,
+                         int ordering, const char *pathname, charm_err *err);
+/* src/shc/shc.h.in:1233-1238 */
+void charm_shc_write_dov(const charm_shc *shcs, unsigned long nmax, const char *formatting,
+                         int ordering, const char *pathname, charm_err *err);
+/* src/shc/shc.h.in:1292-1296 */
+void charm_shc_write_mtx(const charm_shc *shcs, unsigned long nmax, const char *formatting,
+                         const char *pathname, charm_err *err);
 
 /* ------------------------------------------------------------------------ */
 /* Evaluation points: src/crd/crd.h.in:72-628                               */
