@@ -221,6 +221,41 @@ void charm_sha_point(const charm_point *pnt, const double *f, unsigned long nmax
                      charm_shc *shcs, charm_err *err);
 
 /* ------------------------------------------------------------------------ */
+/* Global spectral gravity forward modelling: src/gfm/gfm.h.in:222-320       */
+/* ------------------------------------------------------------------------ */
+/* src/gfm/gfm.h.in:223-239.  Potential coefficients (mu = G M, r = R) of the
+ * masses between the sphere of radius shape_radius_ref and the shape
+ * (shape_shcs, mu = r = 1), density sum_i rho_i(lat, lon) r^i (density_shcs[i],
+ * mu = r = 1), topography powers shape_power_min..max; optionally exports the
+ * coefficients of each power to <path>-p<p>-i<i>.{txt,shcs}. */
+void charm_gfm_global_density_3d(const charm_shc *shape_shcs, unsigned long shape_nmax,
+                                 double shape_radius_ref, charm_shc **density_shcs,
+                                 unsigned long *density_nmax, unsigned density_order,
+                                 double grav_const, double mass, unsigned shape_power_min,
+                                 unsigned shape_power_max, unsigned long potential_shcs_nmax,
+                                 const char *shape_density_shcs_path,
+                                 const char *potential_shcs_path, const char *shcs_file_format,
+                                 charm_shc *potential_shcs, charm_err *err);
+/* src/gfm/gfm.h.in:255-270 (laterally varying density) */
+void charm_gfm_global_density_lateral(const charm_shc *shape_shcs, unsigned long shape_nmax,
+                                      double shape_radius_ref, charm_shc *density_shcs,
+                                      unsigned long density_nmax, double grav_const, double mass,
+                                      unsigned shape_power_min, unsigned shape_power_max,
+                                      unsigned long potential_shcs_nmax,
+                                      const char *shape_density_shcs_path,
+                                      const char *potential_shcs_path,
+                                      const char *shcs_file_format, charm_shc *potential_shcs,
+                                      charm_err *err);
+/* src/gfm/gfm.h.in:305-319 (constant density) */
+void charm_gfm_global_density_const(const charm_shc *shape_shcs, unsigned long shape_nmax,
+                                    double shape_radius_ref, double density, double grav_const,
+                                    double mass, unsigned shape_power_min,
+                                    unsigned shape_power_max, unsigned long potential_shcs_nmax,
+                                    const char *shape_shcs_path, const char *potential_shcs_path,
+                                    const char *shcs_file_format, charm_shc *potential_shcs,
+                                    charm_err *err);
+
+/* ------------------------------------------------------------------------ */
 /* Device-resident entry points (no counterpart in CHarm)                    */
 /* ------------------------------------------------------------------------ */
 /* A Gauss-Legendre grid whose latitude tables live in device memory, for

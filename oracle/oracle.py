@@ -225,6 +225,69 @@ class Oracle:
         out = self.shs_deriv_sctr(what, nmax, C, S, mu, R, la, lo, ra, exact)
         return out.reshape(out.shape[0], nlat, nlon)
 
+    def gfm_global(self, shape, shape_nmax, R, density, G, M, pmin, pmax, nmax_pot,
+                   exact=False):
+        """Global spectral gravity forward modelling (src/gfm/gfm_sgfm.c with
+        SGFM_GLOBAL, :777-1060).  shape = (C, S) of the shape (mu = r = 1);
+        density = a number (constant density) or a list of (C, S, nmax), the
+        terms rho_i of rho = sum_i rho_i r^i.  Returns (C, S) of the potential
+        to degree nmax_pot (mu = G M, r = R)."""
+        import math
+
+        const = not isinstance(density, (list, tuple))
+        terms = [None] if const else list(density)
+        dmax = 0 if const else max(t[2] for t in terms)
+        ncs = (nmax_pot + 1) * (nmax_pot + 2) // 2
+        if exact:
+            import mpmath
+
+            mp = mpmath.mp
+            mp.prec = 113
+            num = mpmath.mpf
+            PI = mpmath.pi
+        else:
+            num = float
+            PI = math.pi
+        Cp = np.zeros(ncs, dtype=object if exact else np.float64)
+        Sp = np.zeros(ncs, dtype=object if exact else np.float64)
+        R = num(R)
+        c = (2 * PI * R ** 3) / num(M)
+        if const:
+            c = c * num(density)
+        for p in range(pmin, pmax + 1):
+            nmaxp = p * shape_nmax + dmax
+            thr = min(nmaxp, nmax_pot)
+            lat, w, lon = self.gl_grid(nmaxp, exact=exact)
+            ones = np.array([num(1)] * len(lat), dtype=object if exact else np.float64)
+            H = self.shs_grid(shape_nmax, shape[0], shape[1], 1.0, 1.0, lat, ones, lon,
+                              exact=exact)
+            for i, term in enumerate(terms):
+                h = (H - R) / R
+                v = h.copy()
+                for _ in range(2, p + 1):
+                    v = v * h
+                if term is None:
+                    f = v
+                else:
+                    rho = self.shs_grid(term[2], term[0], term[1], 1.0, 1.0, lat, ones, lon,
+                                        exact=exact)
+                    f = rho * (v * R ** i)
+                C, S = self.sha_gl(thr, f, lat, w, 1.0, 1.0, 1.0, exact=exact)
+                pr = num(p)
+                for n in range(thr + 1):
+                    np3 = num(n + i + 3)
+                    snp = np3 / pr
+                    for k in range(1, p):
+                        snp = snp * ((np3 - k) / (pr - k))
+                    snp = snp * (num(2) / ((2 * num(n) + 1) * np3))
+                    vn = c * snp
+                    for m in range(n + 1):
+                        a = m * (thr + 1) - (m * (m - 1)) // 2 + (n - m)
+                        b = m * (nmax_pot + 1) - (m * (m - 1)) // 2 + (n - m)
+                        Cp[b] = Cp[b] + C[a] * vn
+                        Sp[b] = Sp[b] + S[a] * vn
+        return Cp, Sp
+
     def leg_pnm(self, nmax, lat, exact=False):
         ncs = (nmax + 1) * (nmax + 2) // 2
         p = self.empty(ncs)

@@ -242,3 +242,48 @@ def dsun_rows(lat):
     for i in range(n // 2):
         rows[n - 1 - i] = -rows[i]
     return rows
+
+
+def read_tbl(path, exact=False):
+    """(nmax, mu, r, C, S) of a tbl file (src/shc/shc_read_tbl.c): first line
+    nmax mu r, then "n m C [S]" lines."""
+    with open(path) as fh:
+        head = fh.readline().split()
+        rows = [ln.split() for ln in fh if ln.strip()]
+    conv = float
+    if exact:
+        import mpmath
+
+        def conv(s):
+            with mpmath.workprec(113):
+                return +mpmath.mpf(s)
+    nmax = int(head[0])
+    ncs = (nmax + 1) * (nmax + 2) // 2
+    C = np.zeros(ncs, dtype=object if exact else np.float64)
+    S = np.zeros(ncs, dtype=object if exact else np.float64)
+    for r in rows:
+        n, m = int(r[0]), int(r[1])
+        C[tri_index(nmax, n, m)] = conv(r[2])
+        S[tri_index(nmax, n, m)] = conv(r[3]) if len(r) > 3 else conv("0")
+    return nmax, conv(head[1]), conv(head[2]), C, S
+
+
+# tests/parameters.h:298-356: the lunar gravity forward modelling case
+MOON_RREF = 1728000.0
+MOON_MASS = 7.346e22
+GRAV_CONST = 6.67430e-11
+MOON_PMIN, MOON_PMAX = 1, 4
+MOON_NMAX_POT = 10
+MOON_DENSITY_CONST = 2550.0
+
+
+def degree_amplitudes(C, S, nmax):
+    """src/shc/shc_da.c: sqrt(sum_m C_nm^2 + S_nm^2)."""
+    out = []
+    for n in range(nmax + 1):
+        v = 0.0
+        for m in range(n + 1):
+            i = tri_index(nmax, n, m)
+            v += float(C[i]) ** 2 + float(S[i]) ** 2
+        out.append(np.sqrt(v))
+    return np.array(out)

@@ -167,6 +167,38 @@ def test_derivative_argument_errors():
     assert all(x.size == 0 for x in charm_amd.shs.point_grad2(p, s, 3))
 
 
+def test_gfm_argument_errors():
+    """src/gfm/gfm_sgfm.c:400-690: checked before any device work."""
+    s = shc.Shc.from_zeros(4)
+    s.c[0] = 1.1
+    g = charm_amd.gfm.global_density_const
+    for args in ((s, 5, 1.0, 1.0, 1.0, 1.0, 1, 2, 4),      # shape_nmax > shcs nmax
+                 (s, 4, -1.0, 1.0, 1.0, 1.0, 1, 2, 4),     # negative radius
+                 (s, 4, 1.0, 1.0, 1.0, 0.0, 1, 2, 4),      # zero mass
+                 (s, 4, 1.0, 1.0, 1.0, 1.0, 0, 2, 4),      # power 0
+                 (s, 4, 1.0, 1.0, 1.0, 1.0, 3, 2, 4)):     # pmin > pmax
+        with pytest.raises(charm_amd.CHarmError):
+            g(*args)
+    with pytest.raises(charm_amd.CHarmError, match="shcs_file_format"):
+        g(s, 4, 1.0, 1.0, 1.0, 1.0, 1, 2, 4, shape_shcs_path="/tmp/x", shcs_file_format="gfc")
+    with pytest.raises(charm_amd.CHarmError, match="potential_shcs_nmax"):
+        charm_amd.lib()  # potential degree above the output's degree
+        out = shc.Shc.from_zeros(2)
+        import ctypes as ct
+        from charm_amd import _err
+        f = charm_amd.lib().charm_gfm_global_density_const
+        f.restype = None
+        err = _err.init()
+        f.argtypes = [ct.POINTER(shc._Shc), ct.c_ulong, ct.c_double, ct.c_double, ct.c_double,
+                      ct.c_double, ct.c_uint, ct.c_uint, ct.c_ulong, ct.c_char_p, ct.c_char_p,
+                      ct.c_char_p, ct.POINTER(shc._Shc), ct.POINTER(_err._Err)]
+        f(s._Shc, 4, 1.0, 1.0, 1.0, 1.0, 1, 2, 4, None, None, None, out._Shc, err)
+        _err.handler(err, True)
+    bad = shc.Shc.from_zeros(4, 2.0, 1.0)
+    with pytest.raises(charm_amd.CHarmError, match="have to be"):
+        g(bad, 4, 1.0, 1.0, 1.0, 1.0, 1, 2, 4)
+
+
 def test_sha_rejects_scattered_points():
     s = shc.Shc.from_zeros(2)
     p = crd.PointSctr(np.zeros(3), np.zeros(3), np.ones(3))
