@@ -24,6 +24,11 @@ Workloads (--workload):
   io      model files of degree 2190 (the serving model): write and read
           tbl, dov, mtx and bin (charm_shc_write_*/read_*), then upload to
           HBM; coefficient pairs per second, host-side parsing threads.
+  gfm     global gravity forward modelling (charm_gfm_global_density_const):
+          a degree-540 shape, topography powers 1..5 (GL grids of degree
+          540..2700, one SHS + power + SHA each), potential to degree 540;
+          value = sum over powers of 2 (nmaxp+1)^2 nlat / wall time; CPU
+          baseline: the oracle on a degree-360 shape, same powers.
   serve   Poisson arrivals against a resident nmax = 2190 model (60% scattered
           SHS of 1e3..1e6 points, 30% GL SHS on grids nmax ~ U{180..5400}
           with the model truncated to the grid, 10% GL SHA nmax ~ U{180..2160});
@@ -347,6 +352,53 @@ def io_bench(args):
     return line
 
 
+def gfm_bench(args):
+    import numpy as np
+
+    from charm_amd import gfm, shc
+    from charm_amd.workloads import kaula_coeffs
+
+    def shape_model(N, seed):
+        C, S = kaula_coeffs(N, seed)
+        C, S = C * 2.0e5, S * 2.0e5
+        C[0] = 1.74e6
+        return C, S
+
+    def units(N, pmax):
+        return sum(2.0 * float(p * N + 1) ** 2 * (p * N + 1) for p in range(1, pmax + 1))
+
+    N, pmax, R, rho, G, M = 540, 5, 1.73e6, 2500.0, 6.6743e-11, 7.346e22
+    C, S = shape_model(N, 7)
+    shape = shc.Shc.from_arrays(N, C, S)
+    gfm.global_density_const(shape, N, R, rho, G, M, 1, pmax, N)   # warm-up (tables, plans)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pot = gfm.global_density_const(shape, N, R, rho, G, M, 1, pmax, N)
+    dt = (time.perf_counter() - t0) / args.steps
+    line = base_line("(nmax+1)^2*nlat/s FP64 SHS+SHA inside global gravity forward modelling",
+                     units(N, pmax) / dt, args.steps, 1, 1e3 * dt,
+                     {"workload": "charm_gfm_global_density_const, shape nmax=%d, powers 1..%d "
+                                  "(GL grids of degree %d..%d), potential nmax=%d"
+                                  % (N, pmax, N, pmax * N, N),
+                      "nmax": N, "parallelism": "single GPU"})
+    line["c00"] = float(pot.c[0])
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        from oracle.oracle import Oracle
+
+        n2 = 360
+        C2, S2 = shape_model(n2, 7)
+        t0 = time.perf_counter()
+        Oracle("d").gfm_global((C2, S2), n2, R, rho, G, M, 1, pmax, n2)
+        dt2 = time.perf_counter() - t0
+        line["cpu_baseline"] = {
+            "value": units(n2, pmax) / dt2, "unit": UNIT, "cores": threads, "kind": "port",
+            "sample": "the same modelling for a degree-%d shape (grids up to degree %d), "
+                      "oracle SHS/SHA in double plus numpy powers, %.2f s wall" % (n2, pmax * n2, dt2)}
+    return line
+
+
 def batch_bench(args):
     import numpy as np
 
@@ -620,7 +672,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="gl2160",
-                    choices=("gl2160", "gl10800", "batch", "multi", "grad", "io", "serve"))
+                    choices=("gl2160", "gl10800", "batch", "multi", "grad", "io", "gfm",
+                             "serve"))
     ap.add_argument("--cpu-rows", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sctr", action="store_true")
@@ -656,6 +709,8 @@ def main():
         line = grad_bench(args)
     elif args.workload == "io":
         line = io_bench(args)
+    elif args.workload == "gfm":
+        line = gfm_bench(args)
     else:
         line = serve_bench(args)
     print(json.dumps(line), flush=True)
