@@ -27,7 +27,7 @@ Workloads (--workload):
   gfm     global gravity forward modelling (charm_gfm_global_density_const):
           a degree-540 shape, topography powers 1..5 (GL grids of degree
           540..2700, one SHS + power + SHA each), potential to degree 540;
-          value = sum over powers of 2 (nmaxp+1)^2 nlat / wall time; CPU
+          value = sum over powers of 2 (540+1)^2 nlat_p / wall time; CPU
           baseline: the oracle on a degree-360 shape, same powers.
   serve   Poisson arrivals against a resident nmax = 2190 model (60% scattered
           SHS of 1e3..1e6 points, 30% GL SHS on grids nmax ~ U{180..5400}
@@ -365,7 +365,9 @@ def gfm_bench(args):
         return C, S
 
     def units(N, pmax):
-        return sum(2.0 * float(p * N + 1) ** 2 * (p * N + 1) for p in range(1, pmax + 1))
+        # per power: SHS of the degree-N shape and SHA to degree N on the
+        # (pN+1)-ring grid
+        return sum(2.0 * float(N + 1) ** 2 * (p * N + 1) for p in range(1, pmax + 1))
 
     N, pmax, R, rho, G, M = 540, 5, 1.73e6, 2500.0, 6.6743e-11, 7.346e22
     C, S = shape_model(N, 7)

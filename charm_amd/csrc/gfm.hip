@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <memory>
+#include <mutex>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -156,6 +157,47 @@ void export_powers(const std::vector<double> &cs, unsigned long nmax, double mu,
     if (!ok) throw Fail{CHARM_EFILEIO, msg};
 }
 
+// Gauss--Legendre grids of radius 1 kept on the device across calls (a
+// modelling run visits one grid per topography power; repeated runs reuse
+// them).  At most GRID_CACHE grids, the least recently used is dropped.
+constexpr size_t GRID_CACHE = 8;
+struct CachedGrid {
+    unsigned long nmax;
+    charm_amd_grid *grd;
+    size_t nlat, nlon;
+    unsigned long use;
+};
+std::mutex g_grid_mu;
+std::vector<CachedGrid> g_grids;
+unsigned long g_grid_clock = 0;
+
+// The cached grid of degree nmax (created if needed).  Callers hold
+// g_grid_mu for the whole modelling run, so no grid in use is dropped.
+CachedGrid unit_gl_grid(unsigned long nmax, charm_err *e)
+{
+    for (auto &g : g_grids)
+        if (g.nmax == nmax) {
+            g.use = ++g_grid_clock;
+            return g;
+        }
+    charm_point *gl = charm_crd_point_gl(nmax, 1.0);
+    if (!gl) throw Fail{CHARM_EFUNCARG, "Failed to compute the Gauss--Legendre grid."};
+    charm_amd_grid *grd = charm_amd_grid_create(gl, e);
+    const size_t nlat = gl->nlat, nlon = gl->nlon;
+    charm_crd_point_free(gl);
+    err_check(e, "grid");
+    if (g_grids.size() >= GRID_CACHE) {
+        auto old = std::min_element(g_grids.begin(), g_grids.end(),
+                                    [](const CachedGrid &a, const CachedGrid &b) {
+                                        return a.use < b.use;
+                                    });
+        charm_amd_grid_free(old->grd);
+        g_grids.erase(old);
+    }
+    g_grids.push_back(CachedGrid{nmax, grd, nlat, nlon, ++g_grid_clock});
+    return g_grids.back();
+}
+
 bool is_one(double x)
 {
     return nearly_equal(x, 1.0, charm_glob_threshold);
@@ -250,9 +292,9 @@ void sgfm_global(const charm_shc *shape_shcs, unsigned long shape_nmax, double R
     if (rho_is_const) c *= rho_const;
     const double gm = G * M;
 
+    // one modelling run at a time per process: the runs share the grid cache
+    std::lock_guard<std::mutex> run_lock(g_grid_mu);
     hipStream_t st = nullptr;
-    charm_amd_grid *grd = nullptr;
-    charm_point *gl = nullptr;
     charm_err *e = charm_err_init();
     try {
         hchk(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreateWithFlags");
@@ -272,17 +314,18 @@ void sgfm_global(const charm_shc *shape_shcs, unsigned long shape_nmax, double R
         Dev dpot(2 * npot_pairs * sizeof(double));
         hchk(hipMemsetAsync(dpot.p, 0, 2 * npot_pairs * sizeof(double), st), "hipMemsetAsync");
         std::vector<double> vnm, hcs;
+        // buffers for the largest grid (the last power), reused by every power
+        const unsigned long nmaxp_top = (unsigned long)pmax * shape_nmax + dens_nmax_max;
+        const unsigned long thr_top = std::min(nmaxp_top, nmax_pot);
+        const size_t npt_top = (size_t)(nmaxp_top + 1) * (2 * nmaxp_top + 2);
+        Dev dsh(npt_top * sizeof(double)), dfield(npt_top * sizeof(double));
+        Dev dcs(2 * tri_len(thr_top) * sizeof(double)), dv((thr_top + 1) * sizeof(double));
         for (unsigned p = pmin; p <= pmax; p++) {
             const unsigned long nmaxp = (unsigned long)p * shape_nmax + dens_nmax_max;
             const unsigned long thr = std::min(nmaxp, nmax_pot);
-            gl = charm_crd_point_gl(nmaxp, 1.0);
-            if (!gl) throw Fail{CHARM_EFUNCARG, "Failed to compute the Gauss--Legendre grid."};
-            hchk(hipStreamSynchronize(st), "hipStreamSynchronize");
-            grd = charm_amd_grid_create(gl, e);
-            err_check(e, "grid");
-            const size_t npt = gl->nlat * gl->nlon;
-            Dev dsh(npt * sizeof(double)), dfield(npt * sizeof(double));
-            Dev dcs(2 * tri_len(thr) * sizeof(double)), dv((thr + 1) * sizeof(double));
+            const CachedGrid cg = unit_gl_grid(nmaxp, e);
+            charm_amd_grid *grd = cg.grd;
+            const size_t npt = cg.nlat * cg.nlon;
             charm_amd_shs_grid_dev(grd, shape_nmax, 1.0, 1.0, dshape.d(), dsh.d(), st, e);
             err_check(e, "shape synthesis");
             double rpow = 1.0;
@@ -331,11 +374,6 @@ void sgfm_global(const charm_shc *shape_shcs, unsigned long shape_nmax, double R
                     hchk(hipStreamSynchronize(st), "hipStreamSynchronize");
                 }
             }
-            hchk(hipStreamSynchronize(st), "hipStreamSynchronize");
-            charm_amd_grid_free(grd);
-            grd = nullptr;
-            charm_crd_point_free(gl);
-            gl = nullptr;
         }
         if (pot) {
             std::vector<double> hp(2 * npot_pairs);
@@ -354,8 +392,6 @@ void sgfm_global(const charm_shc *shape_shcs, unsigned long shape_nmax, double R
     } catch (const Fail &f) {
         CHARM_AMD_ERR(err, f.code, f.msg.c_str());
     }
-    if (grd) charm_amd_grid_free(grd);
-    if (gl) charm_crd_point_free(gl);
     if (st) {
         (void)hipStreamSynchronize(st);
         (void)hipStreamDestroy(st);
