@@ -349,6 +349,142 @@ __global__ void twiddle_table_kernel(int M, cplx *tw)
     tw[q] = fft::mk(c, -s);
 }
 
+// ---------------------------------------------------------------------------
+// Rows longer than one LDS transform: L = L1 L2 with a short factor L1 (one
+// of the codelet sizes) and L2 <= MMAX with an LDS plan, as the four-step
+// algorithm: with n = n1 + L1 n2 and k = k2 + L2 k1,
+//   Z_k = sum_n1 W_L1^(n1 k1) [ W_L^(n1 k2) sum_n2 z_(n1 + L1 n2) W_L2^(n2 k2) ].
+// Pass A (one work-group per row and n1) gathers the stride-L1 subsequence,
+// runs the length-L2 LDS transform and applies the twiddle; pass B (one
+// thread per row and k2) runs the L1-point DFTs and the real/complex packing.
+// ---------------------------------------------------------------------------
+constexpr int SPLIT_R[] = {2, 3, 4, 5, 7, 8, 9, 16};
+
+struct SplitArgs {
+    RowArgs sub;       // the length-L2 plan
+    int L, L1, L2;
+    const cplx *ptf;   // [L+1] exp(-i pi k / L)
+    const cplx *twf;   // [L]   exp(-2 pi i j / L)
+    cplx *Y;           // [row][n1][k2]
+    double *f;
+    size_t ldf;
+    cplx *spec;
+    size_t lds;
+};
+
+// Input n of the inverse transform (c2r_input for the full row length)
+__device__ __forceinline__ cplx c2r_input_full(int L, const cplx *pt, const cplx *X, int n)
+{
+    cplx xk = X[n], xr = X[L - n];
+    if (n == 0) {
+        xk.y = 0.0;
+        xr.y = 0.0;
+    }
+    const cplx e = fft::scale(fft::add(xk, fft::conj(xr)), 0.5);
+    const cplx o = fft::mul(fft::scale(fft::sub(xk, fft::conj(xr)), 0.5), fft::conj(pt[n]));
+    return fft::conj(fft::add(e, fft::rot90<+1>(o)));
+}
+
+template <int MODE, bool C2R>
+__global__ __launch_bounds__(1024) void split_a_kernel(SplitArgs a)
+{
+    cplx *x = lds_rows;
+    const size_t row = blockIdx.x;
+    const int n1 = blockIdx.y;
+    const int L = a.L, L1 = a.L1, L2 = a.L2;
+    const fft::TwSplit tw = lds_twiddles(a.sub.tw, a.sub.plan.M);
+    const cplx *X = a.spec + row * a.lds;
+    const cplx *z = (const cplx *)(a.f + row * a.ldf);
+    auto get = [&](int n2) {
+        const int n = n1 + L1 * n2;
+        return C2R ? c2r_input_full(L, a.ptf, X, n) : z[n];
+    };
+    row_dft<MODE>(a.sub, x, tw, get);
+    const cplx v0 = (MODE == fft::DFT_RADER) ? get(0) : fft::mk(0.0, 0.0);
+    cplx *y = a.Y + (row * L1 + n1) * (size_t)L2;
+    for (int k2 = threadIdx.x; k2 < L2; k2 += blockDim.x) {
+        const int j = (int)(((long)n1 * k2) % L);
+        y[k2] = fft::mul(row_out<MODE>(a.sub, x, k2, v0), a.twf[j]);
+    }
+}
+
+// The L1-point DFT of column k2 of a row of Y
+template <int R1>
+__device__ __forceinline__ void split_column(const SplitArgs &a, size_t row, int k2, cplx *v)
+{
+    const cplx *y = a.Y + row * (size_t)R1 * a.L2 + k2;
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++) v[n1] = y[(size_t)n1 * a.L2];
+    fft::dft<R1, -1>(v);
+}
+
+template <int R1, bool C2R>
+__global__ __launch_bounds__(256) void split_b_kernel(SplitArgs a)
+{
+    const int k2 = blockIdx.y * blockDim.x + threadIdx.x;
+    const size_t row = blockIdx.x;
+    const int L = a.L, L2 = a.L2;
+    if (k2 >= L2) return;
+    cplx v[R1];
+    split_column<R1>(a, row, k2, v);
+    if (C2R) {
+        cplx *f = (cplx *)(a.f + row * a.ldf);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; k1++) f[k2 + L2 * k1] = fft::mk(2.0 * v[k1].x, -2.0 * v[k1].y);
+        return;
+    }
+    // Z_(L-k) for k = k2 + L2 k1 lies in column (L2 - k2) mod L2, at row
+    // L1 - 1 - k1 (k2 > 0) or (L1 - k1) mod L1 (k2 = 0)
+    cplx w[R1];
+    const int k2p = k2 ? L2 - k2 : 0;
+    if (k2p == k2) {
+#pragma unroll
+        for (int i = 0; i < R1; i++) w[i] = v[i];
+    } else {
+        split_column<R1>(a, row, k2p, w);
+    }
+    cplx *out = a.spec + row * a.lds;
+#pragma unroll
+    for (int k1 = 0; k1 < R1; k1++) {
+        const int k = k2 + L2 * k1;
+        const int k1r = k2 ? R1 - 1 - k1 : (R1 - k1) % R1;
+        const cplx zk = v[k1], zr = fft::conj(w[k1r]);
+        const cplx e = fft::scale(fft::add(zk, zr), 0.5);
+        const cplx o = fft::rot90<-1>(fft::scale(fft::sub(zk, zr), 0.5));
+        out[k] = fft::add(e, fft::mul(a.ptf[k], o));
+        if (k == 0) out[L] = fft::add(e, fft::mul(a.ptf[L], o));
+    }
+}
+
+__global__ void full_tables_kernel(int L, cplx *pt, cplx *tw)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n <= L) {
+        double s, c;
+        sincos(3.14159265358979323846 * (double)n / (double)L, &s, &c);
+        if (2 * n == L) {
+            c = 0.0;
+            s = 1.0;
+        }
+        pt[n] = fft::mk(c, -s);
+    }
+    if (n < L) {
+        double s, c;
+        sincos(2.0 * 3.14159265358979323846 * (double)n / (double)L, &s, &c);
+        if (4 * n == L) {
+            c = 0.0;
+            s = 1.0;
+        } else if (2 * n == L) {
+            c = -1.0;
+            s = 0.0;
+        } else if (4 * n == 3 * L) {
+            c = 0.0;
+            s = -1.0;
+        }
+        tw[n] = fft::mk(c, -s);
+    }
+}
+
 struct Plan {
     fft::MixPlan mp;
     int L = 0;
@@ -392,7 +528,13 @@ hipError_t set_lds_limits()
                          (const void *)c2r_kernel<fft::DFT_DIRECT>,
                          (const void *)c2r_kernel<fft::DFT_BLUESTEIN>,
                          (const void *)c2r_kernel<fft::DFT_RADER>,
-                         (const void *)filter_kernel};
+                         (const void *)filter_kernel,
+                         (const void *)split_a_kernel<fft::DFT_DIRECT, false>,
+                         (const void *)split_a_kernel<fft::DFT_BLUESTEIN, false>,
+                         (const void *)split_a_kernel<fft::DFT_RADER, false>,
+                         (const void *)split_a_kernel<fft::DFT_DIRECT, true>,
+                         (const void *)split_a_kernel<fft::DFT_BLUESTEIN, true>,
+                         (const void *)split_a_kernel<fft::DFT_RADER, true>};
     for (const void *fn : fns) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e != hipSuccess) return e;
@@ -496,22 +638,137 @@ bool plan_exists(int L)
     return ok;
 }
 
+// The short factor of a split row length (0: none)
+int split_factor(int L)
+{
+    static std::mutex mu;
+    static std::map<int, int> known;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = known.find(L);
+    if (it != known.end()) return it->second;
+    int best = 0;
+    for (int r : SPLIT_R) {
+        if (L % r || L / r > fft::MMAX) continue;
+        fft::MixPlan mp;
+        if (fft::choose_plan(L / r, mp)) {
+            best = r;
+            break;
+        }
+    }
+    known[L] = best;
+    return best;
+}
+
+struct SplitPlan {
+    int L1 = 0;
+    cplx *pt = nullptr, *tw = nullptr;
+};
+std::map<int, SplitPlan> g_split;
+
+int get_split(int L, hipStream_t s, SplitPlan *out)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_fft_mu);
+        auto it = g_split.find(L);
+        if (it != g_split.end()) {
+            *out = it->second;
+            return 0;
+        }
+    }
+    SplitPlan sp;
+    sp.L1 = split_factor(L);
+    if (!sp.L1) return (int)hipErrorInvalidValue;
+    hipError_t e;
+    if ((e = hipMalloc((void **)&sp.pt, (L + 1) * sizeof(cplx))) != hipSuccess) return (int)e;
+    if ((e = hipMalloc((void **)&sp.tw, L * sizeof(cplx))) != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(full_tables_kernel, dim3((L + 1 + 255) / 256), dim3(256), 0, s, L, sp.pt,
+                       sp.tw);
+    if ((e = hipGetLastError()) != hipSuccess) return (int)e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return (int)e;
+    std::lock_guard<std::mutex> lk(g_fft_mu);
+    g_split[L] = sp;
+    *out = sp;
+    return 0;
+}
+
+bool single_row_plan(size_t nlon)
+{
+    return nlon / 2 <= (size_t)fft::MMAX && plan_exists((int)(nlon / 2));
+}
+
 }  // namespace
 
 bool row_fft_supported(size_t nlon)
 {
-    if (nlon < 2 || nlon % 2 != 0 || nlon / 2 > (size_t)fft::MMAX) return false;
-    return plan_exists((int)(nlon / 2));
+    if (nlon < 2 || nlon % 2 != 0 || nlon / 2 > (size_t)INT32_MAX / 16) return false;
+    return single_row_plan(nlon) || split_factor((int)(nlon / 2)) != 0;
+}
+
+size_t row_fft_scratch(size_t nlon, size_t nrow)
+{
+    if (!row_fft_supported(nlon) || single_row_plan(nlon)) return 0;
+    return nrow * (nlon / 2) * sizeof(cplx);
 }
 
 int launch_row_fft(int c2r, size_t nlon, size_t nrow, double *f, size_t ldf, void *spec, size_t lds,
-                   void *stream)
+                   void *scratch, void *stream)
 {
     if (!row_fft_supported(nlon)) return (int)hipErrorInvalidValue;
     if (nrow == 0) return 0;
     hipStream_t s = (hipStream_t)stream;
     Plan p;
     const int L = (int)(nlon / 2);
+    if (!single_row_plan(nlon)) {
+        if (!scratch) return (int)hipErrorInvalidValue;
+        SplitPlan sp;
+        if (int e = get_split(L, s, &sp)) return e;
+        const int L2 = L / sp.L1;
+        if (int e = get_plan(L2, s, &p)) return e;
+        SplitArgs a{};
+        a.sub = row_args(p);
+        a.L = L;
+        a.L1 = sp.L1;
+        a.L2 = L2;
+        a.ptf = sp.pt;
+        a.twf = sp.tw;
+        a.Y = (cplx *)scratch;
+        a.f = f;
+        a.ldf = ldf;
+        a.spec = (cplx *)spec;
+        a.lds = lds;
+        const size_t sh = lds_bytes(p.mp.M);
+        const dim3 ga((unsigned)nrow, (unsigned)sp.L1), ba((unsigned)p.threads);
+#define SPLIT_A(C2R)                                                                               \
+    switch (p.mp.mode) {                                                                           \
+    case fft::DFT_DIRECT:                                                                          \
+        hipLaunchKernelGGL((split_a_kernel<fft::DFT_DIRECT, C2R>), ga, ba, sh, s, a);              \
+        break;                                                                                     \
+    case fft::DFT_BLUESTEIN:                                                                       \
+        hipLaunchKernelGGL((split_a_kernel<fft::DFT_BLUESTEIN, C2R>), ga, ba, sh, s, a);           \
+        break;                                                                                     \
+    default: hipLaunchKernelGGL((split_a_kernel<fft::DFT_RADER, C2R>), ga, ba, sh, s, a); break;   \
+    }
+        if (c2r) {
+            SPLIT_A(true)
+        } else {
+            SPLIT_A(false)
+        }
+#undef SPLIT_A
+        const dim3 gb((unsigned)nrow, (unsigned)((L2 + 255) / 256)), bb(256);
+#define SPLIT_B(R1)                                                                                \
+    case R1:                                                                                       \
+        if (c2r)                                                                                   \
+            hipLaunchKernelGGL((split_b_kernel<R1, true>), gb, bb, 0, s, a);                       \
+        else                                                                                       \
+            hipLaunchKernelGGL((split_b_kernel<R1, false>), gb, bb, 0, s, a);                      \
+        break;
+        switch (sp.L1) {
+            SPLIT_B(2) SPLIT_B(3) SPLIT_B(4) SPLIT_B(5) SPLIT_B(7) SPLIT_B(8) SPLIT_B(9) SPLIT_B(16)
+        default: return (int)hipErrorInvalidValue;
+        }
+#undef SPLIT_B
+        return (int)hipGetLastError();
+    }
     if (int e = get_plan(L, s, &p)) return e;
     RowArgs a = row_args(p);
     a.f = f;
@@ -544,7 +801,11 @@ int prepare_row_fft(size_t nlon, void *stream)
 {
     if (!row_fft_supported(nlon)) return 0;
     Plan p;
-    return get_plan((int)(nlon / 2), (hipStream_t)stream, &p);
+    const int L = (int)(nlon / 2);
+    if (single_row_plan(nlon)) return get_plan(L, (hipStream_t)stream, &p);
+    SplitPlan sp;
+    if (int e = get_split(L, (hipStream_t)stream, &sp)) return e;
+    return get_plan(L / sp.L1, (hipStream_t)stream, &p);
 }
 
 }  // namespace charm_amd

@@ -215,8 +215,11 @@ int launch_all_near(size_t n, const double *x_dev, double ref, double eps, int *
 // Longitude transforms (fft.hip): rows of n reals <-> n/2+1 complex values
 // (hipFFT conventions, unnormalised); n even and at most 8192.
 bool row_fft_supported(size_t n);
+// device scratch a transform of nrow rows needs (rows longer than one LDS
+// transform are split in two passes through this buffer); 0 if none
+size_t row_fft_scratch(size_t n, size_t nrow);
 int launch_row_fft(int c2r, size_t n, size_t nrow, double *f, size_t ldf, void *spec, size_t lds,
-                   void *stream);
+                   void *scratch, void *stream);
 int prepare_row_fft(size_t n, void *stream);
 
 // Polar optimisation threshold for a transform of degree nmax from the

@@ -317,11 +317,14 @@ hipfftHandle fft_plan(int kind, size_t n, size_t batch, size_t ldc, hipStream_t 
 }
 
 // Row transforms of nrow rows of n reals <-> n/2+1 complex values: the LDS
-// Bluestein kernels for even n up to 8192, hipFFT otherwise.
+// kernels (one pass up to 2 x 9400 points, two passes for longer rows with a
+// short factor), hipFFT otherwise.
 void rows_c2r(size_t n, size_t nrow, const void *spec, size_t ldc, double *f, hipStream_t s)
 {
     if (row_fft_supported(n)) {
-        HIPCHK((hipError_t)launch_row_fft(1, n, nrow, f, n, const_cast<void *>(spec), ldc, s));
+        DevBuf scratch(row_fft_scratch(n, nrow), s);
+        HIPCHK((hipError_t)launch_row_fft(1, n, nrow, f, n, const_cast<void *>(spec), ldc,
+                                          scratch.p, s));
     } else {
         hipfftHandle h = fft_plan(0, n, nrow, ldc, s);
         FFTCHK(hipfftExecZ2D(h, (hipfftDoubleComplex *)spec, f));
@@ -331,7 +334,9 @@ void rows_c2r(size_t n, size_t nrow, const void *spec, size_t ldc, double *f, hi
 void rows_r2c(size_t n, size_t nrow, const double *f, void *spec, size_t ldc, hipStream_t s)
 {
     if (row_fft_supported(n)) {
-        HIPCHK((hipError_t)launch_row_fft(0, n, nrow, const_cast<double *>(f), n, spec, ldc, s));
+        DevBuf scratch(row_fft_scratch(n, nrow), s);
+        HIPCHK((hipError_t)launch_row_fft(0, n, nrow, const_cast<double *>(f), n, spec, ldc,
+                                          scratch.p, s));
     } else {
         hipfftHandle h = fft_plan(1, n, nrow, ldc, s);
         FFTCHK(hipfftExecD2Z(h, (hipfftDoubleReal *)f, (hipfftDoubleComplex *)spec));

@@ -170,9 +170,109 @@ std::vector<cplx> row_dft(const HostPlan &p, const std::vector<cplx> &z)
     return Z;
 }
 
+// Short factor of a row too long for one LDS transform, in the kernels'
+// order of preference (0: none)
+int split_factor(int L)
+{
+    static const int rs[] = {2, 3, 4, 5, 7, 8, 9, 16};
+    for (int r : rs) {
+        if (L % r || L / r > MMAX) continue;
+        MixPlan mp;
+        if (choose_plan(L / r, mp)) return r;
+    }
+    return 0;
+}
+
+cplx w_full(long j, int L)
+{
+    const double a = 2.0 * 3.14159265358979323846 * (double)j / (double)L;
+    return mk(std::cos(a), -std::sin(a));
+}
+
+// Z = DFT_L(z) by the two passes of the split kernels: pass A per n1 (the
+// stride-L1 subsequence through the length-L2 plan, twiddle W_L^(n1 k2)),
+// pass B per k2 (the L1-point DFT, output k = k2 + L2 k1)
+std::vector<cplx> split_dft(int L, int L1, const std::vector<cplx> &z)
+{
+    const int L2 = L / L1;
+    const HostPlan p = make_plan(L2);
+    std::vector<cplx> Y((size_t)L1 * L2), Z(L);
+    for (int n1 = 0; n1 < L1; n1++) {
+        std::vector<cplx> sub(L2);
+        for (int n2 = 0; n2 < L2; n2++) sub[n2] = z[n1 + L1 * n2];
+        const std::vector<cplx> S = row_dft(p, sub);
+        for (int k2 = 0; k2 < L2; k2++) Y[(size_t)n1 * L2 + k2] = mul(S[k2], w_full((long)n1 * k2 % L, L));
+    }
+    for (int k2 = 0; k2 < L2; k2++)
+        for (int k1 = 0; k1 < L1; k1++) {
+            cplx acc = mk(0, 0);
+            for (int n1 = 0; n1 < L1; n1++)
+                acc = add(acc, mul(Y[(size_t)n1 * L2 + k2], w_full((long)n1 * k1 * L2 % L, L)));
+            Z[k2 + L2 * k1] = acc;
+        }
+    return Z;
+}
+
 }  // namespace
 
 extern "C" {
+
+// the short factor the split kernels use for a row of L complex points
+int host_split_factor(int L)
+{
+    return split_factor(L);
+}
+
+// r2c / c2r of rows of 2L reals through split_dft (L with a short factor)
+void host_r2c_split(int L, int nrow, const double *f, double *spec)
+{
+    const int L1 = split_factor(L);
+    if (!L1) return;
+    const double PI = 3.14159265358979323846;
+    for (int row = 0; row < nrow; row++) {
+        const double *zr0 = f + (size_t)row * 2 * L;
+        std::vector<cplx> z(L);
+        for (int n = 0; n < L; n++) z[n] = mk(zr0[2 * n], zr0[2 * n + 1]);
+        const std::vector<cplx> Z = split_dft(L, L1, z);
+        for (int k = 0; k <= L; k++) {
+            const int kk = (k == L) ? 0 : k, kr = (k == 0) ? 0 : L - k;
+            const cplx zk = Z[kk], zr = conj(Z[kr]);
+            const cplx e = scale(add(zk, zr), 0.5);
+            const cplx o = rot90<-1>(scale(sub(zk, zr), 0.5));
+            const cplx pt = mk(std::cos(PI * k / L), -std::sin(PI * k / L));
+            const cplx F = add(e, mul(pt, o));
+            spec[((size_t)row * (L + 1) + k) * 2] = F.x;
+            spec[((size_t)row * (L + 1) + k) * 2 + 1] = F.y;
+        }
+    }
+}
+
+void host_c2r_split(int L, int nrow, const double *spec, double *f)
+{
+    const int L1 = split_factor(L);
+    if (!L1) return;
+    const double PI = 3.14159265358979323846;
+    for (int row = 0; row < nrow; row++) {
+        const double *X = spec + (size_t)row * (L + 1) * 2;
+        std::vector<cplx> v(L);
+        for (int n = 0; n < L; n++) {
+            cplx xk = mk(X[2 * n], X[2 * n + 1]), xr = mk(X[2 * (L - n)], X[2 * (L - n) + 1]);
+            if (n == 0) {
+                xk.y = 0.0;
+                xr.y = 0.0;
+            }
+            const cplx pt = mk(std::cos(PI * n / L), -std::sin(PI * n / L));
+            const cplx e = scale(add(xk, conj(xr)), 0.5);
+            const cplx o = mul(scale(sub(xk, conj(xr)), 0.5), conj(pt));
+            v[n] = conj(add(e, rot90<+1>(o)));
+        }
+        const std::vector<cplx> Y = split_dft(L, L1, v);
+        for (int n = 0; n < L; n++) {
+            f[(size_t)row * 2 * L + 2 * n] = 2.0 * Y[n].x;
+            f[(size_t)row * 2 * L + 2 * n + 1] = -2.0 * Y[n].y;
+        }
+    }
+}
 
 // plan of length L: M, pass count, radices, mode; returns 0 if none
 int host_plan(int L, int *M, int *np, int *R, int *mode)

@@ -27,6 +27,9 @@ def fftlib(tmp_path_factory):
     lib.host_c2r.argtypes = [ct.c_int, ct.c_int, dp, dp]
     ip = ct.POINTER(ct.c_int)
     lib.host_plan.argtypes = [ct.c_int, ip, ip, ip, ip]
+    lib.host_r2c_split.argtypes = [ct.c_int, ct.c_int, dp, dp]
+    lib.host_c2r_split.argtypes = [ct.c_int, ct.c_int, dp, dp]
+    lib.host_split_factor.argtypes = [ct.c_int]
     return lib
 
 
@@ -114,3 +117,37 @@ def test_row_roundtrip_single_harmonic(fftlib):
     expect = np.zeros(L + 1, complex)
     expect[m] = L
     assert np.abs(F - expect).max() <= 1e-12 * L
+
+
+# Rows longer than one LDS transform: L = L1 L2 through two passes
+# (10801 = 7 x 1543 is the nmax = 10800 GL row; 10000 = 2 x 5000; 12285 =
+# 3 x 4095 with a Bluestein L2; 18000 = 2 x 9000)
+SPLIT = [(10801, 7), (10000, 2), (12285, 3), (18000, 2)]
+
+
+@pytest.mark.parametrize("L,L1", SPLIT)
+def test_split_r2c_c2r_match_numpy(fftlib, L, L1):
+    assert fftlib.host_split_factor(L) == L1
+    rng = np.random.default_rng(L)
+    f = rng.standard_normal((2, 2 * L))
+    spec = np.zeros((2, L + 1, 2))
+    fftlib.host_r2c_split(L, 2, _p(f), _p(spec))
+    ref = np.fft.rfft(f, axis=1)
+    got = spec[..., 0] + 1j * spec[..., 1]
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+    X = rng.standard_normal((2, L + 1)) + 1j * rng.standard_normal((2, L + 1))
+    Xi = np.ascontiguousarray(np.stack([X.real, X.imag], -1))
+    out = np.zeros((2, 2 * L))
+    fftlib.host_c2r_split(L, 2, _p(Xi), _p(out))
+    ref = np.fft.irfft(X, n=2 * L, axis=1) * (2 * L)
+    assert np.abs(out - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_split_factor_choice(fftlib):
+    """No split below the LDS row limit or for a prime; the smallest codelet
+    size whose cofactor has an LDS plan otherwise."""
+    assert fftlib.host_split_factor(10007) == 0          # prime
+    assert fftlib.host_split_factor(2 * 9403) == 0       # 9403 prime, > 9400
+    assert fftlib.host_split_factor(9 * 9397) == 0       # 9397 prime, no LDS plan
+    assert fftlib.host_split_factor(2 ** 17) == 16       # 8192 per pass
+    assert fftlib.host_split_factor(27000) == 3          # 13500 > 9400, 9000 direct

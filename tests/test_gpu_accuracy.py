@@ -105,7 +105,13 @@ def test_c3_nmax10800_sampled_rings(orcd, orcq):
     grid.shs(cs, N, GM, R_EARTH, f, st)
     st.synchronize()
     fn = f.numpy()
-    del f
+    # round trip through the analysis (rows of 21602 points: two-pass r2c)
+    out = DevArray(cs.shape)
+    grid.sha(f, N, GM, R_EARTH, out, st)
+    st.synchronize()
+    o = out.numpy()
+    rt = float(max(np.abs(o[:, 0] - C).max(), np.abs(o[:, 1] - S).max()) / np.abs(C).max())
+    del f, out
     p = grid.pnt
     rings = np.array([1, 1543, 5400, 10800], dtype=np.int64)
     fd = orcd.shs_grid(N, C, S, GM, R_EARTH, p.lat, p.r, p.lon, gl=True, rings=rings)
@@ -115,6 +121,6 @@ def test_c3_nmax10800_sampled_rings(orcd, orcq):
     jl = np.linspace(0, grid.nlon - 1, 8).round().astype(np.int64)
     fq = orcq.shs_grid(N, C, S, GM, R_EARTH, p.lat, p.r, p.lon[jl], gl=True, rings=qr)
     e_q = relmax(fn[np.ix_(qr, jl)], fq)
-    print("C3 N=10800 GL: rings %s vs double %.2e; rings %s x 8 longitudes vs quad %.2e"
-          % (rings.tolist(), e_d, qr.tolist(), e_q))
-    assert max(e_d, e_q) <= TOL
+    print("C3 N=10800 GL: rings %s vs double %.2e; rings %s x 8 longitudes vs quad %.2e; "
+          "round trip %.2e" % (rings.tolist(), e_d, qr.tolist(), e_q, rt))
+    assert max(e_d, e_q, rt) <= TOL

@@ -295,6 +295,35 @@ def test_device_grid_radius_differs_from_R():
     assert np.array_equal(f.numpy(), f_host)
 
 
+@pytest.mark.parametrize("nlon", (20000, 21602, 24570))
+def test_split_row_lengths_synthesis(orc, nlon):
+    """Rows longer than one LDS transform (L = 10000 = 2 x 5000, 10801 =
+    7 x 1543, 12285 = 3 x 4095): the two-pass c2r against the oracle's direct
+    longitude sums."""
+    N = 60
+    C, S = kaula_coeffs(N, 17)
+    lat = np.array([1.2, 0.4, -0.05, -0.9])
+    lon = np.arange(nlon) * (2.0 * np.pi / nlon)
+    g = crd.PointGrid(lat, lon, np.full(lat.size, R_EARTH))
+    f = shs.point(g, model(N, C, S, GM, R_EARTH), N)
+    ref = orc.shs_grid(N, C, S, GM, R_EARTH, lat, np.full(lat.size, R_EARTH), lon, gl=False)
+    assert relmax(f, ref) <= TOL, relmax(f, ref)
+
+
+def test_split_row_lengths_roundtrip():
+    """DH2 grid of nmax 4999 (10000 x 20000): two-pass c2r and r2c."""
+    from charm_amd.device import DevArray, Stream
+
+    N = 4999
+    C, S = kaula_coeffs(N, 5)
+    g = crd.PointGridDH2(N, R_EARTH)
+    m = model(N, C, S, GM, R_EARTH)
+    f = shs.point(g, m, N)
+    out = sha.point(g, f, N, GM, R_EARTH)
+    err = max(np.abs(out.c - C).max(), np.abs(out.s - S).max()) / np.abs(C).max()
+    assert err <= TOL, err
+
+
 def test_device_sctr_matches_host_api():
     from charm_amd.device import DevArray, shs_sctr, synchronize
 
