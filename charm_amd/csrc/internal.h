@@ -161,6 +161,24 @@ struct GradCombine {
 };
 int launch_grad_table(int nmax, const void *cs_dev, const GradTable &g, void *out_dev,
                       void *stream);
+// On grids whose rows go through the FFT, the weights depend on the row only,
+// so outputs are combined from a few basis syntheses at the spectrum level:
+// term = coef t^pt u^-pu r^-pr q^pq (i m)^lon X_basis[row][m]
+struct SpecTerm {
+    double coef;
+    int basis, lon, pt, pu, pr, pq;
+};
+constexpr int SPEC_MAXTERM = 64;
+struct SpecCombine {
+    int nout;
+    int beg[GRAD_MAXOUT + 1];
+    SpecTerm term[SPEC_MAXTERM];
+    void *out[GRAD_MAXOUT];             // [row][m] double2 per output
+};
+// basis b of row r at basis + (b * nrow + r) * ld; outputs likewise
+int launch_grad_spec_combine(size_t nrow, size_t ld, const double *t_dev, const double *u_dev,
+                             const double *r_dev, int unit, double R, const void *basis_dev,
+                             const SpecCombine &c, void *stream);
 // Point p of nrow x ncol lies on row p / ncol; t, u, r per row.  With unit
 // set every radius is taken as R.  Table k's values start at tabs + k * tld.
 int launch_grad_combine(size_t nrow, size_t ncol, const double *t_dev, const double *u_dev,

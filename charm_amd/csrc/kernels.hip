@@ -1638,6 +1638,53 @@ __global__ void grad_combine_kernel(size_t nrow, size_t ncol, const double *__re
     }
 }
 
+__global__ void grad_spec_combine_kernel(size_t nrow, size_t ld, const double *__restrict__ t,
+                                         const double *__restrict__ u,
+                                         const double *__restrict__ r, int unit, double R,
+                                         const double2 *__restrict__ basis, SpecCombine c)
+{
+    const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t row = blockIdx.y;
+    if (m >= ld) return;
+    const double tt = t[row], ui = 1.0 / u[row];
+    const double rr = unit ? R : r[row];
+    const double ri = 1.0 / rr, q = unit ? 1.0 : R / rr;
+    const double tp[3] = {1.0, tt, tt * tt}, up[3] = {1.0, ui, ui * ui};
+    const double rp[3] = {1.0, ri, ri * ri}, qp[2] = {1.0, q};
+    const double dm = (double)m;
+    for (int o = 0; o < c.nout; o++) {
+        double ax = 0.0, ay = 0.0;
+        for (int k = c.beg[o]; k < c.beg[o + 1]; k++) {
+            const SpecTerm &g = c.term[k];
+            const double w = g.coef * ((tp[g.pt] * up[g.pu]) * (rp[g.pr] * qp[g.pq]));
+            const double2 b = basis[((size_t)g.basis * nrow + row) * ld + m];
+            // (i m)^lon
+            double bx = b.x, by = b.y;
+            if (g.lon == 1) {
+                bx = -dm * b.y;
+                by = dm * b.x;
+            } else if (g.lon == 2) {
+                bx = -(dm * dm) * b.x;
+                by = -(dm * dm) * b.y;
+            }
+            ax += w * bx;
+            ay += w * by;
+        }
+        ((double2 *)c.out[o])[row * ld + m] = make_double2(ax, ay);
+    }
+}
+
+int launch_grad_spec_combine(size_t nrow, size_t ld, const double *t, const double *u,
+                             const double *r, int unit, double R, const void *basis,
+                             const SpecCombine &c, void *stream)
+{
+    if (nrow == 0 || ld == 0) return 0;
+    hipLaunchKernelGGL(grad_spec_combine_kernel, dim3((unsigned)((ld + 255) / 256), (unsigned)nrow),
+                       dim3(256), 0, S(stream), nrow, ld, t, u, r, unit, R,
+                       (const double2 *)basis, c);
+    return (int)hipGetLastError();
+}
+
 int launch_grad_table(int nmax, const void *cs, const GradTable &g, void *out, void *stream)
 {
     const unsigned gx = (unsigned)((nmax + 1 + 255) / 256);

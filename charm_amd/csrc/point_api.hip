@@ -596,11 +596,13 @@ int tuned_k(int kind, int nring, unsigned long nmax, int kmin, int kmax, hipStre
     return best;
 }
 
-void run_shs_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev, bool use_fft,
-                  unsigned long nmax, double mur, const void *cs_dev, const Tables &tb,
-                  double *f_dev, hipStream_t stream)
+// Lumped sums of a grid synthesis: the Hermitian half-spectra of the rows
+// (fft = true, ld = nlon/2 + 1 values per row, ready for c2r) or the raw
+// (a_m, b_m) (fft = false, ld = nmax + 1 per row)
+void shs_grid_lumped(const GridGeom &g, const DevGrid &d, bool fft, unsigned long nmax, double mur,
+                     const void *cs_dev, const Tables &tb, void *out, size_t ld,
+                     hipStream_t stream)
 {
-    const size_t nlat = g.nlat, nlon = g.nlon;
     ShsGridArgs a{};
     a.nmax = (int)nmax;
     a.nring = (int)g.nring;
@@ -623,40 +625,36 @@ void run_shs_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev, bo
     HIPCHK((hipError_t)launch_scale_cs((int)nmax, cs_dev, tb.sn, tb.nt, cs2.p, stream));
     a.cs = cs2.p;
     a.mur = mur;
-    auto pick_k = [&](ShsGridArgs &x) {
-        x.k = tuned_k(0, (int)g.nring, nmax, 1, 3, stream, [&](int k) {
-            ShsGridArgs y = x;
-            y.k = k;
-            y.ntile = ring_tiles(g.nring, k);
-            HIPCHK((hipError_t)launch_shs_grid(y, stream));
-        });
-        x.ntile = ring_tiles(g.nring, x.k);
-    };
+    a.out = out;
+    a.ldo = ld;
+    a.mcount = (int)ld;
+    a.fft_layout = fft ? 1 : 0;
+    a.k = tuned_k(0, (int)g.nring, nmax, 1, 3, stream, [&](int k) {
+        ShsGridArgs y = a;
+        y.k = k;
+        y.ntile = ring_tiles(g.nring, k);
+        HIPCHK((hipError_t)launch_shs_grid(y, stream));
+    });
+    a.ntile = ring_tiles(g.nring, a.k);
+    ProfScope ps("shs_grid_kernel", stream);
+    HIPCHK((hipError_t)launch_shs_grid(a, stream));
+}
+
+void run_shs_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev, bool use_fft,
+                  unsigned long nmax, double mur, const void *cs_dev, const Tables &tb,
+                  double *f_dev, hipStream_t stream)
+{
+    const size_t nlat = g.nlat, nlon = g.nlon;
     if (use_fft) {
         const size_t nfc = nlon / 2 + 1;
         DevBuf spec(nlat * nfc * 2 * sizeof(double), stream);
-        a.out = spec.p;
-        a.ldo = nfc;
-        a.mcount = (int)nfc;
-        a.fft_layout = 1;
-        pick_k(a);
-        {
-            ProfScope ps("shs_grid_kernel", stream);
-            HIPCHK((hipError_t)launch_shs_grid(a, stream));
-        }
-        {
-            ProfScope ps("fft_c2r", stream);
-            rows_c2r(nlon, nlat, spec.p, nfc, f_dev, stream);
-        }
+        shs_grid_lumped(g, d, true, nmax, mur, cs_dev, tb, spec.p, nfc, stream);
+        ProfScope ps("fft_c2r", stream);
+        rows_c2r(nlon, nlat, spec.p, nfc, f_dev, stream);
     } else {
         const size_t ldl = nmax + 1;
         DevBuf lc(nlat * ldl * 2 * sizeof(double), stream);
-        a.out = lc.p;
-        a.ldo = ldl;
-        a.mcount = (int)ldl;
-        a.fft_layout = 0;
-        pick_k(a);
-        HIPCHK((hipError_t)launch_shs_grid(a, stream));
+        shs_grid_lumped(g, d, false, nmax, mur, cs_dev, tb, lc.p, ldl, stream);
         HIPCHK((hipError_t)launch_lon_direct((int)nlat, (int)nmax, nlon, lon_dev, lc.p, ldl, mur,
                                              f_dev, stream));
     }
@@ -1041,6 +1039,63 @@ void grad_tables_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev
     }
 }
 
+// Grids through the FFT: every table is c0 + c1 n + c2 n^2 (shifted or not)
+// times a longitude map, and the longitude maps are (i m)^lon on the
+// spectrum, so the outputs combine the half-spectra of at most five basis
+// syntheses (n^j X, j = 0..2, and the shifted e X, e n X) row by row, then
+// take one c2r each: grad2 costs five syntheses and six row transforms.
+bool grad_grid_fft(const GradPlan &plan, const GridGeom &g, const DevGrid &d, unsigned long nmax,
+                   double mur, const void *cs_dev, const Tables &tb, const double *trow,
+                   const double *urow, const double *rrow, bool unit, double R, double *out,
+                   hipStream_t s)
+{
+    const size_t nlat = g.nlat, nlon = g.nlon, nfc = nlon / 2 + 1;
+    if (nlat > 65535) return false;
+    // basis index of (power j, shift)
+    int bidx[3][2];
+    for (auto &v : bidx) v[0] = v[1] = -1;
+    std::vector<GradTable> basis;
+    SpecCombine c{};
+    c.nout = (int)plan.outs.size();
+    int q = 0;
+    for (int o = 0; o < c.nout; o++) {
+        c.beg[o] = q;
+        for (const GradTerm &x : plan.outs[o]) {
+            const GradTable &tab = plan.tabs[x.tab];
+            const double cj[3] = {tab.c0, tab.c1, tab.c2};
+            for (int j = 0; j < 3; j++) {
+                if (cj[j] == 0.0) continue;
+                int &b = bidx[j][tab.shift];
+                if (b < 0) {
+                    b = (int)basis.size();
+                    basis.push_back(GradTable{j == 0 ? 1.0 : 0.0, j == 1 ? 1.0 : 0.0,
+                                              j == 2 ? 1.0 : 0.0, tab.shift, 0});
+                }
+                if (q >= SPEC_MAXTERM) return false;
+                c.term[q++] = SpecTerm{x.coef * cj[j], b, tab.lon, x.pt, x.pu, x.pr, x.pq};
+            }
+        }
+    }
+    c.beg[c.nout] = q;
+    const size_t per = nlat * nfc * 2 * sizeof(double);
+    DevBuf ctab(tri_len(nmax) * 2 * sizeof(double), s), bspec(basis.size() * per, s);
+    for (size_t b = 0; b < basis.size(); b++) {
+        HIPCHK((hipError_t)launch_grad_table((int)nmax, cs_dev, basis[b], ctab.p, s));
+        shs_grid_lumped(g, d, true, nmax, mur, ctab.p, tb, (char *)bspec.p + b * per, nfc, s);
+    }
+    DevBuf ospec(c.nout * per, s);
+    for (int o = 0; o < c.nout; o++) c.out[o] = (char *)ospec.p + o * per;
+    {
+        ProfScope ps("grad_combine_kernel", s);
+        HIPCHK((hipError_t)launch_grad_spec_combine(nlat, nfc, trow, urow, rrow, unit ? 1 : 0, R,
+                                                    bspec.p, c, s));
+    }
+    ProfScope ps("fft_c2r", s);
+    for (int o = 0; o < c.nout; o++)
+        rows_c2r(nlon, nlat, c.out[o], nfc, out + (size_t)o * nlat * nlon, s);
+    return true;
+}
+
 // One synthesis per table of the plan at scattered points
 void grad_tables_sctr(size_t npoint, const double *lat_dev, const double *lon_dev,
                       const double *r_dev, bool unit, unsigned long nmax, double mu, double R,
@@ -1098,7 +1153,8 @@ void shs_grad_host(const charm_point *pnt, const charm_shc *shcs, unsigned long 
     upload_coeffs(shcs, nmax, s, cs);
     const size_t ntab = plan.tabs.size();
     const size_t nrow = pnt->nlat, ncol = sctr ? 1 : pnt->nlon, npoint = nrow * ncol;
-    DevBuf tabs(ntab * npoint * sizeof(double), s), outs(nout * npoint * sizeof(double), s);
+    // the per-table grids are needed only off the spectral path
+    DevBuf tabs, outs(nout * npoint * sizeof(double), s);
     DevBuf dt(nrow * sizeof(double), s), du(nrow * sizeof(double), s);
     DevBuf dr(nrow * sizeof(double), s);
     h2d(dr.as<double>(), pnt->r, nrow, s);
@@ -1106,6 +1162,7 @@ void shs_grad_host(const charm_point *pnt, const charm_shc *shcs, unsigned long 
     for (size_t i = 0; i < nrow && unit; i++)
         unit = nearly_equal(pnt->r[i], shcs->r, charm_glob_threshold);
     if (sctr) {
+        tabs.alloc(ntab * npoint * sizeof(double), s);
         DevBuf dlat(npoint * sizeof(double), s), dlon(npoint * sizeof(double), s);
         h2d(dlat.as<double>(), pnt->lat, npoint, s);
         h2d(dlon.as<double>(), pnt->lon, npoint, s);
@@ -1120,11 +1177,20 @@ void shs_grad_host(const charm_point *pnt, const charm_shc *shcs, unsigned long 
         grid_rows_tu(G.g, ht, hu);
         h2d(dt.as<double>(), ht.data(), nrow, s);
         h2d(du.as<double>(), hu.data(), nrow, s);
-        grad_tables_grid(G.g, G.d, G.lon.as<double>(), G.use_fft, nmax, shcs->mu / shcs->r, cs.p,
-                         tb, plan, tabs.as<double>(), s);
+        if (!G.use_fft ||
+            !grad_grid_fft(plan, G.g, G.d, nmax, shcs->mu / shcs->r, cs.p, tb, dt.as<double>(),
+                           du.as<double>(), dr.as<double>(), unit, shcs->r, outs.as<double>(),
+                           s)) {
+            tabs.alloc(ntab * npoint * sizeof(double), s);
+            grad_tables_grid(G.g, G.d, G.lon.as<double>(), G.use_fft, nmax, shcs->mu / shcs->r,
+                             cs.p, tb, plan, tabs.as<double>(), s);
+            grad_combine(plan, nrow, ncol, dt.as<double>(), du.as<double>(), dr.as<double>(),
+                         unit, shcs->r, tabs.as<double>(), outs.as<double>(), s);
+        }
     }
-    grad_combine(plan, nrow, ncol, dt.as<double>(), du.as<double>(), dr.as<double>(), unit,
-                 shcs->r, tabs.as<double>(), outs.as<double>(), s);
+    if (sctr)
+        grad_combine(plan, nrow, ncol, dt.as<double>(), du.as<double>(), dr.as<double>(), unit,
+                     shcs->r, tabs.as<double>(), outs.as<double>(), s);
     for (int o = 0; o < nout; o++)
         HIPCHK(hipMemcpyAsync(f[o], outs.as<double>() + (size_t)o * npoint,
                               npoint * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1775,11 +1841,14 @@ void charm_amd_shs_grid_deriv_dev(const charm_amd_grid *grd, unsigned long nmax,
             d.rld = 0;
         }
         const size_t nrow = grd->g.nlat, ncol = grd->g.nlon;
-        DevBuf tabs(plan.tabs.size() * nrow * ncol * sizeof(double), s);
-        grad_tables_grid(grd->g, d, grd->lon, true, nmax, mu / R, cs_dev, tb, plan,
-                         tabs.as<double>(), s);
-        grad_combine(plan, nrow, ncol, grd->trow, grd->urow, grd->rrow, unit, R, tabs.as<double>(),
-                     f_dev, s);
+        if (!grad_grid_fft(plan, grd->g, d, nmax, mu / R, cs_dev, tb, grd->trow, grd->urow,
+                           grd->rrow, unit, R, f_dev, s)) {
+            DevBuf tabs(plan.tabs.size() * nrow * ncol * sizeof(double), s);
+            grad_tables_grid(grd->g, d, grd->lon, true, nmax, mu / R, cs_dev, tb, plan,
+                             tabs.as<double>(), s);
+            grad_combine(plan, nrow, ncol, grd->trow, grd->urow, grd->rrow, unit, R,
+                         tabs.as<double>(), f_dev, s);
+        }
         HIPCHK(hipGetLastError());
     } catch (const HipFail &e) {
         report(err, e, __func__);
