@@ -22,8 +22,10 @@
 // column as X-numbers (mantissa in [2^-480, 2^480), exponent in units of
 // 2^960, src/leg/leg_func_xnum.h:299-443) and continues in doubles once both
 // terms of a lane are representable.  Here a lane carries scaled mantissas
-// with its own binary exponent (refreshed every 8 degrees); a wave runs the
-// plain loop once no lane has an exponent left.
+// with its own binary exponent, checked every 16 degrees and renormalised
+// only when a lane of the wave needs it; synthesis keeps its sums in the
+// same scaled units, so an X-number term costs what a plain one does.  A wave
+// runs the plain loop once no lane has an exponent left.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -75,58 +77,39 @@ __device__ __forceinline__ bool wave_all(bool p)
     return __ballot(!p) == 0;
 }
 
-// Column state of one lane: Q_{n-1} = x 2^e and Q_{n-2} = y 2^e.  Scaling by
-// a power of two is exact, so the recurrence runs on the mantissas; the
-// exponent is refreshed every 8 degrees and dropped (e = 0) once both true
-// values are at least 2^-480, where CHarm leaves X-numbers.  A term is
-// ldexp(z, e), which rounds (into the subnormal range, and to 0 far below it)
-// as CHarm's x * BIG^ix does.
-__device__ __forceinline__ void lane_renorm(double &x, double &y, int &e)
+// Lane state in the 16-degree tiles of the X-number phase.  The true column
+// values are x 2^e and y 2^e; ex is the binary exponent of |x| + |y| (that of
+// the larger value, within one), so e + ex locates the true magnitude.
+//  * A lane leaves the scaled form (e = 0) once e + ex > XS_PLAIN: from there
+//    on every value and product stays a normal double, so the plain
+//    recurrence gives the same roundings as the scaled one; a smaller
+//    previous value y only loses bits below 2^-1074, against a column value
+//    of at least 2^-800.  CHarm leaves X-numbers at 2^-480
+//    (src/leg/leg_func_xnum.h:436-442); the terms in between are the same
+//    doubles in both forms.
+//  * Mantissas are rescaled (e += ex) once |x| + |y| leaves [2^-256, 2^256]:
+//    16 degrees grow them by less than 2^144 (|alpha t| + 1 < 2^9 per
+//    degree), far from overflow.
+//  * A lane is deep when e + ex < XS_DEEP: its true values stay below 2^-1100
+//    for the next 16 degrees, so every term of the tile is exactly zero.
+constexpr int XS_PLAIN = -800;
+constexpr int XS_DEEP = -1250;
+
+__device__ __forceinline__ int xs_exp(double x, double y)
 {
-    if (e != 0) {
-        int ex, ey;
-        (void)frexp(x, &ex);
-        (void)frexp(y, &ey);
-        if (e + ex > -480 && e + ey > -480) {
-            x = ldexp(x, e);
-            y = ldexp(y, e);
-            e = 0;
-        } else if (ex > 256 || ex < -256) {
-            x = ldexp(x, -ex);
-            y = ldexp(y, -ex);
-            e += ex;
-        }
-    }
+    return __builtin_amdgcn_frexp_exp(fabs(x) + fabs(y));
 }
 
-// The true values stay below 2^-1100 for the next 8 degrees (each degree
-// grows them by at most |alpha t| + 1 < 2^9), so every term of the tile is
-// exactly zero in double precision.
-__device__ __forceinline__ bool lane_deep(double x, double y, int e)
+__device__ __forceinline__ bool xs_need(int e, int ex)
 {
-    int ex, ey;
-    (void)frexp(x, &ex);
-    (void)frexp(y, &ey);
-    return e != 0 && e + max(ex, ey) < -1180;
+    return e != 0 && (e + ex > XS_PLAIN || (unsigned)(ex + 256) > 512u);
 }
 
-// lane_renorm without branches, which also returns whether the next 8-degree
-// tile is deep (lane_deep of the renormalised state: rescaling keeps the true
-// values, so the exponents read here decide it).  The same power-of-two
-// scalings as lane_renorm, selected instead of branched: shift by e (both
-// values back in plain range), by -ex (mantissa drifted past 2^+-256) or 0.
-__device__ __forceinline__ void lane_renorm_deep(double &x, double &y, int &e, bool &deep)
+// power-of-two shift applied to a lane's mantissas (and scaled sums) in a
+// renormalisation: back to plain doubles, or mantissas back to [0.5, 1)
+__device__ __forceinline__ int xs_shift(int e, int ex)
 {
-    const int ex = __builtin_amdgcn_frexp_exp(x);
-    const int ey = __builtin_amdgcn_frexp_exp(y);
-    const bool xe = e != 0;
-    const bool plain = xe && e + min(ex, ey) > -480;
-    const bool drift = xe && (ex > 256 || ex < -256);
-    const int sh = plain ? e : (drift ? -ex : 0);
-    deep = xe && e + max(ex, ey) < -1180;
-    x = ldexp(x, sh);
-    y = ldexp(y, sh);
-    e -= sh;
+    return (e == 0 || e + ex > XS_PLAIN) ? e : -ex;
 }
 
 // The 8 alpha values of a tile from the LDS window, read before any other
@@ -407,7 +390,9 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
         }                                                                  \
     } while (0)
 
-        // degree m: P_00 = 1 or the sectorial seed P_mm (X-number)
+        // degree m: P_00 = 1 or the sectorial seed P_mm (X-number).  The sums
+        // are kept in the lane's scaled units (true sum = sum 2^e) and are
+        // rescaled with the mantissas, so terms need no per-term ldexp.
         {
             const double2 cm = csm[m];
 #pragma unroll
@@ -420,11 +405,11 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
                     e[k] = skip[k] ? 0 : 960 * A.six[(size_t)(m - 1) * A.sld + ring[k]];
                 }
                 y[k] = 0.0;
-                SHS_ACC(k, ldexp(x[k], e[k]), cm, 0);
+                SHS_ACC(k, x[k], cm, 0);
             }
         }
 
-        // degrees m+1..nmax in tiles of 8; (n + m) is odd at every tile start
+        // degrees m+1..nmax; (n + m) is odd at every tile start
         if (m < nmax) {
             ColStage<true> st;
             st.al = sm.al[wave];
@@ -435,20 +420,27 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
             st.lane = lane;
             st.start(m + 1);
             int n = m + 1;
-            // phase 1: tiles while a lane of the wave still carries an exponent
-            for (; n + 7 <= nmax; n += 8) {
+            bool dk[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int ex = xs_exp(x[k], y[k]);
+                dk[k] = !valid[k] || (e[k] != 0 && e[k] + ex < XS_DEEP);
+            }
+            // phase 1: 16-degree tiles while a lane of the wave still carries
+            // an exponent
+            for (; n + 15 <= nmax; n += 16) {
                 if (n - st.n0 == 64) st.next();
                 const int o = n - st.n0;
                 bool anyx = false, deep = true;
 #pragma unroll
                 for (int k = 0; k < K; k++) {
                     anyx = anyx || (e[k] != 0);
-                    deep = deep && (!valid[k] || lane_deep(x[k], y[k], e[k]));
+                    deep = deep && dk[k];
                 }
                 if (!wave_any(anyx)) break;
                 if (wave_all(deep)) {
 #pragma unroll
-                    for (int j = 0; j < 8; j++) {
+                    for (int j = 0; j < 16; j++) {
                         const double al = st.al[o + j];
 #pragma unroll
                         for (int k = 0; k < K; k++) {
@@ -460,22 +452,46 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
                     }
                 } else {
 #pragma unroll
-                    for (int j = 0; j < 8; j++) {
-                        const double al = st.al[o + j];
-                        const double2 c = st.cs[o + j];
+                    for (int j = 0; j < 16; j += 2) {
+                        const double2 a2 = *(const double2 *)(st.al + o + j);
+                        const double2 c0 = st.cs[o + j], c1 = st.cs[o + j + 1];
 #pragma unroll
                         for (int k = 0; k < K; k++) {
-                            const double z = fma(al * x[k], t[k], -y[k]);
-                            y[k] = x[k];
-                            x[k] = z;
-                            SHS_ACC(k, ldexp(z, e[k]), c, (j & 1) ^ 1);
+                            const double z = fma(a2.x * x[k], t[k], -y[k]);
+                            SHS_ACC(k, z, c0, 1);
+                            const double w = fma(a2.y * z, t[k], -x[k]);
+                            SHS_ACC(k, w, c1, 0);
+                            y[k] = z;
+                            x[k] = w;
                         }
                     }
                 }
+                int ex[K], d[K];
+                bool need = false;
 #pragma unroll
-                for (int k = 0; k < K; k++) lane_renorm(x[k], y[k], e[k]);
+                for (int k = 0; k < K; k++) {
+                    ex[k] = xs_exp(x[k], y[k]);
+                    d[k] = e[k] + ex[k];
+                    need = need || xs_need(e[k], ex[k]);
+                }
+                if (wave_any(need)) {
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        const int sh = xs_shift(e[k], ex[k]);
+                        x[k] = ldexp(x[k], sh);
+                        y[k] = ldexp(y[k], sh);
+                        aE[k] = ldexp(aE[k], sh);
+                        aO[k] = ldexp(aO[k], sh);
+                        bE[k] = ldexp(bE[k], sh);
+                        bO[k] = ldexp(bO[k], sh);
+                        e[k] -= sh;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < K; k++) dk[k] = !valid[k] || (e[k] != 0 && d[k] < XS_DEEP);
             }
-            // phase 2: every lane in plain doubles
+            // phase 2: 8-degree tiles; lanes still scaled here have fewer
+            // than 16 degrees left
             for (; n + 7 <= nmax; n += 8) {
                 if (n - st.n0 == 64) st.next();
                 const int o = n - st.n0;
@@ -509,7 +525,7 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
                             const double z = fma(al * x[k], t[k], -y[k]);
                             y[k] = x[k];
                             x[k] = z;
-                            SHS_ACC(k, ldexp(z, e[k]), c, (j & 1) ^ 1);
+                            SHS_ACC(k, z, c, (j & 1) ^ 1);
                         }
                     }
                 }
@@ -517,6 +533,15 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
         }
 #undef SHS_ACC
 #undef SHS_SKIP
+        // sums of lanes that never left the scaled form back to true units
+        // (tiny, or exactly zero)
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            aE[k] = ldexp(aE[k], e[k]);
+            aO[k] = ldexp(aO[k], e[k]);
+            bE[k] = ldexp(bE[k], e[k]);
+            bO[k] = ldexp(bO[k], e[k]);
+        }
         if (m == 0) {
             // S_n0 does not exist; the zonal sums have no sine part
 #pragma unroll
@@ -851,11 +876,17 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, i
     int n = m + 1;
     bool dk[K];
 #pragma unroll
-    for (int k = 0; k < K; k++) dk[k] = !valid[k] || lane_deep(x[k], y[k], e[k]);
-    // phase 1: tiles while a lane of the wave still carries an exponent
-    for (; n + 7 <= nmax; n += 8) {
+    for (int k = 0; k < K; k++) {
+        const int ex = xs_exp(x[k], y[k]);
+        dk[k] = !valid[k] || (e[k] != 0 && e[k] + ex < XS_DEEP);
+    }
+    // phase 1: 16-degree tiles (two reducer tiles each) while a lane of the
+    // wave still carries an exponent; an 8-degree tile at the end of the
+    // column, where 16 degrees do not fit (the plain phase has no ldexp)
+    int hlen = 16;
+    for (; n + 7 <= nmax; n += hlen) {
         if (n - st.n0 == 64) st.next();
-        const int o = n - st.n0;
+        hlen = (n + 15 <= nmax) ? 16 : 8;
         bool anyx = false, deep = true;
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -863,50 +894,69 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, i
             deep = deep && dk[k];
         }
         if (!wave_any(anyx)) break;
-        double a[8];
-        load_alpha8(st.al + o, a);
-        if (PIPE) rd.begin();
-        if (wave_all(deep)) {
-            // every term of the tile is exactly zero
+        const bool alldeep = wave_all(deep);
+#pragma unroll 1
+        for (int h = 0; h < hlen; h += 8) {
+            const int o = n + h - st.n0;
+            double a[8];
+            load_alpha8(st.al + o, a);
+            if (PIPE) rd.begin();
+            if (alldeep) {
+                // every term of the tile is exactly zero
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
+                for (int j = 0; j < 8; j++) {
 #pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const double z = fma(a[j] * x[k], t[k], -y[k]);
-                    y[k] = x[k];
-                    x[k] = z;
+                    for (int k = 0; k < K; k++) {
+                        const double z = fma(a[j] * x[k], t[k], -y[k]);
+                        y[k] = x[k];
+                        x[k] = z;
+                    }
                 }
-            }
-            if (PIPE) rd.finish();
-            if (lane < 16 && !acc) dst[2 * n + lane] = 0.0;
-        } else {
+                if (PIPE) rd.finish();
+                if (lane < 16 && !acc) dst[2 * (n + h) + lane] = 0.0;
+            } else {
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const double al = a[j];
-                double c = 0.0, sv = 0.0;
+                for (int j = 0; j < 8; j++) {
+                    const double al = a[j];
+                    double c = 0.0, sv = 0.0;
 #pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const double z = fma(al * x[k], t[k], -y[k]);
-                    y[k] = x[k];
-                    x[k] = z;
-                    const double p = ldexp(z, e[k]);
-                    c = fma(p, (j & 1) ? ampC[k] : ammC[k], c);
-                    sv = fma(p, (j & 1) ? ampS[k] : ammS[k], sv);
+                    for (int k = 0; k < K; k++) {
+                        const double z = fma(al * x[k], t[k], -y[k]);
+                        y[k] = x[k];
+                        x[k] = z;
+                        const double p = ldexp(z, e[k]);
+                        c = fma(p, (j & 1) ? ampC[k] : ammC[k], c);
+                        sv = fma(p, (j & 1) ? ampS[k] : ammS[k], sv);
+                    }
+                    rd.put(j, c, sv);
                 }
-                rd.put(j, c, sv);
-            }
-            if (PIPE) rd.finish();
-            rd.commit(8, dst + 2 * n);
-            if (!PIPE) {
-                rd.begin();
-                rd.finish();
+                if (PIPE) rd.finish();
+                rd.commit(8, dst + 2 * (n + h));
+                if (!PIPE) {
+                    rd.begin();
+                    rd.finish();
+                }
             }
         }
+        int ex[K], d[K];
+        bool need = false;
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            lane_renorm_deep(x[k], y[k], e[k], dk[k]);
-            dk[k] = dk[k] || !valid[k];
+            ex[k] = xs_exp(x[k], y[k]);
+            d[k] = e[k] + ex[k];
+            need = need || xs_need(e[k], ex[k]);
         }
+        if (wave_any(need)) {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int sh = xs_shift(e[k], ex[k]);
+                x[k] = ldexp(x[k], sh);
+                y[k] = ldexp(y[k], sh);
+                e[k] -= sh;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) dk[k] = !valid[k] || (e[k] != 0 && d[k] < XS_DEEP);
     }
     // phase 2: every lane in plain doubles
     for (; n + 7 <= nmax; n += 8) {
@@ -1127,9 +1177,9 @@ __device__ __forceinline__ void sctr_item(const SctrArgs &A, int blk, int chunk,
             x[k] = skip[k] ? 0.0 : xm_;
             e[k] = skip[k] ? 0 : em;
             y[k] = 0.0;
-            const double p = ldexp(x[k], e[k]);
-            a[k] = p * cm.x;
-            b[k] = p * cm.y;
+            // sums in the lane's scaled units (true sum = sum 2^e)
+            a[k] = x[k] * cm.x;
+            b[k] = x[k] * cm.y;
         }
         if (m < nmax && !allskip) {
             st.gal = A.al + dcol_off(A.al_nt, m) - m;
@@ -1138,20 +1188,27 @@ __device__ __forceinline__ void sctr_item(const SctrArgs &A, int blk, int chunk,
 #define SC_REC(k, al)                                                      \
     (UNIT ? fma((al) * x[k], tq[k], -y[k]) : fma((al) * x[k], tq[k], -(q2[k] * y[k])))
             int n = m + 1;
-            // phase 1: tiles while a lane of the wave still carries an exponent
-            for (; n + 7 <= nmax; n += 8) {
+            bool dk[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int ex = xs_exp(x[k], y[k]);
+                dk[k] = !valid[k] || (e[k] != 0 && e[k] + ex < XS_DEEP);
+            }
+            // phase 1: 16-degree tiles while a lane of the wave still carries
+            // an exponent
+            for (; n + 15 <= nmax; n += 16) {
                 if (n - st.n0 == 64) st.next();
                 const int o = n - st.n0;
                 bool anyx = false, deep = true;
 #pragma unroll
                 for (int k = 0; k < K; k++) {
                     anyx = anyx || (e[k] != 0);
-                    deep = deep && (!valid[k] || lane_deep(x[k], y[k], e[k]));
+                    deep = deep && dk[k];
                 }
                 if (!wave_any(anyx)) break;
                 if (wave_all(deep)) {
 #pragma unroll
-                    for (int j = 0; j < 8; j++) {
+                    for (int j = 0; j < 16; j++) {
                         const double al = st.al[o + j];
 #pragma unroll
                         for (int k = 0; k < K; k++) {
@@ -1162,24 +1219,48 @@ __device__ __forceinline__ void sctr_item(const SctrArgs &A, int blk, int chunk,
                     }
                 } else {
 #pragma unroll
-                    for (int j = 0; j < 8; j++) {
-                        const double al = st.al[o + j];
-                        const double2 c = st.cs[o + j];
+                    for (int j = 0; j < 16; j += 2) {
+                        const double2 a2 = *(const double2 *)(st.al + o + j);
+                        const double2 c0 = st.cs[o + j], c1 = st.cs[o + j + 1];
 #pragma unroll
                         for (int k = 0; k < K; k++) {
-                            const double z = SC_REC(k, al);
+                            const double z = SC_REC(k, a2.x);
+                            a[k] = fma(z, c0.x, a[k]);
+                            b[k] = fma(z, c0.y, b[k]);
                             y[k] = x[k];
                             x[k] = z;
-                            const double p = ldexp(z, e[k]);
-                            a[k] = fma(p, c.x, a[k]);
-                            b[k] = fma(p, c.y, b[k]);
+                            const double w = SC_REC(k, a2.y);
+                            a[k] = fma(w, c1.x, a[k]);
+                            b[k] = fma(w, c1.y, b[k]);
+                            y[k] = x[k];
+                            x[k] = w;
                         }
                     }
                 }
+                int ex[K], d[K];
+                bool need = false;
 #pragma unroll
-                for (int k = 0; k < K; k++) lane_renorm(x[k], y[k], e[k]);
+                for (int k = 0; k < K; k++) {
+                    ex[k] = xs_exp(x[k], y[k]);
+                    d[k] = e[k] + ex[k];
+                    need = need || xs_need(e[k], ex[k]);
+                }
+                if (wave_any(need)) {
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        const int sh = xs_shift(e[k], ex[k]);
+                        x[k] = ldexp(x[k], sh);
+                        y[k] = ldexp(y[k], sh);
+                        a[k] = ldexp(a[k], sh);
+                        b[k] = ldexp(b[k], sh);
+                        e[k] -= sh;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < K; k++) dk[k] = !valid[k] || (e[k] != 0 && d[k] < XS_DEEP);
             }
-            // phase 2: every lane in plain doubles
+            // phase 2: 8-degree tiles; lanes still scaled here have fewer
+            // than 16 degrees left
             for (; n + 7 <= nmax; n += 8) {
                 if (n - st.n0 == 64) st.next();
                 const int o = n - st.n0;
@@ -1217,9 +1298,8 @@ __device__ __forceinline__ void sctr_item(const SctrArgs &A, int blk, int chunk,
                             const double z = SC_REC(k, al);
                             y[k] = x[k];
                             x[k] = z;
-                            const double p = ldexp(z, e[k]);
-                            a[k] = fma(p, c.x, a[k]);
-                            b[k] = fma(p, c.y, b[k]);
+                            a[k] = fma(z, c.x, a[k]);
+                            b[k] = fma(z, c.y, b[k]);
                         }
                     }
                 }
@@ -1228,6 +1308,9 @@ __device__ __forceinline__ void sctr_item(const SctrArgs &A, int blk, int chunk,
         }
 #pragma unroll
         for (int k = 0; k < K; k++) {
+            // sums of lanes still in the scaled form back to true units
+            a[k] = ldexp(a[k], e[k]);
+            b[k] = ldexp(b[k], e[k]);
             if (m == 0) b[k] = 0.0;
             double sn, cn;
             sincos((double)m * lon[k], &sn, &cn);
