@@ -304,7 +304,7 @@ struct ShsSmem {
 };
 
 template <int K, bool UNIT>
-__device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, int quad,
+__device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int rbase, int quad,
                                               ShsSmem<K> &sm)
 {
     const int lane = threadIdx.x & 63;
@@ -319,7 +319,7 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
     double t[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        ring[k] = tile * 64 * K + k * 64 + lane;
+        ring[k] = rbase + k * 64 + lane;
         valid[k] = ring[k] < A.nring;
         t[k] = valid[k] ? A.t[ring[k]] : 0.0;
     }
@@ -577,7 +577,7 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
     for (int ei = threadIdx.x; ei < 2 * 64 * K * 4; ei += 256) {
         const int col = ei & 3, rowsel = ei >> 2;
         const int ns = rowsel / (64 * K), rl = rowsel % (64 * K);
-        const int i = tile * 64 * K + rl;
+        const int i = rbase + rl;
         const int mm = m0 + col;
         if (i >= A.nring || mm >= A.mcount) continue;
         int row = i;
@@ -589,12 +589,41 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
     }
 }
 
+// Rings per lane for ring tile `tile` of a launch with K: the last tile runs
+// with the fewest that cover its rings (its other lanes would idle).
+template <int K>
+__device__ __forceinline__ int tile_k(int nring, int tile)
+{
+    const int rem = nring - tile * 64 * K;
+    return uniform(rem >= 64 * K ? K : max(1, (rem + 63) / 64));
+}
+
+template <int K, bool UNIT>
+__device__ __forceinline__ void shs_tile(const ShsGridArgs &A, int tile, int quad, ShsSmem<K> &sm)
+{
+    const int rbase = tile * 64 * K;
+    const int kt = tile_k<K>(A.nring, tile);
+    if constexpr (K >= 2) {
+        if (kt == 1) {
+            shs_grid_item<1, UNIT>(A, rbase, quad, reinterpret_cast<ShsSmem<1> &>(sm));
+            return;
+        }
+    }
+    if constexpr (K >= 3) {
+        if (kt == 2) {
+            shs_grid_item<2, UNIT>(A, rbase, quad, reinterpret_cast<ShsSmem<2> &>(sm));
+            return;
+        }
+    }
+    shs_grid_item<K, UNIT>(A, rbase, quad, sm);
+}
+
 template <int K, bool UNIT>
 __global__ __launch_bounds__(256) void shs_grid_kernel(ShsGridArgs A)
 {
     __shared__ __align__(16) ShsSmem<K> sm;
     const int b = blockIdx.x;
-    shs_grid_item<K, UNIT>(A, (b >> 3) % A.ntile, ((b >> 3) / A.ntile) * 8 + (b & 7), sm);
+    shs_tile<K, UNIT>(A, (b >> 3) % A.ntile, ((b >> 3) / A.ntile) * 8 + (b & 7), sm);
 }
 
 // Persistent form: blocks pull (request, ring tile, order quad) items from a
@@ -616,9 +645,9 @@ __global__ __launch_bounds__(256) void shs_batch_kernel(const ShsGridArgs *__res
         const ShsGridArgs A = reqs[uniform(w.x)];
         const int wy = uniform(w.y);
         if (A.unit)
-            shs_grid_item<K, true>(A, wy >> 16, wy & 0xFFFF, sm);
+            shs_tile<K, true>(A, wy >> 16, wy & 0xFFFF, sm);
         else
-            shs_grid_item<K, false>(A, wy >> 16, wy & 0xFFFF, sm);
+            shs_tile<K, false>(A, wy >> 16, wy & 0xFFFF, sm);
     }
 }
 
@@ -656,9 +685,12 @@ __global__ void lon_direct_kernel(int nlat, int nmax, size_t nlon, const double 
 // applies s_nm.
 // ---------------------------------------------------------------------------
 namespace {
-// Row pitch (doubles) of the per-wave transpose buffer: a multiple of 8 but
-// not of 16, so that the 16-byte reads of 16 lanes cover all 64 banks.
-constexpr int RED_LD = 72;
+// Per-wave transpose buffer: 8 rows (degrees) of RED_LD doubles.  Lanes
+// 0-31 hold C sums at columns 0-31, lanes 32-63 S sums at columns 36-67
+// (RED_SOFF), and RED_LD = 2 mod 32, so that the 16-byte reads of any 16 lanes
+// cover all 64 banks and the 8-byte writes of 32 lanes are contiguous.
+constexpr int RED_LD = 98;
+constexpr int RED_SOFF = 36;
 
 __device__ __forceinline__ double dpp_quad_xor1(double v)
 {
@@ -676,18 +708,32 @@ __device__ __forceinline__ double dpp_quad_xor2(double v)
     return __builtin_bit_cast(double, w);
 }
 
-// Cross-lane reduction of 16 values per lane (8 degrees x (C, S)) over the 64
-// lanes of a wave, software-pipelined across tiles: a tile's values are
-// written to LDS after the tile is computed, read back at the start of the
-// next tile and summed after that tile's arithmetic, so the LDS latency hides
-// behind the recurrence.  Lane l sums value l/4 over the 16 columns
-// 2q + 8i, 2q + 8i + 1 (q = l mod 4, i = 0..7); the 4 lanes of a quad then
-// combine through DPP.  The order of additions is fixed.
+// Lanes 0-31: c(l) + c(l + 32); lanes 32-63: s(l - 32) + s(l).  One
+// v_permlane32_swap per 32-bit half exchanges the upper half of c with the
+// lower half of s.
+__device__ __forceinline__ double fold_halves(double c, double s)
+{
+    const int2 cb = __builtin_bit_cast(int2, c), sb = __builtin_bit_cast(int2, s);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)cb.x, (unsigned)sb.x, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)cb.y, (unsigned)sb.y, false, false);
+    const double c2 = __builtin_bit_cast(double, make_int2((int)lo[0], (int)hi[0]));
+    const double s2 = __builtin_bit_cast(double, make_int2((int)lo[1], (int)hi[1]));
+    return c2 + s2;
+}
+
+// Cross-lane reduction of the 16 sums (8 degrees x (C, S)) of a tile over the
+// 64 lanes of a wave, software-pipelined across tiles: a tile's sums are
+// folded over the two wave halves (fold_halves) and written to LDS as they
+// are formed, read back at the start of the next tile and summed after that
+// tile's arithmetic, so the LDS latency hides behind the recurrence.  Lane l
+// sums output o = l/4 (degree o/2, C or S) over the 8 columns 8q..8q+7
+// (q = l mod 4) of its half; the 4 lanes of a quad then combine through DPP.
+// The order of additions is fixed.
 struct WaveReducer {
-    double *red;      // [16][RED_LD] doubles, this wave's slice of LDS
+    double *red;      // [8][RED_LD] doubles, this wave's slice of LDS
     double *dst;      // destination of the pending tile (nullptr: none)
     int cnt;          // degrees in the pending tile
-    double r[16];     // pending values read back from LDS
+    double r[8];      // pending values read back from LDS
     int lane;
     bool acc;         // add to dst (a later ring tile of the same slot)
 
@@ -696,11 +742,11 @@ struct WaveReducer {
     // that no branch makes the compiler wait for them early.
     __device__ __forceinline__ void begin()
     {
-        const int j = lane >> 2, q = lane & 3;
-        const double *row = red + j * RED_LD + 2 * q;
+        const int o = lane >> 2, q = lane & 3;
+        const double *row = red + (o >> 1) * RED_LD + (o & 1) * RED_SOFF + 8 * q;
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const double2 w = *(const double2 *)(row + 8 * i);
+        for (int i = 0; i < 4; i++) {
+            const double2 w = *(const double2 *)(row + 2 * i);
             r[2 * i] = w.x;
             r[2 * i + 1] = w.y;
         }
@@ -710,22 +756,22 @@ struct WaveReducer {
         if (dst) {
             double s = r[0];
 #pragma unroll
-            for (int i = 1; i < 16; i++) s += r[i];
+            for (int i = 1; i < 8; i++) s += r[i];
             s += dpp_quad_xor1(s);
             s += dpp_quad_xor2(s);
-            const int j = lane >> 2;
-            // the same lane owns output j for every tile of the slot, so the
+            const int o = lane >> 2;
+            // the same lane owns output o for every tile of the slot, so the
             // earlier tile's value is its own store
-            if ((lane & 3) == 0 && j < 2 * cnt) dst[j] = acc ? dst[j] + s : s;
+            if ((lane & 3) == 0 && o < 2 * cnt) dst[o] = acc ? dst[o] + s : s;
             dst = nullptr;
         }
     }
-    // values of degree j of the tile being computed (written as they are
+    // sums of degree j of the tile being computed (written as they are
     // formed; the pending tile's reads were issued before and execute first)
     __device__ __forceinline__ void put(int j, double c, double s)
     {
-        red[(2 * j) * RED_LD + lane] = c;
-        red[(2 * j + 1) * RED_LD + lane] = s;
+        const double v = fold_halves(c, s);
+        red[j * RED_LD + lane + (lane >= 32 ? RED_SOFF - 32 : 0)] = v;
     }
     // the tile just written becomes the pending one
     __device__ __forceinline__ void commit(int c, double *d)
@@ -739,7 +785,7 @@ struct WaveReducer {
 template <int K>
 struct ShaLds {
     static constexpr int STAGE = 2 * 64 * K * 4 * 2;  // doubles
-    static constexpr int RED = 4 * 16 * RED_LD;       // doubles
+    static constexpr int RED = 4 * 8 * RED_LD;        // doubles
     static constexpr int SIZE = STAGE > RED ? STAGE : RED;
 };
 
@@ -749,10 +795,11 @@ struct ShaSmem {
     double al[4][64];
 };
 
-// Ring tile `tile` of order quad `quad`; its sums go to partial slot `slot`,
+// The ring tile starting at ring `rbase`, order quad `quad`; its sums go to
+// partial slot `slot`,
 // written (acc false: the slot's first tile) or added to (acc true).
 template <int K>
-__device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, int slot, bool acc,
+__device__ __forceinline__ void sha_item(const ShaArgs &A, int rbase, int quad, int slot, bool acc,
                                          ShaSmem<K> &sm)
 {
     const int lane = threadIdx.x & 63;
@@ -774,7 +821,7 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, i
     for (int ei = threadIdx.x; ei < 2 * 64 * K * 4; ei += 256) {
         const int col = ei & 3, rowsel = ei >> 2;
         const int ns = rowsel / (64 * K), rl = rowsel % (64 * K);
-        const int i = tile * 64 * K + rl;
+        const int i = rbase + rl;
         const int mm = m0 + col;
         double2 v = make_double2(0.0, 0.0);
         if (i < A.nring && mm <= nmax) {
@@ -795,7 +842,7 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, i
     bool valid[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        ring[k] = tile * 64 * K + k * 64 + lane;
+        ring[k] = rbase + k * 64 + lane;
         valid[k] = ring[k] < A.nring;
         t[k] = valid[k] ? A.t[ring[k]] : 0.0;
         const double2 N = stage[((0 * 64 * K) + k * 64 + lane) * 4 + wave];
@@ -827,7 +874,7 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, i
         return;
     }
     WaveReducer rd;
-    rd.red = lds + wave * 16 * RED_LD;
+    rd.red = lds + wave * 8 * RED_LD;
     rd.dst = nullptr;
     rd.cnt = 0;
     rd.lane = lane;
@@ -1019,6 +1066,35 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int quad, i
     rd.finish();
 }
 
+// Ring tile `tile` (the last one with the fewest rings per lane that cover
+// its rings, as for synthesis)
+template <int K>
+__device__ __forceinline__ void sha_tile(const ShaArgs &A, int tile, int quad, int slot, bool acc,
+                                         ShaSmem<K> &sm)
+{
+    const int rbase = tile * 64 * K;
+    const int kt = tile_k<K>(A.nring, tile);
+    if constexpr (K >= 2) {
+        if (kt == 1) {
+            sha_item<1>(A, rbase, quad, slot, acc, reinterpret_cast<ShaSmem<1> &>(sm));
+            return;
+        }
+    }
+    if constexpr (K >= 3) {
+        if (kt == 2) {
+            sha_item<2>(A, rbase, quad, slot, acc, reinterpret_cast<ShaSmem<2> &>(sm));
+            return;
+        }
+    }
+    if constexpr (K >= 4) {
+        if (kt == 3) {
+            sha_item<3>(A, rbase, quad, slot, acc, reinterpret_cast<ShaSmem<3> &>(sm));
+            return;
+        }
+    }
+    sha_item<K>(A, rbase, quad, slot, acc, sm);
+}
+
 // Block b runs on XCD b mod 8 (round-robin dispatch).  XCD x takes the order
 // quad pairs p = 8g + x; its consecutive blocks alternate between the two
 // quads of a pair for each ring tile, so the two 64-byte halves of every
@@ -1035,7 +1111,7 @@ void sha_kernel(ShaArgs A)
     const int t1 = min(A.ntile, (grp + 1) * A.tpg);
     for (int tile = grp * A.tpg; tile < t1; tile++) {
         if (tile > grp * A.tpg) __syncthreads();  // the block's LDS is reused
-        sha_item<K>(A, tile, quad, grp, tile > grp * A.tpg, sm);
+        sha_tile<K>(A, tile, quad, grp, tile > grp * A.tpg, sm);
     }
 }
 
@@ -1055,7 +1131,7 @@ __global__ __launch_bounds__(256) void sha_batch_kernel(const ShaArgs *__restric
         const int2 w = items[it];
         const ShaArgs A = reqs[uniform(w.x)];
         const int wy = uniform(w.y);
-        sha_item<K>(A, wy >> 16, wy & 0xFFFF, wy >> 16, false, sm);
+        sha_tile<K>(A, wy >> 16, wy & 0xFFFF, wy >> 16, false, sm);
     }
 }
 
