@@ -59,6 +59,9 @@ struct ShsGridArgs {
     int k;             // ring pairs per lane (1..3)
     const double *u;   // [nring] cos(lat)
     double polar;      // polar optimisation threshold (< 0: off)
+    unsigned char *dead;  // [m * ntile + tile]: 1 if every term of the wave's
+                          // column is exactly zero (see dead_mode)
+    int dead_mode;     // 0: unused; 1: record the flags; 2: skip flagged waves
 };
 
 struct ShaArgs {
@@ -81,6 +84,8 @@ struct ShaArgs {
     int tpg;           // ring tiles per block, added in tile order into slot tile / tpg
     const double *u;   // [nring] cos(lat)
     double polar;      // polar optimisation threshold (< 0: off)
+    unsigned char *dead;  // as in ShsGridArgs
+    int dead_mode;
 };
 
 struct SctrArgs {

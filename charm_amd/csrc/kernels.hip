@@ -304,7 +304,7 @@ struct ShsSmem {
 };
 
 template <int K, bool UNIT>
-__device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int rbase, int quad,
+__device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, int rbase, int quad,
                                               ShsSmem<K> &sm)
 {
     const int lane = threadIdx.x & 63;
@@ -337,8 +337,11 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int rbase, i
         allskip = allskip && skip[k];
     }
     allskip = wave_all(allskip);
+    // a wave whose column was found all zero by an earlier transform on this
+    // grid (same degree and tiling) contributes nothing
+    const bool dead_skip = A.dead_mode == 2 && m <= nmax && A.dead[(size_t)m * A.ntile + tile] != 0;
 
-    if (m <= nmax && !allskip) {
+    if (m <= nmax && !allskip && !dead_skip) {
         const double2 *__restrict__ csm = (const double2 *)A.cs + dcol_off(nmax, m) - m;
         double x[K], y[K];
         int e[K];
@@ -409,6 +412,17 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int rbase, i
             }
         }
 
+        bool dk[K], dead = true;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int ex = xs_exp(x[k], y[k]);
+            dk[k] = !valid[k] || (e[k] != 0 && e[k] + ex < XS_DEEP);
+            dead = dead && dk[k];
+        }
+        // every term of the column is exactly zero while the wave stays deep
+        // from the seed to the last degree
+        dead = wave_all(dead);
+
         // degrees m+1..nmax; (n + m) is odd at every tile start
         if (m < nmax) {
             ColStage<true> st;
@@ -420,12 +434,6 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int rbase, i
             st.lane = lane;
             st.start(m + 1);
             int n = m + 1;
-            bool dk[K];
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                const int ex = xs_exp(x[k], y[k]);
-                dk[k] = !valid[k] || (e[k] != 0 && e[k] + ex < XS_DEEP);
-            }
             // phase 1: 16-degree tiles while a lane of the wave still carries
             // an exponent
             for (; n + 15 <= nmax; n += 16) {
@@ -437,8 +445,13 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int rbase, i
                     anyx = anyx || (e[k] != 0);
                     deep = deep && dk[k];
                 }
-                if (!wave_any(anyx)) break;
-                if (wave_all(deep)) {
+                if (!wave_any(anyx)) {
+                    dead = false;
+                    break;
+                }
+                const bool alldeep = wave_all(deep);
+                dead = dead && alldeep;
+                if (alldeep) {
 #pragma unroll
                     for (int j = 0; j < 16; j++) {
                         const double al = st.al[o + j];
@@ -489,6 +502,13 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int rbase, i
                 }
 #pragma unroll
                 for (int k = 0; k < K; k++) dk[k] = !valid[k] || (e[k] != 0 && d[k] < XS_DEEP);
+            }
+            // the last (fewer than 16) degrees: zero terms if still deep
+            {
+                bool deep = true;
+#pragma unroll
+                for (int k = 0; k < K; k++) deep = deep && dk[k];
+                dead = dead && wave_all(deep);
             }
             // phase 2: 8-degree tiles; lanes still scaled here have fewer
             // than 16 degrees left
@@ -542,6 +562,7 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int rbase, i
             bE[k] = ldexp(bE[k], e[k]);
             bO[k] = ldexp(bO[k], e[k]);
         }
+        if (A.dead_mode == 1 && lane == 0) A.dead[(size_t)m * A.ntile + tile] = dead ? 1 : 0;
         if (m == 0) {
             // S_n0 does not exist; the zonal sums have no sine part
 #pragma unroll
@@ -605,17 +626,17 @@ __device__ __forceinline__ void shs_tile(const ShsGridArgs &A, int tile, int qua
     const int kt = tile_k<K>(A.nring, tile);
     if constexpr (K >= 2) {
         if (kt == 1) {
-            shs_grid_item<1, UNIT>(A, rbase, quad, reinterpret_cast<ShsSmem<1> &>(sm));
+            shs_grid_item<1, UNIT>(A, tile, rbase, quad, reinterpret_cast<ShsSmem<1> &>(sm));
             return;
         }
     }
     if constexpr (K >= 3) {
         if (kt == 2) {
-            shs_grid_item<2, UNIT>(A, rbase, quad, reinterpret_cast<ShsSmem<2> &>(sm));
+            shs_grid_item<2, UNIT>(A, tile, rbase, quad, reinterpret_cast<ShsSmem<2> &>(sm));
             return;
         }
     }
-    shs_grid_item<K, UNIT>(A, rbase, quad, sm);
+    shs_grid_item<K, UNIT>(A, tile, rbase, quad, sm);
 }
 
 template <int K, bool UNIT>
@@ -685,12 +706,9 @@ __global__ void lon_direct_kernel(int nlat, int nmax, size_t nlon, const double 
 // applies s_nm.
 // ---------------------------------------------------------------------------
 namespace {
-// Per-wave transpose buffer: 8 rows (degrees) of RED_LD doubles.  Lanes
-// 0-31 hold C sums at columns 0-31, lanes 32-63 S sums at columns 36-67
-// (RED_SOFF), and RED_LD = 2 mod 32, so that the 16-byte reads of any 16 lanes
-// cover all 64 banks and the 8-byte writes of 32 lanes are contiguous.
-constexpr int RED_LD = 98;
-constexpr int RED_SOFF = 36;
+// Row pitch (doubles) of the per-wave transpose buffer: a multiple of 8 but
+// not of 16, so that the 16-byte reads of 16 lanes cover all 64 banks.
+constexpr int RED_LD = 72;
 
 __device__ __forceinline__ double dpp_quad_xor1(double v)
 {
@@ -708,32 +726,18 @@ __device__ __forceinline__ double dpp_quad_xor2(double v)
     return __builtin_bit_cast(double, w);
 }
 
-// Lanes 0-31: c(l) + c(l + 32); lanes 32-63: s(l - 32) + s(l).  One
-// v_permlane32_swap per 32-bit half exchanges the upper half of c with the
-// lower half of s.
-__device__ __forceinline__ double fold_halves(double c, double s)
-{
-    const int2 cb = __builtin_bit_cast(int2, c), sb = __builtin_bit_cast(int2, s);
-    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)cb.x, (unsigned)sb.x, false, false);
-    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)cb.y, (unsigned)sb.y, false, false);
-    const double c2 = __builtin_bit_cast(double, make_int2((int)lo[0], (int)hi[0]));
-    const double s2 = __builtin_bit_cast(double, make_int2((int)lo[1], (int)hi[1]));
-    return c2 + s2;
-}
-
-// Cross-lane reduction of the 16 sums (8 degrees x (C, S)) of a tile over the
-// 64 lanes of a wave, software-pipelined across tiles: a tile's sums are
-// folded over the two wave halves (fold_halves) and written to LDS as they
-// are formed, read back at the start of the next tile and summed after that
-// tile's arithmetic, so the LDS latency hides behind the recurrence.  Lane l
-// sums output o = l/4 (degree o/2, C or S) over the 8 columns 8q..8q+7
-// (q = l mod 4) of its half; the 4 lanes of a quad then combine through DPP.
-// The order of additions is fixed.
+// Cross-lane reduction of 16 values per lane (8 degrees x (C, S)) over the 64
+// lanes of a wave, software-pipelined across tiles: a tile's values are
+// written to LDS after the tile is computed, read back at the start of the
+// next tile and summed after that tile's arithmetic, so the LDS latency hides
+// behind the recurrence.  Lane l sums value l/4 over the 16 columns
+// 2q + 8i, 2q + 8i + 1 (q = l mod 4, i = 0..7); the 4 lanes of a quad then
+// combine through DPP.  The order of additions is fixed.
 struct WaveReducer {
-    double *red;      // [8][RED_LD] doubles, this wave's slice of LDS
+    double *red;      // [16][RED_LD] doubles, this wave's slice of LDS
     double *dst;      // destination of the pending tile (nullptr: none)
     int cnt;          // degrees in the pending tile
-    double r[8];      // pending values read back from LDS
+    double r[16];     // pending values read back from LDS
     int lane;
     bool acc;         // add to dst (a later ring tile of the same slot)
 
@@ -742,11 +746,11 @@ struct WaveReducer {
     // that no branch makes the compiler wait for them early.
     __device__ __forceinline__ void begin()
     {
-        const int o = lane >> 2, q = lane & 3;
-        const double *row = red + (o >> 1) * RED_LD + (o & 1) * RED_SOFF + 8 * q;
+        const int j = lane >> 2, q = lane & 3;
+        const double *row = red + j * RED_LD + 2 * q;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const double2 w = *(const double2 *)(row + 2 * i);
+        for (int i = 0; i < 8; i++) {
+            const double2 w = *(const double2 *)(row + 8 * i);
             r[2 * i] = w.x;
             r[2 * i + 1] = w.y;
         }
@@ -756,22 +760,22 @@ struct WaveReducer {
         if (dst) {
             double s = r[0];
 #pragma unroll
-            for (int i = 1; i < 8; i++) s += r[i];
+            for (int i = 1; i < 16; i++) s += r[i];
             s += dpp_quad_xor1(s);
             s += dpp_quad_xor2(s);
-            const int o = lane >> 2;
-            // the same lane owns output o for every tile of the slot, so the
+            const int j = lane >> 2;
+            // the same lane owns output j for every tile of the slot, so the
             // earlier tile's value is its own store
-            if ((lane & 3) == 0 && o < 2 * cnt) dst[o] = acc ? dst[o] + s : s;
+            if ((lane & 3) == 0 && j < 2 * cnt) dst[j] = acc ? dst[j] + s : s;
             dst = nullptr;
         }
     }
-    // sums of degree j of the tile being computed (written as they are
+    // values of degree j of the tile being computed (written as they are
     // formed; the pending tile's reads were issued before and execute first)
     __device__ __forceinline__ void put(int j, double c, double s)
     {
-        const double v = fold_halves(c, s);
-        red[j * RED_LD + lane + (lane >= 32 ? RED_SOFF - 32 : 0)] = v;
+        red[(2 * j) * RED_LD + lane] = c;
+        red[(2 * j + 1) * RED_LD + lane] = s;
     }
     // the tile just written becomes the pending one
     __device__ __forceinline__ void commit(int c, double *d)
@@ -785,7 +789,7 @@ struct WaveReducer {
 template <int K>
 struct ShaLds {
     static constexpr int STAGE = 2 * 64 * K * 4 * 2;  // doubles
-    static constexpr int RED = 4 * 8 * RED_LD;        // doubles
+    static constexpr int RED = 4 * 16 * RED_LD;       // doubles
     static constexpr int SIZE = STAGE > RED ? STAGE : RED;
 };
 
@@ -799,7 +803,7 @@ struct ShaSmem {
 // partial slot `slot`,
 // written (acc false: the slot's first tile) or added to (acc true).
 template <int K>
-__device__ __forceinline__ void sha_item(const ShaArgs &A, int rbase, int quad, int slot, bool acc,
+__device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, int quad, int slot, bool acc,
                                          ShaSmem<K> &sm)
 {
     const int lane = threadIdx.x & 63;
@@ -865,7 +869,10 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int rbase, int quad, 
         skip[k] = !valid[k] || (A.polar >= 0.0 && (double)m - (double)nmax * A.u[ring[k]] > A.polar);
         allskip = allskip && skip[k];
     }
-    if (wave_all(allskip)) {
+    // a wave whose column was found all zero by an earlier transform on this
+    // grid (same degree and tiling) contributes nothing
+    const bool dead_skip = A.dead_mode == 2 && A.dead[(size_t)m * A.ntile + tile] != 0;
+    if (wave_all(allskip) || dead_skip) {
         if (!acc)
             for (int n = m + lane; n <= nmax; n += 64) {
                 dst[2 * n] = 0.0;
@@ -874,7 +881,7 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int rbase, int quad, 
         return;
     }
     WaveReducer rd;
-    rd.red = lds + wave * 8 * RED_LD;
+    rd.red = lds + wave * 16 * RED_LD;
     rd.dst = nullptr;
     rd.cnt = 0;
     rd.lane = lane;
@@ -899,6 +906,16 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int rbase, int quad, 
         v0 = fma(p0, ampC[k], v0);
         v1 = fma(p0, ampS[k], v1);
     }
+    bool dk[K], dead = true;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int ex = xs_exp(x[k], y[k]);
+        dk[k] = !valid[k] || (e[k] != 0 && e[k] + ex < XS_DEEP);
+        dead = dead && dk[k];
+    }
+    // every term of the column is exactly zero while the wave stays deep from
+    // the seed to the last degree
+    dead = wave_all(dead);
     rd.put(0, v0, v1);
     rd.commit(1, dst + 2 * m);
     if (!PIPE) {
@@ -906,6 +923,7 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int rbase, int quad, 
         rd.finish();
     }
     if (m == nmax) {
+        if (A.dead_mode == 1 && lane == 0) A.dead[(size_t)m * A.ntile + tile] = dead ? 1 : 0;
         rd.begin();
         rd.finish();
         return;
@@ -921,12 +939,6 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int rbase, int quad, 
     st.lane = lane;
     st.start(m + 1);
     int n = m + 1;
-    bool dk[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        const int ex = xs_exp(x[k], y[k]);
-        dk[k] = !valid[k] || (e[k] != 0 && e[k] + ex < XS_DEEP);
-    }
     // phase 1: 16-degree tiles (two reducer tiles each) while a lane of the
     // wave still carries an exponent; an 8-degree tile at the end of the
     // column, where 16 degrees do not fit (the plain phase has no ldexp)
@@ -940,8 +952,12 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int rbase, int quad, 
             anyx = anyx || (e[k] != 0);
             deep = deep && dk[k];
         }
-        if (!wave_any(anyx)) break;
+        if (!wave_any(anyx)) {
+            dead = false;
+            break;
+        }
         const bool alldeep = wave_all(deep);
+        dead = dead && alldeep;
 #pragma unroll 1
         for (int h = 0; h < hlen; h += 8) {
             const int o = n + h - st.n0;
@@ -1005,6 +1021,14 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int rbase, int quad, 
 #pragma unroll
         for (int k = 0; k < K; k++) dk[k] = !valid[k] || (e[k] != 0 && d[k] < XS_DEEP);
     }
+    // the last (fewer than 8) degrees: zero terms if still deep
+    {
+        bool deep = true;
+#pragma unroll
+        for (int k = 0; k < K; k++) deep = deep && dk[k];
+        dead = dead && wave_all(deep);
+    }
+    if (A.dead_mode == 1 && lane == 0) A.dead[(size_t)m * A.ntile + tile] = dead ? 1 : 0;
     // phase 2: every lane in plain doubles
     for (; n + 7 <= nmax; n += 8) {
         if (n - st.n0 == 64) st.next();
@@ -1076,23 +1100,23 @@ __device__ __forceinline__ void sha_tile(const ShaArgs &A, int tile, int quad, i
     const int kt = tile_k<K>(A.nring, tile);
     if constexpr (K >= 2) {
         if (kt == 1) {
-            sha_item<1>(A, rbase, quad, slot, acc, reinterpret_cast<ShaSmem<1> &>(sm));
+            sha_item<1>(A, tile, rbase, quad, slot, acc, reinterpret_cast<ShaSmem<1> &>(sm));
             return;
         }
     }
     if constexpr (K >= 3) {
         if (kt == 2) {
-            sha_item<2>(A, rbase, quad, slot, acc, reinterpret_cast<ShaSmem<2> &>(sm));
+            sha_item<2>(A, tile, rbase, quad, slot, acc, reinterpret_cast<ShaSmem<2> &>(sm));
             return;
         }
     }
     if constexpr (K >= 4) {
         if (kt == 3) {
-            sha_item<3>(A, rbase, quad, slot, acc, reinterpret_cast<ShaSmem<3> &>(sm));
+            sha_item<3>(A, tile, rbase, quad, slot, acc, reinterpret_cast<ShaSmem<3> &>(sm));
             return;
         }
     }
-    sha_item<K>(A, rbase, quad, slot, acc, sm);
+    sha_item<K>(A, tile, rbase, quad, slot, acc, sm);
 }
 
 // Block b runs on XCD b mod 8 (round-robin dispatch).  XCD x takes the order

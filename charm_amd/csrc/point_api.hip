@@ -518,6 +518,81 @@ int ring_tiles(size_t nring, int K)
     return (int)((nring + 64 * (size_t)K - 1) / (64 * (size_t)K));
 }
 
+// Waves whose column is all zero.  A wave (order m, ring tile) whose rings
+// stay below 2^-1100 in every term of the column contributes exactly zero to
+// synthesis and analysis; which waves those are depends only on the rings,
+// the degree and the rings per lane K.  For a device-resident grid the first
+// transform at (K, nmax) records one flag per wave (kernels with dead_mode 1)
+// and later transforms skip the flagged waves (dead_mode 2), once an event
+// shows that the recording launch has completed.  Tables are only used with
+// the polar optimisation off and, for synthesis, r = R.
+struct DeadFlags {
+    struct Entry {
+        unsigned char *p = nullptr;
+        hipEvent_t done = nullptr;
+        bool recorded = false, ready = false;
+    };
+    std::mutex mu;
+    std::map<std::pair<int, unsigned long>, Entry> tab;
+
+    ~DeadFlags()
+    {
+        for (auto &kv : tab) {
+            if (kv.second.p) (void)hipFree(kv.second.p);
+            if (kv.second.done) (void)hipEventDestroy(kv.second.done);
+        }
+    }
+
+    // Flags for a launch with K rings per lane and ntile ring tiles; mode is
+    // the kernels' dead_mode (0 when no table applies).
+    unsigned char *acquire(int K, unsigned long nmax, int ntile, hipStream_t s, int &mode)
+    {
+        mode = 0;
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        const bool capturing =
+            hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone;
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = tab.find({K, nmax});
+        if (it == tab.end()) {
+            if (capturing) return nullptr;  // no allocation or recording in a graph
+            Entry e;
+            const size_t n = (size_t)(nmax + 1) * (size_t)ntile;
+            if (hipMalloc((void **)&e.p, n) != hipSuccess) {
+                (void)hipGetLastError();
+                return nullptr;
+            }
+            if (hipMemsetAsync(e.p, 0, n, s) != hipSuccess ||
+                hipEventCreateWithFlags(&e.done, hipEventDisableTiming) != hipSuccess) {
+                (void)hipGetLastError();
+                (void)hipStreamSynchronize(s);
+                (void)hipFree(e.p);
+                if (e.done) (void)hipEventDestroy(e.done);
+                return nullptr;
+            }
+            tab[{K, nmax}] = e;
+            mode = 1;
+            return e.p;
+        }
+        Entry &e = it->second;
+        if (!e.ready && e.recorded && !capturing && hipEventQuery(e.done) == hipSuccess)
+            e.ready = true;
+        if (!e.ready) return nullptr;
+        mode = 2;
+        return e.p;
+    }
+
+    // after the recording launch, on its stream
+    void recorded(int K, unsigned long nmax, hipStream_t s)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = tab.find({K, nmax});
+        if (it != tab.end() && !it->second.recorded) {
+            (void)hipEventRecord(it->second.done, s);
+            it->second.recorded = true;
+        }
+    }
+};
+
 // Lumped sums + longitude transform for the rings of a grid whose geometry
 // is already on the device.
 struct DevGrid {
@@ -529,13 +604,15 @@ struct DevGrid {
     const double *ratio;   // nullptr when all radii equal R
     const double *rp;
     size_t rld;
+    DeadFlags *dead = nullptr;  // device-resident grids only
 };
 
 // ---------------------------------------------------------------------------
 // Ring-tile autotuner: on the first large transform of a given kind, ring
 // count and degree, every candidate number of ring pairs per lane is timed
-// with HIP events on the caller's stream and the fastest is kept for the
-// process.  Small transforms and streams under graph capture use the
+// with HIP events on the caller's stream (after one warm-up launch each, the
+// fastest of three launches taken in interleaved rounds, so that clock drift
+// favours no candidate) and the fastest is kept for the process.  Small transforms and streams under graph capture use the
 // heuristic of ring_k.  CHARM_AMD_SHS_K / CHARM_AMD_SHA_K force a value.
 // ---------------------------------------------------------------------------
 std::mutex g_tune_mu;
@@ -570,14 +647,24 @@ int tuned_k(int kind, int nring, unsigned long nmax, int kmin, int kmax, hipStre
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
+    float kms[8];
     for (int k = kmin; k <= kmax; k++) {
         launch(k);  // warm-up (code load, caches)
-        HIPCHK(hipEventRecord(a, s));
-        launch(k);
-        HIPCHK(hipEventRecord(b, s));
-        HIPCHK(hipEventSynchronize(b));
-        float ms = 0.0f;
-        HIPCHK(hipEventElapsedTime(&ms, a, b));
+        kms[k - kmin] = 0.0f;
+    }
+    for (int round = 0; round < 3; round++) {
+        for (int k = kmin; k <= kmax; k++) {
+            HIPCHK(hipEventRecord(a, s));
+            launch(k);
+            HIPCHK(hipEventRecord(b, s));
+            HIPCHK(hipEventSynchronize(b));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, a, b));
+            if (round == 0 || ms < kms[k - kmin]) kms[k - kmin] = ms;
+        }
+    }
+    for (int k = kmin; k <= kmax; k++) {
+        const float ms = kms[k - kmin];
         if (std::getenv("CHARM_AMD_TUNE_LOG"))
             std::fprintf(stderr, "charm_amd tune: kind %d nring %d nmax %lu K=%d %.3f ms\n", kind,
                          nring, nmax, k, (double)ms);
@@ -636,8 +723,13 @@ void shs_grid_lumped(const GridGeom &g, const DevGrid &d, bool fft, unsigned lon
         HIPCHK((hipError_t)launch_shs_grid(y, stream));
     });
     a.ntile = ring_tiles(g.nring, a.k);
-    ProfScope ps("shs_grid_kernel", stream);
-    HIPCHK((hipError_t)launch_shs_grid(a, stream));
+    if (d.dead && a.unit && a.polar < 0.0)
+        a.dead = d.dead->acquire(a.k, nmax, a.ntile, stream, a.dead_mode);
+    {
+        ProfScope ps("shs_grid_kernel", stream);
+        HIPCHK((hipError_t)launch_shs_grid(a, stream));
+    }
+    if (a.dead_mode == 1) d.dead->recorded(a.k, nmax, stream);
 }
 
 void run_shs_grid(const GridGeom &g, const DevGrid &d, const double *lon_dev, bool use_fft,
@@ -1294,10 +1386,12 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
         HIPCHK((hipError_t)launch_sha(y, s));
     });
     group(a);
+    if (d.dead && a.polar < 0.0) a.dead = d.dead->acquire(a.k, nmax, a.ntile, s, a.dead_mode);
     {
         ProfScope ps("sha_kernel", s);
         HIPCHK((hipError_t)launch_sha(a, s));
     }
+    if (a.dead_mode == 1) d.dead->recorded(a.k, nmax, s);
     {
         ProfScope ps("sha_reduce_kernel", s);
         const int nused = (a.ntile + a.tpg - 1) / a.tpg;
@@ -1388,6 +1482,7 @@ struct charm_amd_grid {
     unsigned char *mirror = nullptr;
     size_t sld = 0;
     std::vector<double> wh;
+    mutable charm_amd::DeadFlags dead;  // a cache: logically const
 };
 
 using namespace charm_amd;
@@ -1658,7 +1753,8 @@ void charm_amd_shs_grid_dev(const charm_amd_grid *grd, unsigned long nmax, doubl
     try {
         hipStream_t s = (hipStream_t)stream;
         Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
-        DevGrid d{grd->t, grd->u, grd->mirror, grd->sx, grd->six, grd->sld, nullptr, nullptr, 0};
+        DevGrid d{grd->t, grd->u, grd->mirror, grd->sx, grd->six, grd->sld, nullptr, nullptr, 0,
+                  &grd->dead};
         DevBuf ratio, rp;
         if (!nearly_equal(grd->r0, R, charm_glob_threshold)) {
             // one radius for all rows: a single column of powers (row stride 0)
@@ -1705,7 +1801,8 @@ void charm_amd_sha_grid_dev(const charm_amd_grid *grd, unsigned long nmax, doubl
             dsc.alloc(sc.size() * sizeof(double), s);
             h2d(dsc.as<double>(), sc.data(), sc.size(), s);
         }
-        DevGrid d{grd->t, grd->u, grd->mirror, grd->sx, grd->six, grd->sld, nullptr, nullptr, 0};
+        DevGrid d{grd->t, grd->u, grd->mirror, grd->sx, grd->six, grd->sld, nullptr, nullptr, 0,
+                  &grd->dead};
         run_sha(grd->g, d, dcw.as<double>(), f_dev, nmax, tb, dsc.as<double>(), nmax, cs_dev,
                 cs_dev + 1, 2, s);
         HIPCHK(hipGetLastError());
@@ -1823,7 +1920,8 @@ void charm_amd_shs_grid_deriv_dev(const charm_amd_grid *grd, unsigned long nmax,
     try {
         hipStream_t s = (hipStream_t)stream;
         Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
-        DevGrid d{grd->t, grd->u, grd->mirror, grd->sx, grd->six, grd->sld, nullptr, nullptr, 0};
+        DevGrid d{grd->t, grd->u, grd->mirror, grd->sx, grd->six, grd->sld, nullptr, nullptr, 0,
+                  &grd->dead};
         DevBuf ratio, rp;
         const bool unit = nearly_equal(grd->r0, R, charm_glob_threshold);
         if (!unit) {

@@ -295,6 +295,46 @@ def test_device_grid_radius_differs_from_R():
     assert np.array_equal(f.numpy(), f_host)
 
 
+@pytest.mark.parametrize("K", (2, 3))
+def test_device_grid_skips_all_zero_waves(K, monkeypatch):
+    """Repeated transforms on one device grid skip the waves (order, ring
+    tile) whose every term the first transform found to be exactly zero.  At
+    nmax = 1600 several hundred waves near the poles are such.  Synthesis:
+    every call equals the host entry point (which never skips) bit for bit.
+    Analysis on a fresh grid: the first call records the flags without
+    skipping, later calls skip and must equal it bit for bit."""
+    from charm_amd.device import DevArray, DeviceGrid, Stream
+
+    monkeypatch.setenv("CHARM_AMD_SHS_K", str(K))
+    monkeypatch.setenv("CHARM_AMD_SHA_K", str(K))
+    N = 1600
+    C, S = kaula_coeffs(N, 33)
+    g = crd.PointGridGL(N, R_EARTH)
+    f_host = shs.point(g, model(N, C, S, GM, R_EARTH), N)
+    st = Stream()
+    cs = DevArray.from_numpy(np.stack([C, S], axis=1))
+    dg = DeviceGrid(N, R_EARTH)
+    f = DevArray((dg.nlat, dg.nlon))
+    for call in range(3):
+        dg.shs(cs, N, GM, R_EARTH, f, st)
+        st.synchronize()
+        assert np.array_equal(f.numpy(), f_host), "synthesis call %d" % call
+    dg2 = DeviceGrid(N, R_EARTH)
+    fd = DevArray.from_numpy(f_host)
+    out = DevArray(cs.shape)
+    first = None
+    for call in range(3):
+        dg2.sha(fd, N, GM, R_EARTH, out, st)
+        st.synchronize()
+        o = out.numpy()
+        if first is None:
+            first = o
+            assert np.abs(o[:, 0] - C).max() <= TOL * np.abs(C).max()
+            assert np.abs(o[:, 1] - S).max() <= TOL * np.abs(C).max()
+        else:
+            assert np.array_equal(o, first), "analysis call %d" % call
+
+
 @pytest.mark.parametrize("nlon", (20000, 21602, 24570))
 def test_split_row_lengths_synthesis(orc, nlon):
     """Rows longer than one LDS transform (L = 10000 = 2 x 5000, 10801 =
