@@ -48,11 +48,19 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X vector FP64 (also its FP64 matrix peak)
 UNIT = "(nmax+1)^2*nlat/s"
-# HBM bytes per launch at nmax = 2160 (tuned ring tiles K = 3 for analysis,
-# K = 2 for synthesis), measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE in separate runs and scaled by a 256 MiB device copy measured in
-# the same runs (HBM bytes = 2.0 FETCH_SIZE + 1.0 WRITE_SIZE).
-TRAFFIC_GL2160 = {"sha_kernel": 405.6e6, "shs_grid_kernel": 165.4e6}
+# HBM bytes per launch at nmax = 2160 for each ring-pairs-per-lane K the
+# tuner can pick, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in
+# separate runs and scaled by a 256 MiB device copy measured in the same runs
+# (HBM bytes = 2.0 FETCH_SIZE + 1.0 WRITE_SIZE).  Analysis writes one partial
+# triangle per ring tile, so its traffic falls with K.
+TRAFFIC_GL2160 = {("sha_kernel", 2): 688.6e6, ("sha_kernel", 3): 453.0e6,
+                  ("sha_kernel", 4): 367.0e6, ("shs_grid_kernel", 1): 159.8e6,
+                  ("shs_grid_kernel", 2): 157.9e6, ("shs_grid_kernel", 3): 165.2e6}
+
+
+def last_ring_k(lib, name):
+    """Ring pairs per lane of the last launch of a grid kernel."""
+    return lib.charm_amd_last_ring_k(0 if name == "shs_grid_kernel" else 1)
 
 
 def kernel_ms(lib, name):
@@ -158,8 +166,9 @@ def gl_roundtrip(args, N, with_sctr, with_cpu):
         "roofline": {"bound": "mfma", "compute": "vector FP64 (VALU)", "kernel": dom,
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,
+                     "ring_pairs_per_lane": last_ring_k(lib, dom),
                      "traffic": args.traffic if args.traffic is not None else
-                     (TRAFFIC_GL2160.get(dom) if N == 2160 else None),
+                     (TRAFFIC_GL2160.get((dom, last_ring_k(lib, dom))) if N == 2160 else None),
                      "flops_per_launch": flops,
                      "frac_by_kernel": {k: flops / (avg[k] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
                                         for k in ("shs_grid_kernel", "sha_kernel")}},
@@ -275,7 +284,8 @@ def grad_bench(args):
         "roofline": {"bound": "mfma", "compute": "vector FP64 (VALU)", "kernel": "shs_grid_kernel",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": TRAFFIC_GL2160["shs_grid_kernel"], "flops_per_launch": flops},
+                     "traffic": TRAFFIC_GL2160.get(("shs_grid_kernel", last_ring_k(lib, "shs_grid_kernel"))),
+                     "flops_per_launch": flops},
     })
     del f6, f1
     # grad1 at scattered points

@@ -11,6 +11,7 @@
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -617,6 +618,7 @@ struct DevGrid {
 // ---------------------------------------------------------------------------
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int>, int> g_tuned;
+std::atomic<int> g_last_k[2];
 
 int forced_k(const char *env, int kmin, int kmax)
 {
@@ -723,6 +725,7 @@ void shs_grid_lumped(const GridGeom &g, const DevGrid &d, bool fft, unsigned lon
         HIPCHK((hipError_t)launch_shs_grid(y, stream));
     });
     a.ntile = ring_tiles(g.nring, a.k);
+    g_last_k[0].store(a.k);
     if (d.dead && a.unit && a.polar < 0.0)
         a.dead = d.dead->acquire(a.k, nmax, a.ntile, stream, a.dead_mode);
     {
@@ -1386,6 +1389,7 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
         HIPCHK((hipError_t)launch_sha(y, s));
     });
     group(a);
+    g_last_k[1].store(a.k);
     if (d.dead && a.polar < 0.0) a.dead = d.dead->acquire(a.k, nmax, a.ntile, s, a.dead_mode);
     {
         ProfScope ps("sha_kernel", s);
@@ -1505,6 +1509,11 @@ void charm_amd_prof_reset(void)
     }
     g_prof_pending.clear();
     g_prof_acc.clear();
+}
+
+int charm_amd_last_ring_k(int kind)
+{
+    return (kind == 0 || kind == 1) ? g_last_k[kind].load() : 0;
 }
 
 int charm_amd_prof_read(const char *name, double *total_ms, long *count)

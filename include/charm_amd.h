@@ -398,6 +398,10 @@ int charm_amd_device_count(void);
 void charm_amd_prof_enable(int on);
 void charm_amd_prof_reset(void);
 int charm_amd_prof_read(const char *name, double *total_ms, long *count);
+/* Ring pairs per lane of the last grid synthesis (kind 0) or analysis
+ * (kind 1) launch in this process, as chosen by the ring-tile tuner (0 before
+ * the first launch).  Diagnostic; no counterpart in CHarm. */
+int charm_amd_last_ring_k(int kind);
 
 #ifdef __cplusplus
 }
