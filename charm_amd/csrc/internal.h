@@ -51,7 +51,9 @@ struct ShsGridArgs {
     const unsigned char *mirror;  // [nring] 1 if row nlat-1-i is the mirror ring
     const double *al;  // alpha_nm of the scaled recurrence, m-major, degree al_nt
     int al_nt;
-    const void *cs;    // (C_nm s_nm, S_nm s_nm), m-major, degree nmax
+    const void *cs;    // (C_nm, S_nm) times s_nm, m-major, degree nmax; see sn
+    const double *sn;  // s_nm (layout of al): cs holds unscaled pairs, scaled as
+                       // they are staged; nullptr: cs is already scaled
     void *out;         // [row * ldo + m] double2
     size_t ldo;
     double mur;        // mu / R
@@ -102,7 +104,8 @@ struct SctrArgs {
     const double *dm;     // [nmax+1]
     const double *al;     // alpha_nm, m-major, degree al_nt
     int al_nt;
-    const void *cs;       // (C_nm s_nm, S_nm s_nm), m-major, degree nmax
+    const void *cs;       // (C_nm, S_nm) times s_nm, m-major, degree nmax; see sn
+    const double *sn;     // as in ShsGridArgs
     double *part;         // [chunk * npoint + i]
     int k;                // points per lane (1, 2, 4)
     double polar;         // polar optimisation threshold (< 0: off)

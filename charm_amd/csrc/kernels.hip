@@ -149,6 +149,7 @@ struct ColStage {
     double2 *cs;         // LDS window [64] (CS)
     const double *gal;   // entry n of the column
     const double2 *gcs;  // entry n of the column (CS)
+    const double *gsn;   // entry n of the column of s_nm (CS; nullptr: gcs is scaled)
     int nlast;           // last degree of the column
     int n0;              // first degree of the window in LDS
     int lane;
@@ -160,7 +161,14 @@ struct ColStage {
         const int n = nb + lane;
         const bool ok = n <= nlast;
         pal = ok ? gal[n] : 0.0;
-        if (CS) pcs = ok ? gcs[n] : make_double2(0.0, 0.0);
+        if (CS) {
+            pcs = ok ? gcs[n] : make_double2(0.0, 0.0);
+            if (gsn) {
+                const double f = ok ? gsn[n] : 0.0;
+                pcs.x *= f;
+                pcs.y *= f;
+            }
+        }
     }
     __device__ __forceinline__ void store()
     {
@@ -430,6 +438,8 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
             st.cs = sm.cs[wave];
             st.gal = A.al + dcol_off(A.al_nt, m) - m;
             st.gcs = csm;
+            // s_m = 1: the seed term at degree m needs no scaling
+            st.gsn = A.sn ? A.sn + dcol_off(A.al_nt, m) - m : nullptr;
             st.nlast = nmax;
             st.lane = lane;
             st.start(m + 1);
@@ -1284,6 +1294,8 @@ __device__ __forceinline__ void sctr_item(const SctrArgs &A, int blk, int chunk,
         if (m < nmax && !allskip) {
             st.gal = A.al + dcol_off(A.al_nt, m) - m;
             st.gcs = csm;
+            // s_m = 1: the seed term at degree m needs no scaling
+            st.gsn = A.sn ? A.sn + dcol_off(A.al_nt, m) - m : nullptr;
             st.start(m + 1);
 #define SC_REC(k, al)                                                      \
     (UNIT ? fma((al) * x[k], tq[k], -y[k]) : fma((al) * x[k], tq[k], -(q2[k] * y[k])))

@@ -261,7 +261,7 @@ struct ProfRec {
     hipEvent_t a, b;
 };
 std::mutex g_prof_mu;
-bool g_prof_on = false;
+int g_prof_on = 0;  // 0 off, 1 every named kernel, 2 the recurrence kernels only
 std::vector<ProfRec> g_prof_pending;
 std::map<std::string, std::pair<double, long>> g_prof_acc;
 
@@ -270,12 +270,15 @@ struct ProfScope {
     hipStream_t s;
     ProfScope(const char *name, hipStream_t st) : s(st)
     {
-        bool on;
+        int on;
         {
             std::lock_guard<std::mutex> lk(g_prof_mu);
             on = g_prof_on;
         }
-        if (!on) return;
+        if (on == 0) return;
+        if (on == 2 && std::strcmp(name, "shs_grid_kernel") != 0 && std::strcmp(name, "sha_kernel") != 0 &&
+            std::strcmp(name, "sctr_kernel") != 0)
+            return;
         rec.name = name;
         HIPCHK(hipEventCreate(&rec.a));
         HIPCHK(hipEventCreate(&rec.b));
@@ -709,10 +712,9 @@ void shs_grid_lumped(const GridGeom &g, const DevGrid &d, bool fft, unsigned lon
     a.al_nt = tb.nt;
     a.u = d.u;
     a.polar = polar_threshold(nmax);
-    // coefficients times s_nm for the scaled recurrence
-    DevBuf cs2(tri_len(nmax) * 2 * sizeof(double), stream);
-    HIPCHK((hipError_t)launch_scale_cs((int)nmax, cs_dev, tb.sn, tb.nt, cs2.p, stream));
-    a.cs = cs2.p;
+    // coefficients times s_nm for the scaled recurrence, as the kernel stages them
+    a.cs = cs_dev;
+    a.sn = tb.sn;
     a.mur = mur;
     a.out = out;
     a.ldo = ld;
@@ -894,8 +896,6 @@ void run_sctr(size_t npoint, const double *lat_dev, const double *lon_dev, const
               const Tables &tb, double *f_dev, hipStream_t s)
 {
     const int K = sctr_k();
-    DevBuf cs2(tri_len(nmax) * 2 * sizeof(double), s);
-    HIPCHK((hipError_t)launch_scale_cs((int)nmax, cs_dev, tb.sn, tb.nt, cs2.p, s));
     DevBuf dt(npoint * sizeof(double), s), du(npoint * sizeof(double), s);
     DevBuf dl, dr, perm;
     const double *lon_use = lon_dev, *r_use = r_dev;
@@ -930,7 +930,8 @@ void run_sctr(size_t npoint, const double *lat_dev, const double *lon_dev, const
     a.dm = tb.dm;
     a.al = tb.al;
     a.al_nt = tb.nt;
-    a.cs = cs2.p;
+    a.cs = cs_dev;
+    a.sn = tb.sn;
     a.part = part.as<double>();
     a.k = K;
     a.polar = polar_threshold(nmax);
@@ -1373,9 +1374,14 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
     // Ring tiles are grouped so that the launch has about SHA_BLOCKS blocks:
     // each block adds its group's tiles, in order, into one partial slot.
     // Few slots keep the partial sums small at high degree (one per tile
-    // would be 20-40 GB at nmax = 10800).
+    // would be 20-40 GB at nmax = 10800).  From SHA_BLOCKS / 2 order quads on,
+    // one slot (every block runs all tiles of its quad): at nmax = 10800 that
+    // is 5% faster than two slots (half the partial-sum writes, one slot to
+    // reduce), while at nmax = 2160 fewer, longer blocks are 10% slower.
     const int nquad = (int)(nmax + 1 + 3) / 4;
-    const int nslot = std::max(1, std::min(ring_tiles(g.nring, 2), (SHA_BLOCKS + nquad - 1) / nquad));
+    const int nslot = nquad >= SHA_BLOCKS / 2
+                          ? 1
+                          : std::max(1, std::min(ring_tiles(g.nring, 2), (SHA_BLOCKS + nquad - 1) / nquad));
     auto group = [&](ShaArgs &y) {
         y.ntile = ring_tiles(g.nring, y.k);
         y.tpg = (y.ntile + nslot - 1) / nslot;
@@ -1496,7 +1502,7 @@ extern "C" {
 void charm_amd_prof_enable(int on)
 {
     std::lock_guard<std::mutex> lk(g_prof_mu);
-    g_prof_on = on != 0;
+    g_prof_on = (on == 2) ? 2 : (on != 0 ? 1 : 0);
 }
 
 void charm_amd_prof_reset(void)

@@ -392,7 +392,9 @@ int charm_amd_device_count(void);
 
 /* Kernel timing: when enabled, HIP events on the launching stream bracket
  * each launch of the named kernels ("shs_grid_kernel", "fft_c2r",
- * "sha_kernel", "sha_reduce_kernel", "fft_r2c", "sctr_kernel").
+ * "sha_kernel", "sha_reduce_kernel", "fft_r2c", "sctr_kernel"); on = 2
+ * brackets only the recurrence kernels ("shs_grid_kernel", "sha_kernel",
+ * "sctr_kernel"), each event pair adding a few microseconds between kernels.
  * charm_amd_prof_read waits for the recorded events and returns the summed
  * milliseconds and the launch count of one kernel. */
 void charm_amd_prof_enable(int on);
