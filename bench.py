@@ -692,7 +692,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="gl2160",
                     choices=("gl2160", "gl10800", "batch", "multi", "grad", "io", "gfm",
                              "serve"))
