@@ -1911,8 +1911,8 @@ int launch_zero_ints(int *p, int n, void *stream)
     return (int)hipGetLastError();
 }
 
-constexpr int BATCH_SHS_K = 2;
-constexpr int BATCH_SHA_K = 3;
+constexpr int BATCH_SHS_K = 3;
+constexpr int BATCH_SHA_K = 4;
 constexpr int BATCH_SCTR_K = 2;
 
 int batch_ring_k(int kind)
