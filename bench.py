@@ -53,9 +53,9 @@ UNIT = "(nmax+1)^2*nlat/s"
 # separate runs and scaled by a 256 MiB device copy measured in the same runs
 # (HBM bytes = 2.0 FETCH_SIZE + 1.0 WRITE_SIZE).  Analysis writes one partial
 # triangle per ring tile, so its traffic falls with K.
-TRAFFIC_GL2160 = {("sha_kernel", 2): 688.6e6, ("sha_kernel", 3): 453.0e6,
-                  ("sha_kernel", 4): 367.0e6, ("shs_grid_kernel", 1): 159.8e6,
-                  ("shs_grid_kernel", 2): 157.9e6, ("shs_grid_kernel", 3): 165.2e6}
+TRAFFIC_GL2160 = {("sha_kernel", 2): 688.9e6, ("sha_kernel", 3): 453.1e6,
+                  ("sha_kernel", 4): 366.9e6, ("shs_grid_kernel", 1): 178.6e6,
+                  ("shs_grid_kernel", 2): 177.4e6, ("shs_grid_kernel", 3): 190.2e6}
 
 
 def last_ring_k(lib, name):
