@@ -370,6 +370,7 @@ struct SplitArgs {
     size_t ldf;
     cplx *spec;
     size_t lds;
+    size_t nrow;
 };
 
 // Input n of the inverse transform (c2r_input for the full row length)
@@ -389,9 +390,15 @@ template <int MODE, bool C2R>
 __global__ __launch_bounds__(1024) void split_a_kernel(SplitArgs a)
 {
     cplx *x = lds_rows;
-    const size_t row = blockIdx.x;
-    const int n1 = blockIdx.y;
     const int L = a.L, L1 = a.L1, L2 = a.L2;
+    // Block b runs on XCD b mod 8.  A group of 8 L1 consecutive blocks holds
+    // rows 8g..8g+7, and the L1 column blocks of one row share b mod 8: they
+    // gather the same cache lines of their row (stride L1) through one L2,
+    // at about the same time.
+    const unsigned b = blockIdx.x, span = 8u * (unsigned)L1;
+    const size_t row = (size_t)(b / span) * 8 + (b % span) % 8;
+    const int n1 = (int)((b % span) / 8);
+    if (row >= a.nrow) return;
     const fft::TwSplit tw = lds_twiddles(a.sub.tw, a.sub.plan.M);
     const cplx *X = a.spec + row * a.lds;
     const cplx *z = (const cplx *)(a.f + row * a.ldf);
@@ -737,7 +744,9 @@ int launch_row_fft(int c2r, size_t nlon, size_t nrow, double *f, size_t ldf, voi
         a.spec = (cplx *)spec;
         a.lds = lds;
         const size_t sh = lds_bytes(p.mp.M);
-        const dim3 ga((unsigned)nrow, (unsigned)sp.L1), ba((unsigned)p.threads);
+        a.nrow = nrow;
+        // one block per (row, column n1), rows padded to a multiple of 8
+        const dim3 ga((unsigned)(((nrow + 7) / 8) * 8 * (size_t)sp.L1)), ba((unsigned)p.threads);
 #define SPLIT_A(C2R)                                                                               \
     switch (p.mp.mode) {                                                                           \
     case fft::DFT_DIRECT:                                                                          \
