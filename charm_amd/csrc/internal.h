@@ -88,6 +88,13 @@ struct ShaArgs {
     double polar;      // polar optimisation threshold (< 0: off)
     unsigned char *dead;  // as in ShsGridArgs
     int dead_mode;
+    // Fused finish (fc != nullptr, one partial slot): the block of each order
+    // quad applies s_nm and scale[n] to its slot and writes the coefficients
+    // of degree nmax (what sha_reduce_kernel does otherwise).
+    const double *sn;     // s_nm, m-major, degree al_nt
+    const double *scale;  // [nmax + 1] or nullptr
+    double *fc, *fs;      // outputs at (dcol_off(nmax, m) + n - m) * fstride
+    size_t fstride;
 };
 
 struct SctrArgs {

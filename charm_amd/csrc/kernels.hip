@@ -1133,6 +1133,40 @@ __device__ __forceinline__ void sha_tile(const ShaArgs &A, int tile, int quad, i
 // quad pairs p = 8g + x; its consecutive blocks alternate between the two
 // quads of a pair for each ring tile, so the two 64-byte halves of every
 // 128-byte spectrum line are read through the same L2.
+// Coefficients of order quad `quad` from the ngrp partial slots, added in
+// slot order (as sha_reduce_kernel does for a whole triangle).  Degrees above
+// nmax are not written (the fused finish requires nmax_out == nmax).
+__device__ void sha_finish_quad(const ShaArgs &A, int quad, int ngrp)
+{
+    const int nmax = A.nmax;
+    const size_t P = tri_len_dev(nmax);
+    const double2 *part = (const double2 *)A.part;
+    for (int w = 0; w < 4; w++) {
+        const int m = quad * 4 + w;
+        if (m > nmax) break;
+        const size_t col = dcol_off(nmax, m) - m;
+        const double *snm = A.sn + dcol_off(A.al_nt, m) - m;
+        for (int n = m + (int)threadIdx.x; n <= nmax; n += blockDim.x) {
+            double vc = 0.0, vs = 0.0;
+            for (int sl = 0; sl < ngrp; sl++) {
+                const double2 p = part[(size_t)sl * P + col + n];
+                vc += p.x;
+                vs += p.y;
+            }
+            const double f = snm[n];
+            vc *= f;
+            vs *= f;
+            if (A.scale) {
+                vc *= A.scale[n];
+                vs *= A.scale[n];
+            }
+            if (m == 0) vs = 0.0;
+            A.fc[(col + n) * A.fstride] = vc;
+            A.fs[(col + n) * A.fstride] = vs;
+        }
+    }
+}
+
 template <int K>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 3 ? 4 : 3)))
 void sha_kernel(ShaArgs A)
@@ -1147,6 +1181,14 @@ void sha_kernel(ShaArgs A)
         if (tile > grp * A.tpg) __syncthreads();  // the block's LDS is reused
         sha_tile<K>(A, tile, quad, grp, tile > grp * A.tpg, sm);
     }
+    // One slot: this block alone summed the quad's columns; its threads read
+    // back what the block wrote (workgroup scope).  With more slots a
+    // last-block reduction would need a device-scope fence per block, whose L2
+    // write-back and invalidation cost more than sha_reduce_kernel (0.61 ->
+    // 0.96 ms at nmax 2160).
+    if (!A.fc || ngrp != 1 || quad * 4 > A.nmax) return;
+    __syncthreads();
+    sha_finish_quad(A, quad, 1);
 }
 
 template <int K>

@@ -1393,6 +1393,10 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
         y.ntile = ring_tiles(g.nring, y.k);
         y.tpg = (y.ntile + nslot - 1) / nslot;
     };
+    // With one slot and the output at the transform's degree the analysis
+    // kernel finishes each order quad itself; otherwise sha_reduce_kernel adds
+    // the slots (and zero-fills degrees above nmax).
+    const bool fuse = nslot == 1 && nmax_out == nmax;
     DevBuf part((size_t)nslot * tri_len(nmax) * 2 * sizeof(double), s);
     a.part = part.p;
     a.k = tuned_k(1, (int)g.nring, nmax, 2, 4, s, [&](int k) {
@@ -1403,13 +1407,20 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
     });
     group(a);
     g_last_k[1].store(a.k);
+    if (fuse) {
+        a.sn = tb.sn;
+        a.scale = scale_dev;
+        a.fc = c_dev;
+        a.fs = s_dev;
+        a.fstride = stride;
+    }
     if (d.dead && a.polar < 0.0) a.dead = d.dead->acquire(a.k, nmax, a.ntile, s, a.dead_mode);
     {
         ProfScope ps("sha_kernel", s);
         HIPCHK((hipError_t)launch_sha(a, s));
     }
     if (a.dead_mode == 1) d.dead->recorded(a.k, nmax, s);
-    {
+    if (!fuse) {
         ProfScope ps("sha_reduce_kernel", s);
         const int nused = (a.ntile + a.tpg - 1) / a.tpg;
         HIPCHK((hipError_t)launch_sha_reduce((int)nmax, nused, part.p, tb.sn, tb.nt,
