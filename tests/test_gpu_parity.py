@@ -335,6 +335,34 @@ def test_device_grid_skips_all_zero_waves(K, monkeypatch):
             assert np.array_equal(o, first), "analysis call %d" % call
 
 
+def test_device_grid_one_slot_rescale_roundtrip():
+    """From 2048 order quads on (nmax >= 8188) analysis keeps one partial slot
+    and its blocks apply s_nm and the rescale (r0/R)^n themselves.  Round trip
+    at nmax = 8200 on a GL grid of radius R + 500 m: synthesis, then analysis
+    with the rescale, recovers the coefficients.  (R/r0)^n stays above 0.52 up
+    to n = 8200, so the round trip is well conditioned, while a rescale left
+    out would be off by 4e-10 of max|C| already at degree 2.  (At R + 5 km the
+    factor reaches 1.6e-3 and both analysis paths, one slot or several, lose
+    1e-11 of max|C| in the high zonal degrees.)"""
+    from charm_amd.device import DevArray, DeviceGrid, Stream
+
+    N = 8200
+    C, S = kaula_coeffs(N, 8200)
+    r0 = R_EARTH + 500.0
+    dg = DeviceGrid(N, r0)
+    st = Stream()
+    cs = DevArray.from_numpy(np.stack([C, S], axis=1))
+    f = DevArray((dg.nlat, dg.nlon))
+    dg.shs(cs, N, GM, R_EARTH, f, st)
+    out = DevArray(cs.shape)
+    dg.sha(f, N, GM, R_EARTH, out, st)
+    st.synchronize()
+    o = out.numpy()
+    err = max(np.abs(o[:, 0] - C).max(), np.abs(o[:, 1] - S).max()) / np.abs(C).max()
+    print("nmax 8200, r0 = R + 500 m: round trip %.2e" % err)
+    assert err <= TOL
+
+
 @pytest.mark.parametrize("nlon", (20000, 21602, 24570))
 def test_split_row_lengths_synthesis(orc, nlon):
     """Rows longer than one LDS transform (L = 10000 = 2 x 5000, 10801 =
