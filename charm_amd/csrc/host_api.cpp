@@ -20,6 +20,14 @@ double charm_glob_threshold = 100.0 * 2.220446049250313080847263336181640625e-16
 double charm_glob_threshold2 = 100000.0 * 2.220446049250313080847263336181640625e-16;
 unsigned long charm_glob_polar_optimization_a1 = 100;
 double charm_glob_polar_optimization_a2 = -1.0;
+// Ring blocks of the polar optimisation: CHarm tests it jointly over
+// SIMD_SIZE * BLOCK_S rings (synthesis) or SIMD_SIZE * BLOCK_A rings
+// (analysis), with BLOCK_S/BLOCK_A = glob_{shs,sha}_block_lat_multiplier
+// (src/glob/glob.c:26-29, src/simd/simd.h:298-304,609-611).  The defaults are
+// the AVX-512 double build: 8 lanes, multipliers 4 and 8, blocks of 32 and 64.
+size_t charm_glob_sha_block_lat_multiplier = 8;
+size_t charm_glob_shs_block_lat_multiplier = 4;
+int charm_amd_polar_simd_size = 8;
 }
 
 namespace charm_amd {
@@ -29,6 +37,44 @@ double polar_threshold(unsigned long nmax)
     if (charm_glob_polar_optimization_a2 < 0.0) return -1.0;
     return std::max((double)charm_glob_polar_optimization_a1,
                     charm_glob_polar_optimization_a2 * (double)nmax);
+}
+
+size_t polar_block(int analysis)
+{
+    const size_t v = charm_amd_polar_simd_size > 0 ? (size_t)charm_amd_polar_simd_size : 1;
+    const size_t b = analysis ? charm_glob_sha_block_lat_multiplier
+                              : charm_glob_shs_block_lat_multiplier;
+    return v * (b > 0 ? b : 1);
+}
+
+// First order skipped by the polar optimisation at a block whose largest
+// cos(lat) is umax: misc_polar_optimization_apply skips m when
+// thr < m - nmax * umax (src/misc/misc_polar_optimization_apply.c:30-37),
+// with the product and the difference rounded separately as CHarm's SIMD
+// code does.  Both roundings are monotone, so the skipped orders of a block
+// are m0..nmax; returns nmax + 1 when none is skipped or thr < 0.
+int polar_first_skip(unsigned long nmax, double thr, double umax)
+{
+    if (thr < 0.0) return (int)nmax + 1;
+    const double c = (double)nmax * umax;
+    long m = (long)std::floor(thr + c);
+    if (m < 0) m = 0;
+    if (m > (long)nmax + 1) m = (long)nmax + 1;
+    while (m > 0 && thr < (double)(m - 1) - c) m--;
+    while (m <= (long)nmax && !(thr < (double)m - c)) m++;
+    return (int)m;
+}
+
+void polar_ring_m0(const double *u, size_t nring, unsigned long nmax, double thr, size_t block,
+                   int *m0)
+{
+    for (size_t b0 = 0; b0 < nring; b0 += block) {
+        const size_t b1 = std::min(nring, b0 + block);
+        double umax = u[b0];
+        for (size_t i = b0 + 1; i < b1; i++) umax = std::max(umax, u[i]);
+        const int v = polar_first_skip(nmax, thr, umax);
+        for (size_t i = b0; i < b1; i++) m0[i] = v;
+    }
 }
 
 // misc_is_nearly_equal (src/misc/misc_is_nearly_equal.c:16-32)

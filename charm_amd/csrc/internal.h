@@ -4,6 +4,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "charm_amd.h"
 
@@ -59,8 +60,8 @@ struct ShsGridArgs {
     double mur;        // mu / R
     int fft_layout;    // 1: out = Hermitian half-spectrum for c2r; 0: raw (a, b)
     int k;             // ring pairs per lane (1..3)
-    const double *u;   // [nring] cos(lat)
-    double polar;      // polar optimisation threshold (< 0: off)
+    const int *pm0;    // polar optimisation: [nring] first order skipped at the
+                       // ring (its CHarm ring block); nullptr: off
     unsigned char *dead;  // [m * ntile + tile]: 1 if every term of the wave's
                           // column is exactly zero (see dead_mode)
     int dead_mode;     // 0: unused; 1: record the flags; 2: skip flagged waves
@@ -84,8 +85,7 @@ struct ShaArgs {
     void *part;        // [slot][tri_len(nmax)] double2 partial sums (before s_nm)
     int k;             // ring pairs per lane (2..4)
     int tpg;           // ring tiles per block, added in tile order into slot tile / tpg
-    const double *u;   // [nring] cos(lat)
-    double polar;      // polar optimisation threshold (< 0: off)
+    const int *pm0;    // as in ShsGridArgs (analysis ring blocks)
     unsigned char *dead;  // as in ShsGridArgs
     int dead_mode;
     // Fused finish (fc != nullptr, one partial slot): the block of each order
@@ -115,7 +115,11 @@ struct SctrArgs {
     const double *sn;     // as in ShsGridArgs
     double *part;         // [chunk * npoint + i]
     int k;                // points per lane (1, 2, 4)
-    double polar;         // polar optimisation threshold (< 0: off)
+    const int *pm0;       // polar optimisation: first order skipped per point
+                          // (its CHarm point block), in the order of lat_t; nullptr: off
+    const int *unit_dev;  // non-null: the radii test was made on the device and
+                          // unit is taken from *unit_dev (both variants are
+                          // launched; the one that does not apply returns at once)
 };
 
 // Many coefficient sets on one grid (multi.hip): synthesis by FP64 MFMA.
@@ -139,8 +143,7 @@ struct MultiShsArgs {
     void *spec;           // [set][row][lds] double2
     size_t lds;
     double mur;
-    const double *u;      // [nring] cos(lat)
-    double polar;         // polar optimisation threshold (< 0: off)
+    const int *pm0;       // as in ShsGridArgs; nullptr: off
 };
 int launch_shs_multi(const MultiShsArgs &a, void *stream);
 int launch_scale_cs_multi(int nmax, size_t nset, const void *cs_dev, const double *sn_dev, int nt,
@@ -224,7 +227,33 @@ int launch_sctr_reduce(size_t npoint, int nchunk, const double *part_dev, const 
 int launch_sort_points(size_t n, const double *lat_dev, const double *lon_dev, const double *r_dev,
                        int *perm_dev, double *t_dev, double *u_dev, double *lon_out_dev,
                        double *r_out_dev, void *stream);
+// The same in a caller's workspace of ws_bytes >= sort_points_ws_bytes(n)
+// (0: the size query failed); the permutation ends in perm_dev or in the
+// workspace, and *perm_out names it.
+size_t sort_points_ws_bytes(size_t n);
+int launch_sort_points_ws(size_t n, const double *lat_dev, const double *lon_dev,
+                          const double *r_dev, int *perm_dev, void *ws, size_t ws_bytes,
+                          double *t_dev, double *u_dev, double *lon_out_dev, double *r_out_dev,
+                          const int **perm_out, void *stream);
+// *flag_dev = 1 if every x[i] is nearly equal to ref (misc_is_nearly_equal),
+// else 0, decided on the device
+int launch_all_near_dev(size_t n, const double *x_dev, double ref, double eps, int *flag_dev,
+                        void *stream);
+// p_dev[0..n) = v[0..n), n <= 72, from kernel arguments (no host copy)
+struct IntVals {
+    int n;
+    int v[72];
+};
+int launch_set_ints(int *p_dev, const IntVals &v, void *stream);
 int sctr_points_per_block(int k);
+// Polar optimisation at scattered points: m0[i] = first order skipped at the
+// i-th point in the order of perm (nullptr: input order), for CHarm's blocks
+// of bs consecutive input points, threshold thr >= 0.
+int launch_polar_points(size_t n, const double *lat_dev, const int *perm_dev, size_t bs, int nmax,
+                        double thr, int *m0_dev, void *stream);
+// The same with (n + bs - 1) / bs doubles of caller workspace
+int launch_polar_points_ws(size_t n, const double *lat_dev, const int *perm_dev, size_t bs,
+                           int nmax, double thr, double *umax_ws, int *m0_dev, void *stream);
 int launch_pack_cs(const double *c_dev, const double *s_dev, size_t n, void *cs_dev,
                    void *stream);
 int launch_sincos(size_t n, const double *lat_dev, double *t_dev, double *u_dev, void *stream);
@@ -259,6 +288,27 @@ int prepare_row_fft(size_t n, void *stream);
 // globals: max(a1, a2 nmax), or -1 (off) when a2 < 0
 // (src/misc/misc_polar_optimization_threshold.c:16-20).
 double polar_threshold(unsigned long nmax);
+// Rings (points) per polar-optimisation block: SIMD size times the block
+// multiplier of synthesis (analysis = 0) or analysis (analysis = 1).
+size_t polar_block(int analysis);
+// First order the polar optimisation skips for a block with largest cos(lat)
+// umax (nmax + 1: none), and the same per ring for consecutive blocks of
+// `block` rings of u[0..nring).
+int polar_first_skip(unsigned long nmax, double thr, double umax);
+void polar_ring_m0(const double *u, size_t nring, unsigned long nmax, double thr, size_t block,
+                   int *m0);
+
+// Scattered synthesis plan: points per lane and the order chunks (chunk c
+// holds orders mc[c] .. mc[c+1]-1), from the point count and the degree only.
+// r[k] = sqrt(k), ri[k] = 1/sqrt(k) for k <= 2 nt + 3 and dm[n] for n <= nt,
+// the host inputs of launch_leg_tables (src/leg/leg_func_r_ri.c:15-30,
+// src/leg/leg_func_dm.c:14-23)
+void leg_host_tables(int nt, std::vector<double> &r, std::vector<double> &ri,
+                     std::vector<double> &dm);
+
+constexpr int SCTR_MAX_CHUNKS = 64;
+int sctr_points_per_lane(size_t npoint);
+int sctr_order_chunks(size_t npoint, unsigned long nmax, int *mc);  // mc: nchunk + 1 entries
 
 // Ring pairs per lane for nring rings, kmin <= k <= kmax: the fewest idle
 // lanes in the last ring tile, then the most rings per lane.

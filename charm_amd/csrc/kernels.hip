@@ -336,12 +336,14 @@ __device__ __forceinline__ void shs_grid_item(const ShsGridArgs &A, int tile, in
 #pragma unroll
     for (int k = 0; k < K; k++) aN[k] = bN[k] = aS[k] = bS[k] = 0.0;
 
-    // polar optimisation: order m is dropped at rings with m - nmax u > polar
-    // (src/misc/misc_polar_optimization_apply.c:16-41, tested per ring here)
+    // polar optimisation: order m is dropped at every ring of a CHarm ring
+    // block once m - nmax u > threshold holds for all rings of the block
+    // (src/misc/misc_polar_optimization_apply.c:16-41); pm0 holds the first
+    // dropped order per ring
     bool skip[K], allskip = true;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        skip[k] = !valid[k] || (A.polar >= 0.0 && (double)m - (double)nmax * A.u[ring[k]] > A.polar);
+        skip[k] = !valid[k] || (A.pm0 != nullptr && m >= A.pm0[ring[k]]);
         allskip = allskip && skip[k];
     }
     allskip = wave_all(allskip);
@@ -871,12 +873,14 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
     if (m > nmax) return;
 
     double *dst = (double *)((double2 *)A.part + (size_t)slot * tri_len_dev(nmax) + dcol_off(nmax, m) - m);
-    // polar optimisation: order m is dropped at rings with m - nmax u > polar
-    // (src/misc/misc_polar_optimization_apply.c:16-41, tested per ring here)
+    // polar optimisation: order m is dropped at every ring of a CHarm ring
+    // block once m - nmax u > threshold holds for all rings of the block
+    // (src/misc/misc_polar_optimization_apply.c:16-41); pm0 holds the first
+    // dropped order per ring
     bool skip[K], allskip = true;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        skip[k] = !valid[k] || (A.polar >= 0.0 && (double)m - (double)nmax * A.u[ring[k]] > A.polar);
+        skip[k] = !valid[k] || (A.pm0 != nullptr && m >= A.pm0[ring[k]]);
         allskip = allskip && skip[k];
     }
     // a wave whose column was found all zero by an earlier transform on this
@@ -1272,11 +1276,12 @@ __device__ __forceinline__ void sctr_item(const SctrArgs &A, int blk, int chunk,
     bool valid[K];
     double t[K], tq[K], q2[K], u[K], lon[K], q[K];
     double xs[K], rpc[K], f[K];
-    int ixs[K];
+    int ixs[K], m0[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const size_t i = base + k * 64 + lane;
         valid[k] = i < A.npoint;
+        m0[k] = (A.pm0 != nullptr && valid[k]) ? A.pm0[i] : nmax + 1;
         t[k] = valid[k] ? A.lat_t[i] : 0.0;
         u[k] = valid[k] ? A.lat_u[i] : 0.0;
         lon[k] = valid[k] ? A.lon[i] : 0.0;
@@ -1306,12 +1311,14 @@ __device__ __forceinline__ void sctr_item(const SctrArgs &A, int blk, int chunk,
         double x[K], y[K], a[K], b[K];
         int e[K];
         const double2 cm = csm[m];
-        // polar optimisation (src/misc/misc_polar_optimization_apply.c:16-41,
-        // tested per point here)
+        // polar optimisation: order m is dropped at every point of a CHarm
+        // point block once m - nmax u > threshold holds for all its points
+        // (src/misc/misc_polar_optimization_apply.c:16-41); m0 holds the
+        // first dropped order per point
         bool skip[K], allskip = true;
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            skip[k] = !valid[k] || (A.polar >= 0.0 && (double)m - (double)nmax * u[k] > A.polar);
+            skip[k] = !valid[k] || m >= m0[k];
             allskip = allskip && skip[k];
         }
         allskip = wave_all(allskip);
@@ -1488,6 +1495,7 @@ template <int K, bool UNIT>
 __global__ __launch_bounds__(256) void sctr_kernel(SctrArgs A)
 {
     __shared__ __align__(16) SctrSmem sm;
+    if (A.unit_dev != nullptr && ((*A.unit_dev != 0) != UNIT)) return;
     sctr_item<K, UNIT>(A, blockIdx.x, blockIdx.y, sm);
 }
 
@@ -1541,6 +1549,47 @@ __global__ void gather_points_kernel(size_t n, const int *__restrict__ perm,
     sincos(lat[p], &t_out[i], &u_out[i]);
     lon_out[i] = lon[p];
     r_out[i] = r[p];
+}
+
+// Polar optimisation at scattered points: CHarm tests it jointly over blocks
+// of bs consecutive input points (src/shs/shs_point_sctr.c:291-320,518-520).
+// The largest cos(lat) of each block ...
+__global__ void polar_block_umax_kernel(size_t n, size_t bs, const double *__restrict__ lat,
+                                        double *__restrict__ umax)
+{
+    const size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t i0 = b * bs;
+    if (i0 >= n) return;
+    const size_t i1 = i0 + bs < n ? i0 + bs : n;
+    double u = cos(lat[i0]);
+    for (size_t i = i0 + 1; i < i1; i++) u = fmax(u, cos(lat[i]));
+    umax[b] = u;
+}
+
+// ... and the first order skipped at each point (nmax + 1: none), for the
+// points in the order of perm (nullptr: input order).  The test
+// thr < m - nmax umax (src/misc/misc_polar_optimization_apply.c:30-37) rounds
+// the product and the difference separately, as CHarm's SIMD code does.
+__device__ int polar_first_skip_dev(int nmax, double thr, double umax)
+{
+#pragma clang fp contract(off)
+    const double c = (double)nmax * umax;
+    long m = (long)floor(thr + c);
+    if (m < 0) m = 0;
+    if (m > (long)nmax + 1) m = (long)nmax + 1;
+    while (m > 0 && thr < (double)(m - 1) - c) m--;
+    while (m <= (long)nmax && !(thr < (double)m - c)) m++;
+    return (int)m;
+}
+
+__global__ void polar_point_m0_kernel(size_t n, size_t bs, const int *__restrict__ perm,
+                                      const double *__restrict__ umax, int nmax, double thr,
+                                      int *__restrict__ m0)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t p = perm ? (size_t)perm[i] : i;
+    m0[i] = polar_first_skip_dev(nmax, thr, umax[p / bs]);
 }
 
 // sort key of a point: |sin lat| (a monotone function of the distance from the
@@ -1711,9 +1760,9 @@ template <int K>
 static void sctr_launch_k(const SctrArgs &a, hipStream_t s)
 {
     dim3 grid((unsigned)((a.npoint + 256 * K - 1) / (256 * K)), (unsigned)a.nchunk);
-    if (a.unit)
+    if (a.unit_dev != nullptr || a.unit)
         hipLaunchKernelGGL((sctr_kernel<K, true>), grid, dim3(256), 0, s, a);
-    else
+    if (a.unit_dev != nullptr || !a.unit)
         hipLaunchKernelGGL((sctr_kernel<K, false>), grid, dim3(256), 0, s, a);
 }
 
@@ -1735,41 +1784,115 @@ int launch_sctr_reduce(size_t npoint, int nchunk, const double *part, const int 
     return (int)hipGetLastError();
 }
 
-// Points ordered by |sin lat|: perm (device, n ints) and the gathered
-// t = sin lat, u = cos lat, lon, r.  Temporary storage comes from the
-// stream-ordered pool.
+// Points ordered by |sin lat|: the permutation (device, n ints) and the
+// gathered t = sin lat, u = cos lat, lon, r.  The sort ping-pongs between
+// two key and two index buffers of the workspace (layout of SortWs), so its
+// own temporary storage is small; the permutation ends in perm_buf[0] or
+// perm_buf[1] and *perm_out names which.
+struct SortWs {
+    size_t k1, i1, tmp, bytes;
+    SortWs(size_t n, size_t tmp_bytes)
+    {
+        auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        k1 = al(n * sizeof(double));
+        i1 = k1 + al(n * sizeof(double));
+        tmp = i1 + al(n * sizeof(int));
+        bytes = tmp + al(tmp_bytes);
+    }
+};
+
+static hipError_t sort_tmp_bytes(size_t n, size_t &tmp_bytes)
+{
+    hipcub::DoubleBuffer<double> keys(nullptr, nullptr);
+    hipcub::DoubleBuffer<int> vals(nullptr, nullptr);
+    tmp_bytes = 0;
+    return hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, vals, (int)n, 0,
+                                              (int)(8 * sizeof(double)), (hipStream_t)0);
+}
+
+size_t sort_points_ws_bytes(size_t n)
+{
+    size_t tb = 0;
+    if (sort_tmp_bytes(n, tb) != hipSuccess) return 0;
+    return SortWs(n, std::max<size_t>(tb, 16)).bytes;
+}
+
+int launch_sort_points_ws(size_t n, const double *lat, const double *lon, const double *r,
+                          int *perm, void *ws, size_t ws_bytes, double *t_out, double *u_out,
+                          double *lon_out, double *r_out, const int **perm_out, void *stream)
+{
+    hipStream_t s = S(stream);
+    size_t tb = 0;
+    hipError_t e = sort_tmp_bytes(n, tb);
+    if (e != hipSuccess) return (int)e;
+    const SortWs L(n, std::max<size_t>(tb, 16));
+    if (L.bytes > ws_bytes) return (int)hipErrorInvalidValue;
+    char *w = (char *)ws;
+    double *k0 = (double *)w, *k1 = (double *)(w + L.k1);
+    int *i1 = (int *)(w + L.i1);
+    hipLaunchKernelGGL(sort_key_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, lat,
+                       k0, perm);
+    e = hipGetLastError();
+    hipcub::DoubleBuffer<double> keys(k0, k1);
+    hipcub::DoubleBuffer<int> vals(perm, i1);
+    if (e == hipSuccess)
+        e = hipcub::DeviceRadixSort::SortPairs(w + L.tmp, tb, keys, vals, (int)n, 0,
+                                               (int)(8 * sizeof(double)), s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           n, vals.Current(), lat, lon, r, t_out, u_out, lon_out, r_out);
+        e = hipGetLastError();
+    }
+    *perm_out = vals.Current();
+    return (int)e;
+}
+
+// The same with the workspace from the stream-ordered pool; the permutation
+// is copied to perm when it ends in the second buffer.
 int launch_sort_points(size_t n, const double *lat, const double *lon, const double *r, int *perm,
                        double *t_out, double *u_out, double *lon_out, double *r_out, void *stream)
 {
     hipStream_t s = S(stream);
-    double *k0 = nullptr, *k1 = nullptr;
-    int *i0 = nullptr;
-    void *tmp = nullptr;
-    size_t tmp_bytes = 0;
-    hipError_t e = hipMallocAsync((void **)&k0, n * sizeof(double), s);
-    if (e == hipSuccess) e = hipMallocAsync((void **)&k1, n * sizeof(double), s);
-    if (e == hipSuccess) e = hipMallocAsync((void **)&i0, n * sizeof(int), s);
+    const size_t bytes = sort_points_ws_bytes(n);
+    if (!bytes) return (int)hipErrorInvalidValue;
+    void *ws = nullptr;
+    hipError_t e = hipMallocAsync(&ws, bytes, s);
+    if (e != hipSuccess) return (int)e;
+    const int *cur = perm;
+    e = (hipError_t)launch_sort_points_ws(n, lat, lon, r, perm, ws, bytes, t_out, u_out, lon_out,
+                                          r_out, &cur, stream);
+    if (e == hipSuccess && cur != perm)
+        e = hipMemcpyAsync(perm, cur, n * sizeof(int), hipMemcpyDeviceToDevice, s);
+    (void)hipFreeAsync(ws, s);
+    return (int)e;
+}
+
+int launch_polar_points_ws(size_t n, const double *lat, const int *perm, size_t bs, int nmax,
+                           double thr, double *umax_ws, int *m0, void *stream)
+{
+    hipStream_t s = S(stream);
+    const size_t nb = (n + bs - 1) / bs;
+    hipLaunchKernelGGL(polar_block_umax_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s,
+                       n, bs, lat, umax_ws);
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(sort_key_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
-                           lat, k0, i0);
+        hipLaunchKernelGGL(polar_point_m0_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           s, n, bs, perm, umax_ws, nmax, thr, m0);
         e = hipGetLastError();
     }
-    if (e == hipSuccess)
-        e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0, k1, i0, perm, (int)n, 0,
-                                               (int)(8 * sizeof(double)), s);
-    if (e == hipSuccess) e = hipMallocAsync(&tmp, std::max<size_t>(tmp_bytes, 16), s);
-    if (e == hipSuccess)
-        e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k1, i0, perm, (int)n, 0,
-                                               (int)(8 * sizeof(double)), s);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                           n, perm, lat, lon, r, t_out, u_out, lon_out, r_out);
-        e = hipGetLastError();
-    }
-    if (tmp) (void)hipFreeAsync(tmp, s);
-    if (i0) (void)hipFreeAsync(i0, s);
-    if (k1) (void)hipFreeAsync(k1, s);
-    if (k0) (void)hipFreeAsync(k0, s);
+    return (int)e;
+}
+
+int launch_polar_points(size_t n, const double *lat, const int *perm, size_t bs, int nmax,
+                        double thr, int *m0, void *stream)
+{
+    hipStream_t s = S(stream);
+    const size_t nb = (n + bs - 1) / bs;
+    double *umax = nullptr;
+    hipError_t e = hipMallocAsync((void **)&umax, std::max<size_t>(nb, 1) * sizeof(double), s);
+    if (e != hipSuccess) return (int)e;
+    e = (hipError_t)launch_polar_points_ws(n, lat, perm, bs, nmax, thr, umax, m0, stream);
+    (void)hipFreeAsync(umax, s);
     return (int)e;
 }
 
@@ -1778,6 +1901,33 @@ int launch_sincos(size_t n, const double *lat, double *t, double *u, void *strea
     hipLaunchKernelGGL(sincos_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), n,
                        lat, t, u);
     return (int)hipGetLastError();
+}
+
+__global__ void set_ints_kernel(int *p, IntVals v)
+{
+    const int i = (int)threadIdx.x;
+    if (i < v.n) p[i] = v.v[i];
+}
+
+int launch_set_ints(int *p, const IntVals &v, void *stream)
+{
+    if (v.n < 0 || v.n > 72) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(set_ints_kernel, dim3(1), dim3(128), 0, S(stream), p, v);
+    return (int)hipGetLastError();
+}
+
+int launch_all_near_dev(size_t n, const double *x, double ref, double eps, int *flag, void *stream)
+{
+    IntVals one{};
+    one.n = 1;
+    one.v[0] = 1;
+    hipError_t e = (hipError_t)launch_set_ints(flag, one, stream);
+    if (e == hipSuccess && n) {
+        hipLaunchKernelGGL(all_near_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           S(stream), n, x, ref, eps, flag);
+        e = hipGetLastError();
+    }
+    return (int)e;
 }
 
 int launch_all_near(size_t n, const double *x, double ref, double eps, int *result, void *stream)

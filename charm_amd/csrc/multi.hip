@@ -106,9 +106,10 @@ __global__ __launch_bounds__(64) void shs_multi_kernel(MultiShsArgs A)
 #pragma unroll
             for (int p = 0; p < 2; p++) acc[q][c][p] = (d4){0.0, 0.0, 0.0, 0.0};
 
-    // polar optimisation: order m is dropped at rings with m - nmax u > polar
-    // (src/misc/misc_polar_optimization_apply.c:16-41, tested per ring here)
-    const bool skip = !valid || (A.polar >= 0.0 && (double)m - (double)nmax * A.u[ring] > A.polar);
+    // polar optimisation: order m is dropped at every ring of a CHarm ring
+    // block (src/misc/misc_polar_optimization_apply.c:16-41); pm0 holds the
+    // first dropped order per ring
+    const bool skip = !valid || (A.pm0 != nullptr && m >= A.pm0[ring]);
     if (m <= nmax && !all64(skip)) {
         const double t = valid ? A.t[ring] : 0.0;
         double xq, yq = 0.0;

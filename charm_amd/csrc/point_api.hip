@@ -377,15 +377,8 @@ Tables tables(int nmax, hipStream_t stream)
     int nt = std::max(nmax, std::max(64, g_tab.nt + g_tab.nt / 4));
     // r[k] = sqrt(k), ri[k] = 1/sqrt(k) (src/leg/leg_func_r_ri.c:15-30) and
     // dm[n] (src/leg/leg_func_dm.c:14-23)
-    std::vector<double> r(2 * (size_t)nt + 4), ri(2 * (size_t)nt + 4), dm((size_t)nt + 1);
-    r[0] = ri[0] = 0.0;
-    for (size_t k = 1; k <= 2 * (size_t)nt + 3; k++) {
-        double w = std::sqrt((double)k);
-        r[k] = w;
-        ri[k] = 1.0 / w;
-    }
-    dm[0] = 0.0;
-    for (size_t n = 1; n <= (size_t)nt; n++) dm[n] = r[2 * n + 3] * ri[2 * n + 2];
+    std::vector<double> r, ri, dm;
+    leg_host_tables(nt, r, ri, dm);
     void *ab = nullptr;
     double *al = nullptr, *sn = nullptr, *ddm = nullptr, *dr = nullptr, *dri = nullptr;
     HIPCHK(hipMalloc(&ab, tri_len(nt) * 2 * sizeof(double)));
@@ -688,6 +681,26 @@ int tuned_k(int kind, int nring, unsigned long nmax, int kmin, int kmax, hipStre
     return best;
 }
 
+// Polar optimisation on a grid: the first order skipped per ring, for CHarm's
+// ring blocks of synthesis (analysis = 0) or analysis (analysis = 1), on the
+// device; p stays null when the optimisation is off.  The rings are CHarm's
+// "do" rings in its loop order (src/shs/shs_point_grd.c:594-600,860-862,
+// src/sha/sha_point.c:716-718).
+struct PolarRings {
+    DevBuf buf;
+    const int *p = nullptr;
+    PolarRings(const GridGeom &g, unsigned long nmax, int analysis, hipStream_t s)
+    {
+        const double thr = polar_threshold(nmax);
+        if (thr < 0.0 || g.nring == 0) return;
+        std::vector<int> m0(g.nring);
+        polar_ring_m0(g.u.data(), g.nring, nmax, thr, polar_block(analysis), m0.data());
+        buf.alloc(g.nring * sizeof(int), s);
+        h2d(buf.as<int>(), m0.data(), g.nring, s);
+        p = buf.as<int>();
+    }
+};
+
 // Lumped sums of a grid synthesis: the Hermitian half-spectra of the rows
 // (fft = true, ld = nlon/2 + 1 values per row, ready for c2r) or the raw
 // (a_m, b_m) (fft = false, ld = nmax + 1 per row)
@@ -710,8 +723,8 @@ void shs_grid_lumped(const GridGeom &g, const DevGrid &d, bool fft, unsigned lon
     a.mirror = d.mirror;
     a.al = tb.al;
     a.al_nt = tb.nt;
-    a.u = d.u;
-    a.polar = polar_threshold(nmax);
+    PolarRings pol(g, nmax, 0, stream);
+    a.pm0 = pol.p;
     // coefficients times s_nm for the scaled recurrence, as the kernel stages them
     a.cs = cs_dev;
     a.sn = tb.sn;
@@ -728,7 +741,7 @@ void shs_grid_lumped(const GridGeom &g, const DevGrid &d, bool fft, unsigned lon
     });
     a.ntile = ring_tiles(g.nring, a.k);
     g_last_k[0].store(a.k);
-    if (d.dead && a.unit && a.polar < 0.0)
+    if (d.dead && a.unit && a.pm0 == nullptr)
         a.dead = d.dead->acquire(a.k, nmax, a.ntile, stream, a.dead_mode);
     {
         ProfScope ps("shs_grid_kernel", stream);
@@ -878,21 +891,9 @@ int sctr_k(size_t npoint)
 // than one, the waves of a long column balance better).
 std::vector<int> sctr_chunks(size_t npoint, unsigned long nmax)
 {
-    const size_t nwave = (npoint + 127) / 128;
-    int nchunk = (int)std::min<size_t>(64, std::max<size_t>(1, 16384 / std::max<size_t>(nwave, 1)));
-    nchunk = (int)std::min<unsigned long>((unsigned long)nchunk, nmax + 1);
-    std::vector<int> mc(nchunk + 1);
-    const double total = 0.5 * (double)(nmax + 1) * (double)(nmax + 2);
-    double acc = 0.0;
-    int c = 1;
-    mc[0] = 0;
-    for (unsigned long m = 0; m <= nmax && c < nchunk; m++) {
-        acc += (double)(nmax + 1 - m);
-        if (acc >= total * c / nchunk) mc[c++] = (int)m + 1;
-    }
-    while (c < nchunk) mc[c++] = (int)nmax + 1;
-    mc[nchunk] = (int)nmax + 1;
-    return mc;
+    int buf[SCTR_MAX_CHUNKS + 1];
+    const int nchunk = sctr_order_chunks(npoint, nmax, buf);
+    return std::vector<int>(buf, buf + nchunk + 1);
 }
 
 // Scattered synthesis on device arrays: lat/lon/r of npoint points, unscaled
@@ -941,7 +942,14 @@ void run_sctr(size_t npoint, const double *lat_dev, const double *lon_dev, const
     a.sn = tb.sn;
     a.part = part.as<double>();
     a.k = K;
-    a.polar = polar_threshold(nmax);
+    const double thr = polar_threshold(nmax);
+    DevBuf pm0;
+    if (thr >= 0.0) {
+        pm0.alloc(npoint * sizeof(int), s);
+        HIPCHK((hipError_t)launch_polar_points(npoint, lat_dev, perm.p ? perm.as<int>() : nullptr,
+                                               polar_block(0), (int)nmax, thr, pm0.as<int>(), s));
+        a.pm0 = pm0.as<int>();
+    }
     {
         ProfScope ps("sctr_kernel", s);
         HIPCHK((hipError_t)launch_sctr(a, s));
@@ -1376,8 +1384,8 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
     a.lds = nfc;
     a.al = tb.al;
     a.al_nt = tb.nt;
-    a.u = d.u;
-    a.polar = polar_threshold(nmax);
+    PolarRings pol(g, nmax, 1, s);
+    a.pm0 = pol.p;
     // Ring tiles are grouped so that the launch has about SHA_BLOCKS blocks:
     // each block adds its group's tiles, in order, into one partial slot.
     // Few slots keep the partial sums small at high degree (one per tile
@@ -1414,7 +1422,7 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
         a.fs = s_dev;
         a.fstride = stride;
     }
-    if (d.dead && a.polar < 0.0) a.dead = d.dead->acquire(a.k, nmax, a.ntile, s, a.dead_mode);
+    if (d.dead && a.pm0 == nullptr) a.dead = d.dead->acquire(a.k, nmax, a.ntile, s, a.dead_mode);
     {
         ProfScope ps("sha_kernel", s);
         HIPCHK((hipError_t)launch_sha(a, s));
@@ -1493,6 +1501,47 @@ void sha_host(const charm_point *pnt, const double *f, unsigned long nmax, charm
 }
 
 }  // namespace
+
+// Plan of a scattered synthesis shared with the allocation-free entry point
+int sctr_points_per_lane(size_t npoint)
+{
+    return sctr_k(npoint);
+}
+
+int sctr_order_chunks(size_t npoint, unsigned long nmax, int *mc)
+{
+    const size_t nwave = (npoint + 127) / 128;
+    int nchunk = (int)std::min<size_t>(SCTR_MAX_CHUNKS,
+                                       std::max<size_t>(1, 16384 / std::max<size_t>(nwave, 1)));
+    nchunk = (int)std::min<unsigned long>((unsigned long)nchunk, nmax + 1);
+    const double total = 0.5 * (double)(nmax + 1) * (double)(nmax + 2);
+    double acc = 0.0;
+    int c = 1;
+    mc[0] = 0;
+    for (unsigned long m = 0; m <= nmax && c < nchunk; m++) {
+        acc += (double)(nmax + 1 - m);
+        if (acc >= total * c / nchunk) mc[c++] = (int)m + 1;
+    }
+    while (c < nchunk) mc[c++] = (int)nmax + 1;
+    mc[nchunk] = (int)nmax + 1;
+    return nchunk;
+}
+
+// r[k] = sqrt(k), ri[k] = 1/sqrt(k), k <= 2 nt + 3, and dm[n], n <= nt, as
+// tables() computes them
+void leg_host_tables(int nt, std::vector<double> &r, std::vector<double> &ri,
+                     std::vector<double> &dm)
+{
+    r.assign(2 * (size_t)nt + 4, 0.0);
+    ri.assign(2 * (size_t)nt + 4, 0.0);
+    dm.assign((size_t)nt + 1, 0.0);
+    for (size_t k = 1; k <= 2 * (size_t)nt + 3; k++) {
+        double w = std::sqrt((double)k);
+        r[k] = w;
+        ri[k] = 1.0 / w;
+    }
+    for (size_t n = 1; n <= (size_t)nt; n++) dm[n] = r[2 * n + 3] * ri[2 * n + 2];
+}
 
 }  // namespace charm_amd
 
@@ -1888,8 +1937,8 @@ void charm_amd_shs_grid_multi_dev(const charm_amd_grid *grd, unsigned long nmax,
         a.spec = spec.p;
         a.lds = nfc;
         a.mur = mu / R;
-        a.u = grd->u;
-        a.polar = polar_threshold(nmax);
+        PolarRings pol(grd->g, nmax, 0, s);
+        a.pm0 = pol.p;
         {
             ProfScope ps("shs_multi_kernel", s);
             HIPCHK((hipError_t)launch_shs_multi(a, s));
@@ -2077,6 +2126,18 @@ T *dupload(charm_amd_batch *b, const std::vector<T> &v, hipStream_t s)
     return p;
 }
 
+// First skipped order per ring for the polar optimisation (PolarRings), in
+// memory the batch owns; nullptr when the optimisation is off
+const int *polar_rings_owned(charm_amd_batch *b, const GridGeom &g, unsigned long nmax,
+                             int analysis, hipStream_t s)
+{
+    const double thr = polar_threshold(nmax);
+    if (thr < 0.0 || g.nring == 0) return nullptr;
+    std::vector<int> m0(g.nring);
+    polar_ring_m0(g.u.data(), g.nring, nmax, thr, polar_block(analysis), m0.data());
+    return dupload(b, m0, s);
+}
+
 void batch_enqueue(charm_amd_batch *b, hipStream_t s)
 {
     HIPCHK((hipError_t)launch_zero_ints(b->d_counter, 3, s));
@@ -2190,7 +2251,13 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
                 double *part = (double *)dmalloc(b, (size_t)nchunk * np * sizeof(double));
                 a.part = part;
                 a.k = K;
-                a.polar = polar_threshold(q.nmax);
+                const double thr = polar_threshold(q.nmax);
+                if (thr >= 0.0) {
+                    int *m0 = (int *)dmalloc(b, np * sizeof(int));
+                    HIPCHK((hipError_t)launch_polar_points(np, q.lat, perm, polar_block(0),
+                                                           (int)q.nmax, thr, m0, s));
+                    a.pm0 = m0;
+                }
                 const int idx = (int)sct_args.size();
                 sct_args.push_back(a);
                 b->sct.push_back({i, cs2, part, nchunk, perm});
@@ -2221,8 +2288,7 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
                 a.mirror = g->mirror;
                 a.al = tb.al;
                 a.al_nt = tb.nt;
-                a.u = g->u;
-                a.polar = polar_threshold(q.nmax);
+                a.pm0 = polar_rings_owned(b, g->g, q.nmax, 0, s);
                 void *cs2 = dmalloc(b, tri_len(q.nmax) * 2 * sizeof(double));
                 a.cs = cs2;
                 a.mur = q.mu / q.R;
@@ -2275,8 +2341,7 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
                 a.lds = nfc;
                 a.al = tb.al;
                 a.al_nt = tb.nt;
-                a.u = g->u;
-                a.polar = polar_threshold(q.nmax);
+                a.pm0 = polar_rings_owned(b, g->g, q.nmax, 1, s);
                 void *part = dmalloc(b, (size_t)a.ntile * tri_len(q.nmax) * 2 * sizeof(double));
                 a.part = part;
                 std::vector<double> sc;

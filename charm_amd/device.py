@@ -259,14 +259,6 @@ class Batch:
         f.argtypes = [_vp]
         return bool(f(self._h))
 
-    def shs_deriv(self, cs, nmax, mu, R, what, out, stream=None):
-        """Gradient or guru derivative into ``out`` (nout, nlat, nlon):
-        ``what`` is DERIV_GRAD1, DERIV_GRAD2 or 100 dr + 10 dlat + dlon."""
-        _call("charm_amd_shs_grid_deriv_dev",
-              [_vp, _ct.c_ulong, _ct.c_double, _ct.c_double, _vp, _ct.c_int, _vp, _vp],
-              self._h, int(nmax), float(mu), float(R), cs.ptr, int(what), out.ptr, _s(stream))
-        return out
-
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

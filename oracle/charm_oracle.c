@@ -34,9 +34,11 @@
  *   src/misc/misc_polar_optimization_threshold.c:16-20,
  *   misc_polar_optimization_apply.c:16-41 (polar optimisation, off by
  *   default; CHarm tests the criterion jointly for a block of SIMD_SIZE x
- *   BLOCK_S points, a size that depends on its build, and skips the order for
- *   the block when every point passes; here it is tested per ring or point.
- *   Parity unpinned: no CHarm test or fixture covers the polar optimisation.)
+ *   BLOCK_S rings or points in synthesis, SIMD_SIZE x BLOCK_A rings in
+ *   analysis, and skips the order for the whole block when every member
+ *   passes: shs_point_grd.c:594-600,860-862, shs_point_sctr.c:291-320,518-520,
+ *   sha_point.c:716-718.  The block sizes are set with orc_set_polar; the
+ *   AVX-512 double build has 32 and 64.)
  */
 #include <math.h>
 #include <stdint.h>
@@ -141,6 +143,7 @@ static int nearly_equal(real a, real b, real eps)
 }
 
 #define GLOB_THRESHOLD (((real)100) * REPS) /* src/glob/glob.c:16 */
+#define GLOB_THRESHOLD2 (((real)100000) * REPS) /* src/glob/glob.c:17 */
 
 /* ------------------------------------------------------------------------ */
 /* Legendre recurrence coefficients                                         */
@@ -399,26 +402,86 @@ static int fft_ok(size_t nlon, const real *lon, unsigned long nmax)
 /* ------------------------------------------------------------------------ */
 /* Public oracle API                                                        */
 /* ------------------------------------------------------------------------ */
-/* Polar optimisation (glob_polar_optimization_a1/a2): order m is skipped at a
- * point with cos(lat) = u when m - nmax u > max(a1, a2 nmax); a2 < 0 turns it
- * off (misc_polar_optimization_threshold.c:16-20,
- * misc_polar_optimization_apply.c:16-41). */
+/* Polar optimisation (glob_polar_optimization_a1/a2): order m is skipped for
+ * a block of rings or points when every member, cos(lat) = u, passes
+ * m - nmax u > max(a1, a2 nmax); a2 < 0 turns it off
+ * (misc_polar_optimization_threshold.c:16-20,
+ * misc_polar_optimization_apply.c:16-41).  Blocks hold bs_s consecutive rings
+ * or points in synthesis and bs_a rings in analysis (SIMD_SIZE times
+ * BLOCK_S / BLOCK_A of the CHarm build, src/simd/simd.h:298-304,609-611). */
 static unsigned long polar_a1 = 100;
 static double polar_a2 = -1.0;
+static size_t polar_bs_s = 32, polar_bs_a = 64;
 
-void orc_set_polar(unsigned long a1, double a2)
+void orc_set_polar(unsigned long a1, double a2, size_t bs_s, size_t bs_a)
 {
     polar_a1 = a1;
     polar_a2 = a2;
+    polar_bs_s = bs_s ? bs_s : 1;
+    polar_bs_a = bs_a ? bs_a : 1;
 }
 
-static int polar_skip(unsigned long m, unsigned long nmax, real u)
+/* the test of one member, cos(lat) = u, for order m */
+static int polar_pass(unsigned long m, unsigned long nmax, real u)
 {
     if (polar_a2 < 0.0) return 0;
     real thr = (real)polar_a1;
     real t2 = (real)polar_a2 * (real)nmax;
     if (t2 > thr) thr = t2;
     return thr < (real)m - (real)nmax * u;
+}
+
+/* order m skipped for the block of cos(lat) values u[0..n) */
+static int polar_skip_block(unsigned long m, unsigned long nmax, const real *u, size_t n)
+{
+    for (size_t i = 0; i < n; i++)
+        if (!polar_pass(m, nmax, u[i])) return 0;
+    return n > 0;
+}
+
+/* misc_arr_chck_symm (src/misc/misc_arr_chck_symm.c:16-52) about 0 */
+static int arr_symm(const real *a, size_t n, real eps);
+
+/* The polar block of each row of a grid: rb[row] = first ring of the block,
+ * rn[row] = its length, over the rings CHarm evaluates directly.  On a
+ * symmetric grid these are the first nlatdo = (n + 1 - even) / 2 rings, with
+ * n = nlat + 1 on Driscoll-Healy grids (the missing south pole) and n = nlat
+ * otherwise; a mirror row n - 1 - i takes ring i's block.  Otherwise every
+ * ring is evaluated directly (shs_point_grd.c:116-197,594-600,
+ * crd_grd_check_symm.c:21-72).  type: 0 custom grid (symmetric when
+ * misc_arr_chck_symm passes at glob_threshold2), 1 or 3 Gauss-Legendre,
+ * 4/5 Driscoll-Healy. */
+static void polar_rows(int type, size_t nlat, const real *lat, size_t bs, size_t *rb, size_t *rn);
+
+static int arr_symm(const real *a, size_t n, real eps)
+{
+    for (size_t i = 0; i < n / 2; i++)
+        if (!nearly_equal(a[i] - 0, 0 - a[n - 1 - i], eps)) return 0;
+    if (n % 2 && !nearly_equal(0, a[n / 2], eps)) return 0;
+    return 1;
+}
+
+static void polar_rows(int type, size_t nlat, const real *lat, size_t bs, size_t *rb, size_t *rn)
+{
+    const int dh = type == 4 || type == 5;
+    const size_t n = dh ? nlat + 1 : nlat;
+    int symm;
+    if (dh) symm = 1;
+    else if (nlat == 1) symm = 0;
+    else if (type == 1 || type == 3) symm = 1;
+    else symm = arr_symm(lat, nlat, GLOB_THRESHOLD2);
+    const int even = !(n % 2);
+    const size_t nlatdo = symm ? (n + 1 - even) / 2 : n;
+    for (size_t i = 0; i < nlatdo; i++) {
+        const size_t b0 = (i / bs) * bs;
+        const size_t len = (b0 + bs < nlatdo ? b0 + bs : nlatdo) - b0;
+        rb[i] = b0;
+        rn[i] = len;
+        if (symm) {
+            const size_t j = n - 1 - i;
+            if (j < nlat && j != i) { rb[j] = b0; rn[j] = len; }
+        }
+    }
 }
 
 int orc_ncomp(void) { return NCOMP; }
@@ -520,7 +583,9 @@ int orc_dh_grid(unsigned long nmax, int type, double *lat_o, double *w_o, double
  *             by definition and the southern rings reuse the northern
  *             latitudes, as the reference does), 0 for a custom grid (each
  *             ring evaluated at its own latitude, which is bitwise the same
- *             as the reference's fold for an exactly symmetric grid).
+ *             as the reference's fold for an exactly symmetric grid), 4 or 5
+ *             for a Driscoll-Healy grid (evaluated as a custom grid; the type
+ *             sets the polar-optimisation ring blocks, polar_rows).
  *   C, S    : m-major triangles of a model of degree nmax_shc >= nmax.
  *   mu, R   : one value each.  lat, r: nlat values; lon: nlon values.
  *   rings   : if non-NULL, only these nring ring indices are synthesised and
@@ -553,10 +618,14 @@ int orc_shs_grid(unsigned long nmax, unsigned long nmax_shc, const double *C_i,
     if (use_fft) twiddles(nlon, &ct, &stb);
 
     const size_t nout = rings ? nring : nlat;
-    const int even = !(nlat % 2);
+    const int type = gl;
+    gl = gl == 1;
     const size_t equator = (gl && !(nlat - 1) % 2) ? nlat / 2 : (size_t)-1;
-    const size_t nlatdo = (gl && nlat > 1) ? (nlat + 1 - even) / 2 : nlat;
-    (void)nlatdo;
+    /* polar optimisation blocks (polar_rows) */
+    real *ucos = (real *)malloc(nlat * sizeof(real));
+    size_t *rb = (size_t *)malloc(nlat * sizeof(size_t)), *rn = (size_t *)malloc(nlat * sizeof(size_t));
+    for (size_t i = 0; i < nlat; i++) ucos[i] = RCOS(lat[i]);
+    polar_rows(type, nlat, lat, polar_bs_s, rb, rn);
 
 #pragma omp parallel
     {
@@ -585,7 +654,7 @@ int orc_shs_grid(unsigned long nmax, unsigned long nmax_shc, const double *C_i,
         leg_seed(u, dm, nmax, ps, ips);
         rpows(R, rr[i], unit, nmax + 1, rp);
         for (unsigned long m = 0; m <= nmax; m++) {
-            if (polar_skip(m, nmax, u)) { a[m] = 0; b[m] = 0; continue; }
+            if (polar_skip_block(m, nmax, ucos + rb[i], rn[i])) { a[m] = 0; b[m] = 0; continue; }
             leg_anm_bnm(nmax, m, rt, rit, anm, bnm);
             size_t off = col_off(nmax_shc, m);
             lumped_ring(nmax, m, C + off, S + off, anm, bnm, t, ps, ips, rp, NULL,
@@ -608,7 +677,7 @@ int orc_shs_grid(unsigned long nmax, unsigned long nmax_shc, const double *C_i,
     free(anm); free(bnm); free(ps); free(ips); free(rp); free(a); free(b); free(out);
     }
     free(C); free(S); free(lat); free(rr); free(lon); free(rt); free(rit); free(dm);
-    free(ct); free(stb);
+    free(ct); free(stb); free(ucos); free(rb); free(rn);
     return 0;
 }
 
@@ -637,7 +706,8 @@ int orc_lumped(unsigned long nmax, unsigned long nmax_shc, const double *C_i,
     real t = RSIN(lat);
     for (unsigned long m = 0; m <= nmax; m++) {
         real a = 0, b = 0;
-        if (!polar_skip(m, nmax, RCOS(lat))) {
+        const real ul = RCOS(lat);
+        if (!polar_skip_block(m, nmax, &ul, 1)) {
             leg_anm_bnm(nmax, m, rt, rit, anm, bnm);
             size_t off = col_off(nmax_shc, m);
             lumped_ring(nmax, m, C + off, S + off, anm, bnm, t, ps, ips, rp, NULL, &a, &b, NULL, NULL);
@@ -671,6 +741,10 @@ int orc_shs_sctr(unsigned long nmax, unsigned long nmax_shc, const double *C_i,
     real *dm = (real *)malloc((nmax + 1) * sizeof(real));
     leg_tables(nmax, rt, rit, dm);
     const real mur = mu / R;
+    /* polar optimisation blocks: polar_bs_s consecutive points */
+    real *ucos = (real *)malloc((npoint ? npoint : 1) * sizeof(real));
+    for (size_t i = 0; i < npoint; i++) ucos[i] = RCOS(lat[i]);
+    const size_t bs = polar_bs_s;
 #pragma omp parallel
     {
     real *anm = (real *)malloc((nmax + 2) * sizeof(real));
@@ -681,12 +755,13 @@ int orc_shs_sctr(unsigned long nmax, unsigned long nmax_shc, const double *C_i,
 #pragma omp for schedule(dynamic)
     for (size_t i = 0; i < npoint; i++) {
         real t = RSIN(lat[i]), u = RCOS(lat[i]);
+        const size_t b0 = (i / bs) * bs, bn = (b0 + bs < npoint ? b0 + bs : npoint) - b0;
         leg_seed(u, dm, nmax, ps, ips);
         rpows(R, rr[i], unit, nmax + 1, rp);
         real fi = 0;
         for (unsigned long m = 0; m <= nmax; m++) {
             real a, b;
-            if (polar_skip(m, nmax, u)) continue;
+            if (polar_skip_block(m, nmax, ucos + b0, bn)) continue;
             leg_anm_bnm(nmax, m, rt, rit, anm, bnm);
             size_t off = col_off(nmax_shc, m);
             lumped_ring(nmax, m, C + off, S + off, anm, bnm, t, ps, ips, rp, NULL, &a, &b, NULL, NULL);
@@ -697,7 +772,7 @@ int orc_shs_sctr(unsigned long nmax, unsigned long nmax_shc, const double *C_i,
     }
     free(anm); free(bnm); free(ps); free(ips); free(rp);
     }
-    free(C); free(S); free(lat); free(lon); free(rr); free(rt); free(rit); free(dm);
+    free(C); free(S); free(lat); free(lon); free(rr); free(rt); free(rit); free(dm); free(ucos);
     return 0;
 }
 
@@ -811,7 +886,9 @@ static int orc_sha_quad(int type, unsigned long nmax, unsigned long nmax_shc, si
         leg_anm_bnm(nmax, m, rt, rit, anm, bnm);
         real *Cm = C + col_off(nmax_shc, m), *Sm = S + col_off(nmax_shc, m);
         for (size_t i = 0; i < nlatdo; i++) {
-            if (polar_skip(m, nmax, uu[i])) continue;
+            const size_t b0 = (i / polar_bs_a) * polar_bs_a;
+            const size_t bn = (b0 + polar_bs_a < nlatdo ? b0 + polar_bs_a : nlatdo) - b0;
+            if (polar_skip_block(m, nmax, uu + b0, bn)) continue;
             real t = tt[i];
             size_t q = m * nlatdo + i;
             real ap = amp[q], am = amm[q], bp = bmp[q], bm = bmm[q];
@@ -966,7 +1043,7 @@ static void guru_point(unsigned long nmax, unsigned long nmax_shc, const real *C
                        const real *S, real mu, real R, real lat, real lon, real r,
                        int unit, const real *rt, const real *rit, const real *dm,
                        real *anm, real *bnm, real *enm, real *ps, int64_t *ips, real *rp,
-                       real *p, real *g)
+                       real *p, const real *ub, size_t nb, real *g)
 {
     const real t = RSIN(lat), u = RCOS(lat);
     const real tu = t / u, ui = ((real)1) / u, ui2 = ui * ui;
@@ -977,7 +1054,7 @@ static void guru_point(unsigned long nmax, unsigned long nmax_shc, const real *C
         for (int j = 0; j < 3; j++)
             for (int k = 0; k < 3; k++) F[i][j][k] = 0;
     for (unsigned long m = 0; m <= nmax; m++) {
-        if (polar_skip(m, nmax, u)) continue;
+        if (polar_skip_block(m, nmax, ub, nb)) continue;
         leg_anm_bnm(nmax, m, rt, rit, anm, bnm);
         enm[m] = 0;
         for (unsigned long n = m + 1; n <= nmax; n++)
@@ -1052,6 +1129,10 @@ int orc_shs_deriv_sctr(int kind, unsigned long nmax, unsigned long nmax_shc,
     real *dm = (real *)malloc((nmax + 1) * sizeof(real));
     leg_tables(nmax, rt, rit, dm);
     const int nout = kind < 10 ? 1 : (kind == 10 ? 3 : 6);
+    /* polar optimisation blocks: polar_bs_s consecutive points */
+    real *ucos = (real *)malloc((npoint ? npoint : 1) * sizeof(real));
+    for (size_t i = 0; i < npoint; i++) ucos[i] = RCOS(lat[i]);
+    const size_t bs = polar_bs_s;
 #pragma omp parallel
     {
     real *anm = (real *)malloc((nmax + 2) * sizeof(real));
@@ -1064,8 +1145,9 @@ int orc_shs_deriv_sctr(int kind, unsigned long nmax, unsigned long nmax_shc,
 #pragma omp for schedule(dynamic)
     for (size_t i = 0; i < npoint; i++) {
         real g[10], o[6];
+        const size_t b0 = (i / bs) * bs, bn = (b0 + bs < npoint ? b0 + bs : npoint) - b0;
         guru_point(nmax, nmax_shc, C, S, mu, R, lat[i], lon[i], rr[i], unit, rt, rit, dm,
-                   anm, bnm, enm, ps, ips, rp, p, g);
+                   anm, bnm, enm, ps, ips, rp, p, ucos + b0, bn, g);
         const real r = unit ? R : rr[i], ri = ((real)1) / r;
         const real tu = RSIN(lat[i]) / RCOS(lat[i]);
         if (kind < 10) {
@@ -1086,7 +1168,7 @@ int orc_shs_deriv_sctr(int kind, unsigned long nmax, unsigned long nmax_shc,
     }
     free(anm); free(bnm); free(enm); free(ps); free(ips); free(rp); free(p);
     }
-    free(C); free(S); free(lat); free(lon); free(rr); free(rt); free(rit); free(dm);
+    free(C); free(S); free(lat); free(lon); free(rr); free(rt); free(rit); free(dm); free(ucos);
     return 0;
 }
 

@@ -57,7 +57,7 @@ class Oracle:
         L.orc_leg_pnm.argtypes = [_ul, _dp, _dp]
         L.orc_shs_deriv_sctr.argtypes = [ct.c_int, _ul, _ul, _dp, _dp, _dp, _dp, _sz, _dp, _dp,
                                          _dp, _dp]
-        L.orc_set_polar.argtypes = [_ul, ct.c_double]
+        L.orc_set_polar.argtypes = [_ul, ct.c_double, _sz, _sz]
 
     # -- layout helpers ------------------------------------------------------
     def pack(self, x):
@@ -104,9 +104,12 @@ class Oracle:
         return a.ctypes.data_as(_dp)
 
     # -- API -------------------------------------------------------------------
-    def set_polar(self, a1, a2):
-        """Polar optimisation constants (a2 < 0: off, the default)."""
-        self.lib.orc_set_polar(int(a1), float(a2))
+    def set_polar(self, a1, a2, block_shs=32, block_sha=64):
+        """Polar optimisation constants (a2 < 0: off, the default) and the
+        ring/point blocks over which CHarm tests it jointly: SIMD_SIZE x
+        BLOCK_S in synthesis, SIMD_SIZE x BLOCK_A in analysis (32 and 64 for
+        the AVX-512 double build)."""
+        self.lib.orc_set_polar(int(a1), float(a2), int(block_shs), int(block_sha))
 
     def gl_grid(self, nmax, exact=False):
         L = nmax + 1
@@ -128,7 +131,11 @@ class Oracle:
         return (self.unpack(lat, nlat, exact), self.unpack(w, nlat, exact),
                 self.unpack(lon, nlon, exact))
 
-    def shs_grid(self, nmax, C, S, mu, R, lat, r, lon, gl=True, rings=None, exact=False):
+    def shs_grid(self, nmax, C, S, mu, R, lat, r, lon, gl=True, rings=None, exact=False,
+                 dh=0):
+        """Grid synthesis; gl selects a Gauss-Legendre grid, dh = 1 or 2 a
+        Driscoll-Healy grid (evaluated ring by ring; dh sets the polar
+        optimisation's ring blocks), otherwise a custom grid."""
         nmax_shc = _nmax_from_len(len(C))
         nlat, nlon = len(lat), len(lon)
         Cb, Sb = self.pack(C), self.pack(S)
@@ -141,7 +148,8 @@ class Oracle:
             rp, nring, nout = rarr.ctypes.data_as(ct.POINTER(_sz)), rarr.size, rarr.size
         f = self.empty(nout * nlon)
         self.lib.orc_shs_grid(nmax, nmax_shc, self._p(Cb), self._p(Sb), self._p(mub),
-                              self._p(Rb), 1 if gl else 0, nlat, self._p(latb), self._p(rb),
+                              self._p(Rb), 3 + dh if dh else (1 if gl else 0), nlat,
+                              self._p(latb), self._p(rb),
                               nlon, self._p(lonb), rp, nring, self._p(f))
         out = self.unpack(f, nout * nlon, exact)
         return out.reshape(nout, nlon)
