@@ -193,6 +193,20 @@ extern double charm_glob_threshold;
 extern double charm_glob_threshold2;
 extern unsigned long charm_glob_polar_optimization_a1;
 extern double charm_glob_polar_optimization_a2;
+/* The polar optimisation is tested jointly over blocks of consecutive rings
+ * (points): simd * charm_glob_shs_block_lat_multiplier in synthesis,
+ * simd * charm_glob_sha_block_lat_multiplier in analysis, with simd =
+ * charm_amd_polar_simd_size (src/misc/misc_polar_optimization_apply.c:16-41,
+ * src/shs/shs_point_grd.c:860-862, src/shs/shs_point_sctr.c:518-520,
+ * src/sha/sha_point.c:716-718).  The multipliers are CHarm's globals
+ * (src/glob/glob.h.in:231,241; defaults SIMD_BLOCK_A = 8, SIMD_BLOCK_S = 4,
+ * src/simd/simd.h:303-304); the SIMD size names the CHarm build whose blocks
+ * are reproduced: 8 for AVX-512 (the default), 4 for AVX/AVX2, 2 for
+ * SSE4.1/NEON, 1 for builds without SIMD (whose multipliers are both 8,
+ * src/simd/simd.h:609-611).  Read at every call. */
+extern size_t charm_glob_sha_block_lat_multiplier;
+extern size_t charm_glob_shs_block_lat_multiplier;
+extern int charm_amd_polar_simd_size;
 
 /* ------------------------------------------------------------------------ */
 /* Synthesis and analysis: src/shs/shs.h.in:168-173, src/sha/sha.h.in:164-169 */
