@@ -298,6 +298,20 @@ void charm_amd_shs_sctr_dev(size_t npoint, const double *lat_dev, const double *
                             const double *r_dev, unsigned long nmax, double mu, double R,
                             const double *cs_dev, double *f_dev, void *stream,
                             charm_err *err);
+/* No counterpart in CHarm.  charm_amd_shs_sctr_dev without any device
+ * allocation, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): every
+ * buffer beyond the inputs and f_dev lives in a static device array of
+ * 128 MiB per GPU, held until the process ends.  At most 1048576 points and
+ * degree 2190 (larger requests fail with CHARM_EFUNCARG).  The first call on
+ * a device builds the recurrence tables there and must not be under stream
+ * capture.  Calls that overlap on one GPU share that array and race: the
+ * caller serialises them.  Same results as charm_amd_shs_sctr_dev. */
+void charm_amd_shs_sctr_dev_noalloc(size_t npoint, const double *lat_dev,
+                                    const double *lon_dev, const double *r_dev,
+                                    unsigned long nmax, double mu, double R,
+                                    const double *cs_dev, double *f_dev, void *stream,
+                                    charm_err *err);
 /* Derivative requests of the device-resident entry points below:
  * CHARM_AMD_DERIV_GRAD1 (3 outputs, as charm_shs_point_grad1),
  * CHARM_AMD_DERIV_GRAD2 (6 outputs, as charm_shs_point_grad2), or
