@@ -428,17 +428,14 @@ def batch_bench(args):
     import charm_amd  # noqa: F401
     from charm_amd import crd, sha, shc, shs
     from charm_amd.device import SHA_GRID, SHS_GRID, Batch, DevArray, DeviceGrid, Stream
-    from charm_amd.workloads import GM, R_EARTH, kaula_coeffs
+    from charm_amd.workloads import GM, R_EARTH, batch_mix, kaula_coeffs
 
-    rng = np.random.default_rng(64)
-    degs = rng.integers(90, 2701, 64)
     st = Stream()
     reqs, host = [], []
     units = 0.0
-    for i, N in enumerate(degs):
-        N = int(N)
-        kind = SHS_GRID if i % 2 == 0 else SHA_GRID
-        C, S = kaula_coeffs(N, 1000 + i)
+    for is_shs, N, seed in batch_mix():
+        kind = SHS_GRID if is_shs else SHA_GRID
+        C, S = kaula_coeffs(N, seed)
         g = DeviceGrid(N, R_EARTH)
         cs = DevArray.from_numpy(np.stack([C, S], axis=1))
         f = DevArray((g.nlat, g.nlon))

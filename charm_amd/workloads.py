@@ -45,3 +45,22 @@ def degree_order(nmax):
                       np.arange(nmax + 1, 0, -1))
     n_of = m_of + (np.arange(ncs) - start)
     return n_of, m_of
+
+
+def uniform_coeffs(nmax, seed):
+    """CHarm's accuracy protocol (bench/bench.c:192-225): C_nm, S_nm from
+    U[-1, 1), S_n0 = 0, m-major triangles of degree nmax."""
+    rng = np.random.default_rng(seed)
+    ncs = (nmax + 1) * (nmax + 2) // 2
+    C = rng.uniform(-1.0, 1.0, ncs)
+    S = rng.uniform(-1.0, 1.0, ncs)
+    S[: nmax + 1] = 0.0
+    return C, S
+
+
+def batch_mix(count=64, seed=64):
+    """The mixed-degree request stream of the batched workload: request i has
+    degree N_i ~ U{90..2700}, synthesis for even i and analysis for odd i, and
+    Kaula coefficients of seed 1000 + i.  Returns [(is_synthesis, N_i, seed_i)]."""
+    degs = np.random.default_rng(seed).integers(90, 2701, count)
+    return [(i % 2 == 0, int(N), 1000 + i) for i, N in enumerate(degs)]

@@ -19,7 +19,7 @@ def header_symbols():
     txt = open(os.path.join(ROOT, "include", "charm_amd.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     funcs = re.findall(r"\b(charm_[a-z0-9_]+)\s*\(", txt)
-    data = re.findall(r"extern\s+[a-z ]+\s+(charm_[a-z0-9_]+)\s*;", txt)
+    data = re.findall(r"extern\s+[a-z_ ]+\s+(charm_[a-z0-9_]+)\s*;", txt)
     return sorted(set(funcs)), sorted(set(data))
 
 
@@ -39,6 +39,40 @@ def test_struct_layouts_match_charm():
     assert crd._Point.lat.offset == 32 and crd._Point.owner.offset == 64
     assert ct.sizeof(charm_amd._err._Err) == 56
     assert charm_amd._err._Err.msg.offset == 40
+
+
+# CHarm's non-MPI x86-64 layouts (src/shc/shc.h.in:139-381,
+# src/crd/crd.h.in:125-332, src/err/err.h.in:159-222): size, then offsets
+CHARM_LAYOUTS = {
+    "charm_shc": (64, [("nmax", 0), ("mu", 8), ("r", 16), ("nc", 24), ("ns", 32), ("c", 40),
+                       ("s", 48), ("owner", 56), ("distributed", 57)]),
+    "charm_point": (72, [("type", 0), ("nlat", 8), ("nlon", 16), ("npoint", 24), ("lat", 32),
+                         ("lon", 40), ("r", 48), ("w", 56), ("owner", 64),
+                         ("distributed", 65)]),
+    "charm_err": (56, [("level", 0), ("file", 8), ("line", 16), ("func", 24), ("code", 32),
+                       ("msg", 40), ("saturated", 48), ("distributed", 49)]),
+}
+
+
+def test_header_struct_layouts_compiled(tmp_path):
+    """The C header itself, compiled with gcc, has CHarm's struct layouts."""
+    import subprocess
+    body = ""
+    for name, (_, fields) in CHARM_LAYOUTS.items():
+        body += '    printf("%%zu", sizeof(%s));\n' % name
+        body += "".join('    printf(" %%zu", offsetof(%s, %s));\n' % (name, f) for f, _ in fields)
+        body += '    printf("\\n");\n'
+    src = tmp_path / "charm_layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "charm_amd.h"\n'
+                   "int main(void) {\n" + body + "    return 0;\n}\n")
+    exe = tmp_path / "charm_layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    lines = subprocess.run([str(exe)], check=True, capture_output=True,
+                           text=True).stdout.split("\n")
+    for line, (name, (size, fields)) in zip(lines, CHARM_LAYOUTS.items()):
+        got = [int(v) for v in line.split()]
+        assert got == [size] + [o for _, o in fields], name
 
 
 def test_request_layout_matches_header(tmp_path):
@@ -69,6 +103,28 @@ def test_globals_default_values():
     assert ct.c_double.in_dll(lib, "charm_glob_threshold2").value == 1e5 * eps
     assert ct.c_ulong.in_dll(lib, "charm_glob_polar_optimization_a1").value == 100
     assert ct.c_double.in_dll(lib, "charm_glob_polar_optimization_a2").value == -1.0
+    # polar-optimisation blocks of CHarm's AVX-512 double build (src/simd/simd.h:303-304)
+    assert ct.c_size_t.in_dll(lib, "charm_glob_shs_block_lat_multiplier").value == 4
+    assert ct.c_size_t.in_dll(lib, "charm_glob_sha_block_lat_multiplier").value == 8
+    assert ct.c_int.in_dll(lib, "charm_amd_polar_simd_size").value == 8
+
+
+def test_noalloc_rejects_sizes_beyond_its_limits():
+    """The allocation-free entry point checks its limits before any device
+    work (null device pointers are never touched)."""
+    from charm_amd import _err
+    f = charm_amd.lib().charm_amd_shs_sctr_dev_noalloc
+    f.restype = None
+    f.argtypes = [ct.c_size_t, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_ulong, ct.c_double,
+                  ct.c_double, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.POINTER(_err._Err)]
+    for npoint, nmax in ((1 << 20) + 1, 10), (100, 2191):
+        err = _err.init()
+        try:
+            f(npoint, None, None, None, nmax, 1.0, 1.0, None, None, None, err)
+            assert err.contents.code == 2   # CHARM_EFUNCARG
+            assert b"at most 1048576 points and degree 2190" in ct.string_at(err.contents.msg)
+        finally:
+            _err.free(err)
 
 
 @pytest.mark.parametrize("nmax", range(0, 11))
