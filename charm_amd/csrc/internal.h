@@ -310,9 +310,10 @@ constexpr int SCTR_MAX_CHUNKS = 64;
 int sctr_points_per_lane(size_t npoint);
 int sctr_order_chunks(size_t npoint, unsigned long nmax, int *mc);  // mc: nchunk + 1 entries
 
-// Ring pairs per lane for nring rings, kmin <= k <= kmax: the fewest idle
-// lanes in the last ring tile, then the most rings per lane.
-int ring_k(int nring, int kmin, int kmax);
+// Run-time ring-tile tuning (CHARM_AMD_TUNE=1) on or off for the calling
+// thread; the serving worker switches it off.
+void set_thread_tuning(bool on);
+
 
 }  // namespace charm_amd
 

@@ -1680,23 +1680,6 @@ int launch_rpows(int nrow, const double *ratio, int kmax, size_t rld, double *rp
     return (int)hipGetLastError();
 }
 
-int ring_k(int nring, int kmin, int kmax)
-{
-    // rings per lane: the fewest idle lanes in the last tile, then the most
-    // rings per lane
-    int best = kmax;
-    long best_cap = -1;
-    for (int k = kmax; k >= kmin; k--) {
-        const long tiles = (nring + 64L * k - 1) / (64L * k);
-        const long cap = tiles * 64L * k;
-        if (best_cap < 0 || cap < best_cap) {
-            best_cap = cap;
-            best = k;
-        }
-    }
-    return best;
-}
-
 template <int K>
 static void shs_launch_k(const ShsGridArgs &a, unsigned nblk, hipStream_t s)
 {

@@ -605,16 +605,28 @@ struct DevGrid {
 };
 
 // ---------------------------------------------------------------------------
-// Ring-tile autotuner: on the first large transform of a given kind, ring
-// count and degree, every candidate number of ring pairs per lane is timed
-// with HIP events on the caller's stream (after one warm-up launch each, the
-// fastest of three launches taken in interleaved rounds, so that clock drift
-// favours no candidate) and the fastest is kept for the process.  Small transforms and streams under graph capture use the
-// heuristic of ring_k.  CHARM_AMD_SHS_K / CHARM_AMD_SHA_K force a value.
+// Ring pairs per lane K of the grid kernels.
+//
+// By default a rule fitted on MI355X to the kernel times of every K at 18
+// degrees from 64 to 10800 on Gauss-Legendre grids (`python -m
+// charm_amd.tune`, which reruns that sweep): few rings leave the chip short
+// of waves, so small grids want few rings per lane; large grids amortise the
+// per-degree work of a wave (column staging, analysis's cross-lane reduction)
+// over more rings.  In ring pairs n: synthesis K = 1 up to 450, 2 up to 2500,
+// 3 beyond; analysis K = 2 up to 450, 4 beyond.
+//
+// With CHARM_AMD_TUNE=1 an autotuner refines the rule at run time: the first
+// large transform of a kind in a ring-count bucket (a quarter octave) times
+// every candidate with HIP events on the caller's stream (one warm-up launch
+// each, then the fastest of three launches in interleaved rounds) and keeps
+// the fastest for the bucket.  It never runs under stream capture or on a
+// thread that has switched tuning off (the serving worker), and
+// CHARM_AMD_SHS_K / CHARM_AMD_SHA_K force a value.
 // ---------------------------------------------------------------------------
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int>, int> g_tuned;
+std::map<std::pair<int, int>, int> g_tuned;
 std::atomic<int> g_last_k[2];
+thread_local bool t_no_tune = false;
 
 int forced_k(const char *env, int kmin, int kmax)
 {
@@ -624,23 +636,37 @@ int forced_k(const char *env, int kmin, int kmax)
     return (k >= kmin && k <= kmax) ? k : 0;
 }
 
+int fitted_k(int kind, size_t nring)
+{
+    if (kind == 0) return nring <= 450 ? 1 : (nring <= 2500 ? 2 : 3);
+    return nring <= 450 ? 2 : 4;
+}
+
+bool tuning_on()
+{
+    const char *v = std::getenv("CHARM_AMD_TUNE");
+    return v && v[0] == '1' && !t_no_tune;
+}
+
 template <class Launch>
 int tuned_k(int kind, int nring, unsigned long nmax, int kmin, int kmax, hipStream_t s,
             Launch launch)
 {
     if (int k = forced_k(kind == 0 ? "CHARM_AMD_SHS_K" : "CHARM_AMD_SHA_K", kmin, kmax)) return k;
-    const int heuristic = (kind == 0) ? 2 : ring_k(nring, kmin, kmax);
+    const int rule = fitted_k(kind, (size_t)nring);
     const double work = (double)nring * (double)(nmax + 1) * (double)(nmax + 1);
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    if (work < 2e8 || hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
-        return heuristic;
-    const auto key = std::make_tuple(kind, nring, (int)nmax);
+    if (!tuning_on() || work < 2e8 || hipStreamIsCapturing(s, &cap) != hipSuccess ||
+        cap != hipStreamCaptureStatusNone)
+        return rule;
+    const int bucket = (int)std::lround(4.0 * std::log2((double)std::max(nring, 1)));
+    const auto key = std::make_pair(kind, bucket);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(key);
         if (it != g_tuned.end()) return it->second;
     }
-    int best = heuristic;
+    int best = rule;
     float best_ms = 0.0f;
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
@@ -674,8 +700,8 @@ int tuned_k(int kind, int nring, unsigned long nmax, int kmin, int kmax, hipStre
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (std::getenv("CHARM_AMD_TUNE_LOG"))
-        std::fprintf(stderr, "charm_amd tune: kind %d nring %d nmax %lu -> K=%d (%.3f ms)\n", kind,
-                     nring, nmax, best, (double)best_ms);
+        std::fprintf(stderr, "charm_amd tune: kind %d ring bucket %d -> K=%d (%.3f ms; rule %d)\n",
+                     kind, bucket, best, (double)best_ms, rule);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[key] = best;
     return best;
@@ -1501,6 +1527,11 @@ void sha_host(const charm_point *pnt, const double *f, unsigned long nmax, charm
 }
 
 }  // namespace
+
+void set_thread_tuning(bool on)
+{
+    t_no_tune = !on;
+}
 
 // Plan of a scattered synthesis shared with the allocation-free entry point
 int sctr_points_per_lane(size_t npoint)

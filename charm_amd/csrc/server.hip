@@ -304,6 +304,7 @@ void finish(charm_amd_server *s, SrvReq *q)
 
 void worker_loop(charm_amd_server *s)
 {
+    set_thread_tuning(false);  // never time kernels against requests in flight
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
         bool progress = false;
