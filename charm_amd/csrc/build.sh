@@ -23,6 +23,6 @@ done
 for p in $pids; do wait "$p"; done
 "$ROCM/bin/hipcc" --offload-arch=gfx950 -shared -fPIC -o "$OUT.tmp" "$HERE/_build/kernels.o" \
   "$HERE/_build/fft.o" "$HERE/_build/multi.o" "$HERE/_build/point_api.o" "$HERE/_build/server.o" "$HERE/_build/gfm.o" "$HERE/_build/noalloc.o" "$HERE/_build/host_api.o" "$HERE/_build/shc_io.o" -L"$ROCM/lib" -lhipfft -lamdhip64 \
-  -Wl,-rpath,"$ROCM/lib" -lpthread
+  -Wl,-rpath,"$ROCM/lib" -lpthread -Wl,--no-undefined
 mv "$OUT.tmp" "$OUT"
 echo "built $OUT"

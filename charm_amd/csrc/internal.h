@@ -77,7 +77,8 @@ struct ShaArgs {
     const int *six;
     size_t sld;
     const unsigned char *mirror;
-    const double *cw;  // [nlat] quadrature weight times the global constant
+    const double *cw;  // [nlat] quadrature weights w; the rows are weighted by cws * w
+    double cws;        // the global constant of the analysis (sha_point.c:313-322)
     const void *spec;  // r2c spectrum of f, [row * lds + k] double2
     size_t lds;
     const double *al;  // alpha_nm, m-major, degree al_nt
@@ -251,6 +252,13 @@ int sctr_points_per_block(int k);
 // of bs consecutive input points, threshold thr >= 0.
 int launch_polar_points(size_t n, const double *lat_dev, const int *perm_dev, size_t bs, int nmax,
                         double thr, int *m0_dev, void *stream);
+// Polar optimisation on grid rings given by cos(lat) (device), blocks of bs
+// consecutive rings; (n + bs - 1) / bs doubles of workspace
+int launch_polar_rings_ws(size_t n, const double *u_dev, size_t bs, int nmax, double thr,
+                          double *umax_ws, int *m0_dev, void *stream);
+// out[k] = q^k for k = 0..n by repeated products in order; out[i] = v for i < n
+int launch_powers(int n, double q, double *out_dev, void *stream);
+int launch_fill(size_t n, double v, double *out_dev, void *stream);
 // The same with (n + bs - 1) / bs doubles of caller workspace
 int launch_polar_points_ws(size_t n, const double *lat_dev, const int *perm_dev, size_t bs,
                            int nmax, double thr, double *umax_ws, int *m0_dev, void *stream);

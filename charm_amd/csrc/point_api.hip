@@ -712,17 +712,22 @@ int tuned_k(int kind, int nring, unsigned long nmax, int kmin, int kmax, hipStre
 // device; p stays null when the optimisation is off.  The rings are CHarm's
 // "do" rings in its loop order (src/shs/shs_point_grd.c:594-600,860-862,
 // src/sha/sha_point.c:716-718).
+// The per-ring cos(lat) u_dev are the grid's (glibc cos on the host), and the
+// device applies CHarm's arithmetic exactly, so the result equals the host's
+// polar_ring_m0.
 struct PolarRings {
-    DevBuf buf;
+    DevBuf buf, umax;
     const int *p = nullptr;
-    PolarRings(const GridGeom &g, unsigned long nmax, int analysis, hipStream_t s)
+    PolarRings(const GridGeom &g, const double *u_dev, unsigned long nmax, int analysis,
+               hipStream_t s)
     {
         const double thr = polar_threshold(nmax);
         if (thr < 0.0 || g.nring == 0) return;
-        std::vector<int> m0(g.nring);
-        polar_ring_m0(g.u.data(), g.nring, nmax, thr, polar_block(analysis), m0.data());
+        const size_t bs = polar_block(analysis);
         buf.alloc(g.nring * sizeof(int), s);
-        h2d(buf.as<int>(), m0.data(), g.nring, s);
+        umax.alloc(((g.nring + bs - 1) / bs) * sizeof(double), s);
+        HIPCHK((hipError_t)launch_polar_rings_ws(g.nring, u_dev, bs, (int)nmax, thr,
+                                                 umax.as<double>(), buf.as<int>(), s));
         p = buf.as<int>();
     }
 };
@@ -749,7 +754,7 @@ void shs_grid_lumped(const GridGeom &g, const DevGrid &d, bool fft, unsigned lon
     a.mirror = d.mirror;
     a.al = tb.al;
     a.al_nt = tb.nt;
-    PolarRings pol(g, nmax, 0, stream);
+    PolarRings pol(g, d.u, nmax, 0, stream);
     a.pm0 = pol.p;
     // coefficients times s_nm for the scaled recurrence, as the kernel stages them
     a.cs = cs_dev;
@@ -925,9 +930,11 @@ std::vector<int> sctr_chunks(size_t npoint, unsigned long nmax)
 // Scattered synthesis on device arrays: lat/lon/r of npoint points, unscaled
 // coefficients cs (interleaved pairs, degree nmax).  Points are processed in
 // the order of |sin lat| (results go back to the caller's order).
+// unit_dev non-null: the radii test was made on the device (*unit_dev) and
+// overrides unit
 void run_sctr(size_t npoint, const double *lat_dev, const double *lon_dev, const double *r_dev,
               bool unit, unsigned long nmax, double mu, double R, const void *cs_dev,
-              const Tables &tb, double *f_dev, hipStream_t s)
+              const Tables &tb, double *f_dev, hipStream_t s, const int *unit_dev = nullptr)
 {
     const int K = sctr_k(npoint);
     DevBuf dt(npoint * sizeof(double), s), du(npoint * sizeof(double), s);
@@ -945,10 +952,13 @@ void run_sctr(size_t npoint, const double *lat_dev, const double *lon_dev, const
     } else {
         HIPCHK((hipError_t)launch_sincos(npoint, lat_dev, dt.as<double>(), du.as<double>(), s));
     }
-    const std::vector<int> mc = sctr_chunks(npoint, nmax);
-    const int nchunk = (int)mc.size() - 1;
+    // the order chunks go to the device as kernel arguments (no copy from
+    // host memory, which a stream under capture would record by address)
+    IntVals mcv{};
+    const int nchunk = sctr_order_chunks(npoint, nmax, mcv.v);
+    mcv.n = nchunk + 1;
     DevBuf dmc((nchunk + 1) * sizeof(int), s);
-    h2d(dmc.as<int>(), mc.data(), nchunk + 1, s);
+    HIPCHK((hipError_t)launch_set_ints(dmc.as<int>(), mcv, s));
     DevBuf part((size_t)nchunk * npoint * sizeof(double), s);
     SctrArgs a{};
     a.nmax = (int)nmax;
@@ -956,6 +966,7 @@ void run_sctr(size_t npoint, const double *lat_dev, const double *lon_dev, const
     a.nchunk = nchunk;
     a.mchunk = dmc.as<int>();
     a.unit = unit;
+    a.unit_dev = unit_dev;
     a.R = R;
     a.lat_t = dt.as<double>();
     a.lat_u = du.as<double>();
@@ -1386,9 +1397,24 @@ bool rescale_table(double r0, double R, unsigned long nmax_out, std::vector<doub
 // blocks an analysis launch aims at (a few waves of resident blocks)
 constexpr int SHA_BLOCKS = 4096;
 
-void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const double *f_dev,
-             unsigned long nmax, const Tables &tb, const double *scale_dev, unsigned long nmax_out,
-             double *c_dev, double *s_dev, size_t stride, hipStream_t s)
+// Device table of (r0/R)^n, n = 0..nmax_out (the rescale of src/shc/shc_rescale.c:24-90 with
+// mu unchanged), or nullptr when r0 equals R; built on the device, so a call under
+// stream capture copies nothing from host memory.
+struct RescaleTable {
+    DevBuf buf;
+    const double *p = nullptr;
+    RescaleTable(double r0, double R, unsigned long nmax_out, hipStream_t s)
+    {
+        if (nearly_equal(R, r0, charm_glob_threshold)) return;
+        buf.alloc((nmax_out + 1) * sizeof(double), s);
+        HIPCHK((hipError_t)launch_powers((int)nmax_out, r0 / R, buf.as<double>(), s));
+        p = buf.as<double>();
+    }
+};
+
+void run_sha(const GridGeom &g, const DevGrid &d, const double *w_dev, double cws,
+             const double *f_dev, unsigned long nmax, const Tables &tb, const double *scale_dev,
+             unsigned long nmax_out, double *c_dev, double *s_dev, size_t stride, hipStream_t s)
 {
     const size_t nlat = g.nlat, nlon = g.nlon, nfc = nlon / 2 + 1;
     DevBuf spec(nlat * nfc * 2 * sizeof(double), s);
@@ -1405,12 +1431,13 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
     a.six = d.six;
     a.sld = d.sld;
     a.mirror = d.mirror;
-    a.cw = cw_dev;
+    a.cw = w_dev;
+    a.cws = cws;
     a.spec = spec.p;
     a.lds = nfc;
     a.al = tb.al;
     a.al_nt = tb.nt;
-    PolarRings pol(g, nmax, 1, s);
+    PolarRings pol(g, d.u, nmax, 1, s);
     a.pm0 = pol.p;
     // Ring tiles are grouped so that the launch has about SHA_BLOCKS blocks:
     // each block adds its group's tiles, in order, into one partial slot.
@@ -1420,9 +1447,10 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *cw_dev, const do
     // is 5% faster than two slots (half the partial-sum writes, one slot to
     // reduce), while at nmax = 2160 fewer, longer blocks are 10% slower.
     const int nquad = (int)(nmax + 1 + 3) / 4;
-    const int nslot = nquad >= SHA_BLOCKS / 2
+    int nslot = nquad >= SHA_BLOCKS / 2
                           ? 1
                           : std::max(1, std::min(ring_tiles(g.nring, 2), (SHA_BLOCKS + nquad - 1) / nquad));
+    if (const char *v = std::getenv("CHARM_AMD_SHA_SLOTS")) nslot = std::max(1, std::atoi(v));
     auto group = [&](ShaArgs &y) {
         y.ntile = ring_tiles(g.nring, y.k);
         y.tpg = (y.ntile + nslot - 1) / nslot;
@@ -1494,14 +1522,12 @@ void sha_host(const charm_point *pnt, const double *f, unsigned long nmax, charm
     Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
     const size_t nr = g.nring;
     const double c = sha_const(type, nlat, r0, shcs->mu);
-    std::vector<double> cw(nlat);
-    for (size_t i = 0; i < nlat; i++) cw[i] = c * pnt->w[i];
     DevBuf dt(nr * sizeof(double), s), du(nr * sizeof(double), s), dmir(nr, s);
-    DevBuf dcw(nlat * sizeof(double), s);
+    DevBuf dw(nlat * sizeof(double), s);
     h2d(dt.as<double>(), g.t.data(), nr, s);
     h2d(du.as<double>(), g.u.data(), nr, s);
     h2d(dmir.as<unsigned char>(), g.mirror.data(), nr, s);
-    h2d(dcw.as<double>(), cw.data(), nlat, s);
+    h2d(dw.as<double>(), pnt->w, nlat, s);
     const size_t sld = nr;
     DevBuf sx(std::max<size_t>(nmax, 1) * sld * sizeof(double), s);
     DevBuf six(std::max<size_t>(nmax, 1) * sld * sizeof(int), s);
@@ -1511,15 +1537,10 @@ void sha_host(const charm_point *pnt, const double *f, unsigned long nmax, charm
               six.as<int>(), sld, nullptr, nullptr, 0};
     DevBuf df(nlat * nlon * sizeof(double), s);
     h2d(df.as<double>(), f, nlat * nlon, s);
-    std::vector<double> sc;
-    DevBuf dsc;
-    if (rescale_table(r0, shcs->r, shcs->nmax, sc)) {
-        dsc.alloc(sc.size() * sizeof(double), s);
-        h2d(dsc.as<double>(), sc.data(), sc.size(), s);
-    }
+    RescaleTable sc(r0, shcs->r, shcs->nmax, s);
     const size_t nc = tri_len(shcs->nmax);
     DevBuf dc(nc * sizeof(double), s), ds(nc * sizeof(double), s);
-    run_sha(g, d, dcw.as<double>(), df.as<double>(), nmax, tb, dsc.as<double>(), shcs->nmax,
+    run_sha(g, d, dw.as<double>(), c, df.as<double>(), nmax, tb, sc.p, shcs->nmax,
             dc.as<double>(), ds.as<double>(), 1, s);
     HIPCHK(hipMemcpyAsync(shcs->c[0], dc.p, nc * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(shcs->s[0], ds.p, nc * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1589,7 +1610,6 @@ struct charm_amd_grid {
     int *six = nullptr;
     unsigned char *mirror = nullptr;
     size_t sld = 0;
-    std::vector<double> wh;
     mutable charm_amd::DeadFlags dead;  // a cache: logically const
 };
 
@@ -1801,7 +1821,6 @@ charm_amd_grid *charm_amd_grid_create(const charm_point *pnt, charm_err *err)
         grd->type = pnt->type;
         grd->r0 = r0;
         grd->nmax_grd = pnt->nlat - 1;
-        grd->wh.assign(pnt->w, pnt->w + pnt->nlat);
         hipStream_t s = thread_stream();
         const size_t nr = grd->g.nring, nlat = pnt->nlat;
         const unsigned long nm = std::max<unsigned long>(grd->nmax_grd, 1);
@@ -1870,16 +1889,13 @@ void charm_amd_shs_grid_dev(const charm_amd_grid *grd, unsigned long nmax, doubl
                   &grd->dead};
         DevBuf ratio, rp;
         if (!nearly_equal(grd->r0, R, charm_glob_threshold)) {
-            // one radius for all rows: a single column of powers (row stride 0)
+            // one radius for all rows: a single column of powers (row stride
+            // 0), made on the device (no host copy under stream capture)
             const double q = R / grd->r0;
-            std::vector<double> pw(nmax + 2);
-            pw[0] = 1.0;
-            for (unsigned long k = 1; k <= nmax + 1; k++) pw[k] = pw[k - 1] * q;
-            std::vector<double> qq(grd->g.nlat, q);
-            ratio.alloc(qq.size() * sizeof(double), s);
-            rp.alloc(pw.size() * sizeof(double), s);
-            h2d(ratio.as<double>(), qq.data(), qq.size(), s);
-            h2d(rp.as<double>(), pw.data(), pw.size(), s);
+            ratio.alloc(grd->g.nlat * sizeof(double), s);
+            rp.alloc((nmax + 2) * sizeof(double), s);
+            HIPCHK((hipError_t)launch_fill(grd->g.nlat, q, ratio.as<double>(), s));
+            HIPCHK((hipError_t)launch_powers((int)nmax + 1, q, rp.as<double>(), s));
             d.ratio = ratio.as<double>();
             d.rp = rp.as<double>();
             d.rld = 0;
@@ -1904,20 +1920,10 @@ void charm_amd_sha_grid_dev(const charm_amd_grid *grd, unsigned long nmax, doubl
         Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
         const size_t nlat = grd->g.nlat;
         const double c = sha_const(grd->type, nlat, grd->r0, mu);
-        std::vector<double> cw(nlat);
-        for (size_t i = 0; i < nlat; i++) cw[i] = c * grd->wh[i];
-        DevBuf dcw(nlat * sizeof(double), s);
-        h2d(dcw.as<double>(), cw.data(), nlat, s);
-        std::vector<double> sc;
-        DevBuf dsc;
-        if (rescale_table(grd->r0, R, nmax, sc)) {
-            dsc.alloc(sc.size() * sizeof(double), s);
-            h2d(dsc.as<double>(), sc.data(), sc.size(), s);
-        }
+        RescaleTable sc(grd->r0, R, nmax, s);
         DevGrid d{grd->t, grd->u, grd->mirror, grd->sx, grd->six, grd->sld, nullptr, nullptr, 0,
                   &grd->dead};
-        run_sha(grd->g, d, dcw.as<double>(), f_dev, nmax, tb, dsc.as<double>(), nmax, cs_dev,
-                cs_dev + 1, 2, s);
+        run_sha(grd->g, d, grd->w, c, f_dev, nmax, tb, sc.p, nmax, cs_dev, cs_dev + 1, 2, s);
         HIPCHK(hipGetLastError());
     } catch (const HipFail &e) {
         report(err, e, __func__);
@@ -1968,7 +1974,7 @@ void charm_amd_shs_grid_multi_dev(const charm_amd_grid *grd, unsigned long nmax,
         a.spec = spec.p;
         a.lds = nfc;
         a.mur = mu / R;
-        PolarRings pol(grd->g, nmax, 0, s);
+        PolarRings pol(grd->g, grd->u, nmax, 0, s);
         a.pm0 = pol.p;
         {
             ProfScope ps("shs_multi_kernel", s);
@@ -2006,10 +2012,14 @@ void charm_amd_shs_sctr_dev(size_t npoint, const double *lat_dev, const double *
     try {
         hipStream_t s = (hipStream_t)stream;
         Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
-        // radii equal to R within glob_threshold (src/shs/shs_r_eq_rref.c:17-30)
-        int unit = 0;
-        HIPCHK((hipError_t)launch_all_near(npoint, r_dev, R, charm_glob_threshold, &unit, s));
-        run_sctr(npoint, lat_dev, lon_dev, r_dev, unit != 0, nmax, mu, R, cs_dev, tb, f_dev, s);
+        // radii equal to R within glob_threshold (src/shs/shs_r_eq_rref.c:17-30),
+        // decided on the device: no host round trip, nothing a stream under
+        // capture could not record
+        DevBuf flag(sizeof(int), s);
+        HIPCHK((hipError_t)launch_all_near_dev(npoint, r_dev, R, charm_glob_threshold,
+                                               flag.as<int>(), s));
+        run_sctr(npoint, lat_dev, lon_dev, r_dev, false, nmax, mu, R, cs_dev, tb, f_dev, s,
+                 flag.as<int>());
         HIPCHK(hipGetLastError());
     } catch (const HipFail &e) {
         report(err, e, __func__);
@@ -2039,14 +2049,10 @@ void charm_amd_shs_grid_deriv_dev(const charm_amd_grid *grd, unsigned long nmax,
         const bool unit = nearly_equal(grd->r0, R, charm_glob_threshold);
         if (!unit) {
             const double q = R / grd->r0;
-            std::vector<double> pw(nmax + 2);
-            pw[0] = 1.0;
-            for (unsigned long k = 1; k <= nmax + 1; k++) pw[k] = pw[k - 1] * q;
-            std::vector<double> qq(grd->g.nlat, q);
-            ratio.alloc(qq.size() * sizeof(double), s);
-            rp.alloc(pw.size() * sizeof(double), s);
-            h2d(ratio.as<double>(), qq.data(), qq.size(), s);
-            h2d(rp.as<double>(), pw.data(), pw.size(), s);
+            ratio.alloc(grd->g.nlat * sizeof(double), s);
+            rp.alloc((nmax + 2) * sizeof(double), s);
+            HIPCHK((hipError_t)launch_fill(grd->g.nlat, q, ratio.as<double>(), s));
+            HIPCHK((hipError_t)launch_powers((int)nmax + 1, q, rp.as<double>(), s));
             d.ratio = ratio.as<double>();
             d.rp = rp.as<double>();
             d.rld = 0;
@@ -2363,10 +2369,8 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
                 a.six = g->six;
                 a.sld = g->sld;
                 a.mirror = g->mirror;
-                const double c = sha_const(g->type, nlat, g->r0, q.mu);
-                std::vector<double> cw(nlat);
-                for (size_t r = 0; r < nlat; r++) cw[r] = c * g->wh[r];
-                a.cw = dupload(b, cw, s);
+                a.cw = g->w;
+                a.cws = sha_const(g->type, nlat, g->r0, q.mu);
                 void *spec = dmalloc(b, nlat * nfc * 2 * sizeof(double));
                 a.spec = spec;
                 a.lds = nfc;
