@@ -1447,10 +1447,9 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *w_dev, double cw
     // is 5% faster than two slots (half the partial-sum writes, one slot to
     // reduce), while at nmax = 2160 fewer, longer blocks are 10% slower.
     const int nquad = (int)(nmax + 1 + 3) / 4;
-    int nslot = nquad >= SHA_BLOCKS / 2
+    const int nslot = nquad >= SHA_BLOCKS / 2
                           ? 1
                           : std::max(1, std::min(ring_tiles(g.nring, 2), (SHA_BLOCKS + nquad - 1) / nquad));
-    if (const char *v = std::getenv("CHARM_AMD_SHA_SLOTS")) nslot = std::max(1, std::atoi(v));
     auto group = [&](ShaArgs &y) {
         y.ntile = ring_tiles(g.nring, y.k);
         y.tpg = (y.ntile + nslot - 1) / nslot;
