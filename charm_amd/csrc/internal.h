@@ -318,6 +318,27 @@ constexpr int SCTR_MAX_CHUNKS = 64;
 int sctr_points_per_lane(size_t npoint);
 int sctr_order_chunks(size_t npoint, unsigned long nmax, int *mc);  // mc: nchunk + 1 entries
 
+// Device bytes one scattered synthesis or one grid synthesis/analysis
+// allocates internally (for the server's admission by footprint), the bytes
+// the library keeps cached for reuse, and a trim of that cache.
+size_t sctr_call_scratch(size_t npoint, unsigned long nmax);
+size_t grid_call_scratch(int analysis, size_t nring, size_t nlat, size_t nlon, unsigned long nmax);
+size_t scratch_cached_bytes();
+void scratch_trim(size_t keep_bytes);
+
+// Several scattered syntheses in one persistent launch (each as
+// charm_amd_shs_sctr_dev computes it, so with the same sums); device pointers,
+// enqueued on stream.  Returns a hipError_t.
+struct SctrJob {
+    size_t npoint;
+    const double *lat, *lon, *r;
+    unsigned long nmax;
+    double mu, R;
+    const double *cs;  // (C, S) pairs of degree nmax
+    double *f;
+};
+int run_sctr_jobs(const SctrJob *jobs, int njob, void *stream);
+
 // Run-time ring-tile tuning (CHARM_AMD_TUNE=1) on or off for the calling
 // thread; the serving worker switches it off.
 void set_thread_tuning(bool on);

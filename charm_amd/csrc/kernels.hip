@@ -1515,7 +1515,7 @@ __global__ __launch_bounds__(256) void sctr_batch_kernel(const SctrArgs *__restr
         const int2 w = items[it];
         const SctrArgs A = reqs[uniform(w.x)];
         const int wy = uniform(w.y);
-        if (A.unit)
+        if (A.unit_dev ? (*A.unit_dev != 0) : (A.unit != 0))
             sctr_item<K, true>(A, wy >> 6, wy & 63, sm);
         else
             sctr_item<K, false>(A, wy >> 6, wy & 63, sm);

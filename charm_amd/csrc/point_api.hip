@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <mutex>
 #include <tuple>
@@ -211,7 +212,7 @@ struct DevBuf {
             auto sit = g_sc.find(st);
             if (sit != g_sc.end()) {
                 auto it = sit->second.lower_bound(bytes);
-                if (it != sit->second.end() && it->first <= 2 * bytes) {
+                if (it != sit->second.end() && it->first <= bytes + bytes / 4) {
                     const CachedBuf b = it->second;
                     sit->second.erase(it);
                     g_sc_bytes -= b.bytes;
@@ -1412,6 +1413,22 @@ struct RescaleTable {
     }
 };
 
+// Partial slots of an analysis: ring tiles are grouped so that the launch has
+// about SHA_BLOCKS blocks, each block adding its group's tiles, in order, into
+// one slot.  Few slots keep the partial sums small at high degree (one per
+// tile would be 20-40 GB at nmax = 10800).  From SHA_BLOCKS / 2 order quads
+// on, one slot (every block runs all tiles of its quad): at nmax = 10800 that
+// is 5% faster than two slots (half the partial-sum writes, one slot to
+// reduce), while at nmax = 2160 fewer, longer blocks are slower (5 slots:
+// 0.54 ms, 3: 0.60 ms, 1: 1.14 ms).
+int sha_slots(size_t nring, unsigned long nmax)
+{
+    const int nquad = (int)(nmax + 1 + 3) / 4;
+    return nquad >= SHA_BLOCKS / 2
+               ? 1
+               : std::max(1, std::min(ring_tiles(nring, 2), (SHA_BLOCKS + nquad - 1) / nquad));
+}
+
 void run_sha(const GridGeom &g, const DevGrid &d, const double *w_dev, double cws,
              const double *f_dev, unsigned long nmax, const Tables &tb, const double *scale_dev,
              unsigned long nmax_out, double *c_dev, double *s_dev, size_t stride, hipStream_t s)
@@ -1439,17 +1456,7 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *w_dev, double cw
     a.al_nt = tb.nt;
     PolarRings pol(g, d.u, nmax, 1, s);
     a.pm0 = pol.p;
-    // Ring tiles are grouped so that the launch has about SHA_BLOCKS blocks:
-    // each block adds its group's tiles, in order, into one partial slot.
-    // Few slots keep the partial sums small at high degree (one per tile
-    // would be 20-40 GB at nmax = 10800).  From SHA_BLOCKS / 2 order quads on,
-    // one slot (every block runs all tiles of its quad): at nmax = 10800 that
-    // is 5% faster than two slots (half the partial-sum writes, one slot to
-    // reduce), while at nmax = 2160 fewer, longer blocks are 10% slower.
-    const int nquad = (int)(nmax + 1 + 3) / 4;
-    const int nslot = nquad >= SHA_BLOCKS / 2
-                          ? 1
-                          : std::max(1, std::min(ring_tiles(g.nring, 2), (SHA_BLOCKS + nquad - 1) / nquad));
+    const int nslot = sha_slots(g.nring, nmax);
     auto group = [&](ShaArgs &y) {
         y.ntile = ring_tiles(g.nring, y.k);
         y.tpg = (y.ntile + nslot - 1) / nslot;
@@ -1551,6 +1558,138 @@ void sha_host(const charm_point *pnt, const double *f, unsigned long nmax, charm
 void set_thread_tuning(bool on)
 {
     t_no_tune = !on;
+}
+
+size_t sctr_call_scratch(size_t npoint, unsigned long nmax)
+{
+    int mc[SCTR_MAX_CHUNKS + 1];
+    const int nchunk = sctr_order_chunks(npoint, nmax, mc);
+    size_t b = npoint * (4 * sizeof(double) + sizeof(int));   // sorted t, u, lon, r; permutation
+    b += sort_points_ws_bytes(npoint);
+    b += (size_t)nchunk * npoint * sizeof(double);           // partial sums per order chunk
+    b += npoint * (sizeof(int) + sizeof(double));            // polar optimisation
+    return b + 4096;                                         // order chunks, radii flag
+}
+
+size_t grid_call_scratch(int analysis, size_t nring, size_t nlat, size_t nlon, unsigned long nmax)
+{
+    size_t b = nlat * (nlon / 2 + 1) * 2 * sizeof(double);   // half-spectra of the rows
+    b += row_fft_scratch(nlon, nlat);
+    b += nring * (sizeof(int) + sizeof(double));             // polar optimisation
+    b += (nmax + 2 + nlat) * sizeof(double);                 // radius powers or rescale table
+    if (analysis) b += (size_t)sha_slots(nring, nmax) * tri_len(nmax) * 2 * sizeof(double);
+    return b + 4096;
+}
+
+size_t scratch_cached_bytes()
+{
+    std::lock_guard<std::mutex> lk(g_sc_mu);
+    return g_sc_bytes;
+}
+
+void scratch_trim(size_t keep_bytes)
+{
+    std::lock_guard<std::mutex> lk(g_sc_mu);
+    while (g_sc_bytes > keep_bytes) scratch_evict_oldest();
+}
+
+int run_sctr_jobs(const SctrJob *jobs, int njob, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    try {
+        unsigned long nt = 1;
+        for (int j = 0; j < njob; j++) nt = std::max(nt, jobs[j].nmax);
+        Tables tb = tables((int)nt, s);
+        const int K = batch_ring_k(2);
+        std::vector<std::unique_ptr<DevBuf>> keep;
+        auto buf = [&](size_t bytes) {
+            keep.emplace_back(new DevBuf(std::max<size_t>(bytes, 16), s));
+            return keep.back()->p;
+        };
+        std::vector<SctrArgs> args(njob);
+        struct Item {
+            long cost;
+            int2 w;
+        };
+        std::vector<Item> items;
+        std::vector<int> nchunk(njob, 0);
+        std::vector<const int *> perm(njob, nullptr);
+        for (int j = 0; j < njob; j++) {
+            const SctrJob &q = jobs[j];
+            const size_t np = q.npoint;
+            if (np == 0) continue;
+            if (np >= (size_t)INT32_MAX) return (int)hipErrorInvalidValue;
+            // as run_sctr does for one call: points by latitude, radii test on
+            // the device, the same order chunks (so the sums are the same)
+            int *pm = (int *)buf(np * sizeof(int));
+            double *t = (double *)buf(np * sizeof(double)), *u = (double *)buf(np * sizeof(double));
+            double *l = (double *)buf(np * sizeof(double)), *r = (double *)buf(np * sizeof(double));
+            HIPCHK((hipError_t)launch_sort_points(np, q.lat, q.lon, q.r, pm, t, u, l, r, s));
+            int *flag = (int *)buf(sizeof(int));
+            HIPCHK((hipError_t)launch_all_near_dev(np, q.r, q.R, charm_glob_threshold, flag, s));
+            IntVals mcv{};
+            nchunk[j] = sctr_order_chunks(np, q.nmax, mcv.v);
+            mcv.n = nchunk[j] + 1;
+            int *dmc = (int *)buf(mcv.n * sizeof(int));
+            HIPCHK((hipError_t)launch_set_ints(dmc, mcv, s));
+            SctrArgs &a = args[j];
+            a.nmax = (int)q.nmax;
+            a.npoint = np;
+            a.nchunk = nchunk[j];
+            a.mchunk = dmc;
+            a.unit_dev = flag;
+            a.R = q.R;
+            a.lat_t = t;
+            a.lat_u = u;
+            a.lon = l;
+            a.r = r;
+            a.dm = tb.dm;
+            a.al = tb.al;
+            a.al_nt = tb.nt;
+            a.cs = q.cs;
+            a.sn = tb.sn;
+            a.part = (double *)buf((size_t)nchunk[j] * np * sizeof(double));
+            a.k = K;
+            const double thr = polar_threshold(q.nmax);
+            if (thr >= 0.0) {
+                int *m0 = (int *)buf(np * sizeof(int));
+                double *um = (double *)buf(np * sizeof(double));
+                HIPCHK((hipError_t)launch_polar_points_ws(np, q.lat, pm, polar_block(0),
+                                                          (int)q.nmax, thr, um, m0, s));
+                a.pm0 = m0;
+            }
+            perm[j] = pm;
+            const int nblk = (int)((np + 256 * (size_t)K - 1) / (256 * (size_t)K));
+            for (int c = 0; c < nchunk[j]; c++) {
+                long work = 0;
+                for (int m = mcv.v[c]; m < mcv.v[c + 1]; m++) work += (long)q.nmax + 1 - m;
+                for (int bk = 0; bk < nblk; bk++) items.push_back({work, make_int2(j, (bk << 6) | c)});
+            }
+        }
+        if (items.empty()) return 0;
+        std::stable_sort(items.begin(), items.end(),
+                         [](const Item &x, const Item &y) { return x.cost > y.cost; });
+        std::vector<int2> w(items.size());
+        for (size_t i = 0; i < items.size(); i++) w[i] = items[i].w;
+        void *dargs = buf(args.size() * sizeof(SctrArgs));
+        void *ditems = buf(w.size() * sizeof(int2));
+        int *counter = (int *)buf(sizeof(int));
+        h2d((SctrArgs *)dargs, args.data(), args.size(), s);
+        h2d((int2 *)ditems, w.data(), w.size(), s);
+        HIPCHK((hipError_t)launch_zero_ints(counter, 1, s));
+        {
+            ProfScope ps("sctr_kernel", s);
+            HIPCHK((hipError_t)launch_sctr_batch(dargs, ditems, (int)w.size(), counter,
+                                                 batch_grid_size(2), s));
+        }
+        for (int j = 0; j < njob; j++)
+            if (jobs[j].npoint)
+                HIPCHK((hipError_t)launch_sctr_reduce(jobs[j].npoint, nchunk[j], args[j].part,
+                                                      perm[j], jobs[j].mu / jobs[j].R, jobs[j].f, s));
+        return 0;
+    } catch (const HipFail &e) {
+        return (int)e.code;
+    }
 }
 
 // Plan of a scattered synthesis shared with the allocation-free entry point

@@ -1,14 +1,22 @@
 // Request-serving engine: continuous admission of synthesis/analysis
 // requests against models and grids resident in HBM.
 //
-// A worker thread owns all device work.  Requests are admitted in arrival
-// order while the HBM budget and the in-flight limit allow; each admitted
-// request runs on its own stream from a pool, so requests that arrive while
-// others are on the GPU overlap with them.  Completion is detected with
-// events; results are copied out through pinned staging buffers, device
-// memory is returned to the stream-ordered pool, and the next requests are
-// admitted.  Results do not depend on what else is in flight: every kernel is
-// deterministic and requests share only read-only data.
+// A submitting thread does every piece of host work a request needs before it
+// is queued: it copies the request's input arrays into pinned staging
+// buffers and makes its Gauss-Legendre grid resident (host nodes and device
+// tables, kept in a cache under its own lock).  A worker thread owns the
+// device work: it admits queued requests while the HBM budget and the
+// in-flight limit allow -- in arrival order, a request that does not fit yet
+// letting later ones pass at most BYPASS_MAX times -- and runs each admitted
+// request on its own stream, so that requests arriving while others are on the
+// GPU overlap with them.  Small scattered-point requests admitted together
+// run as one group: one persistent launch over all their (point block, order
+// chunk) items (run_sctr_jobs), with each result equal to the request run
+// alone.  Completion is detected with events; charm_amd_server_wait copies
+// the result out of pinned staging in the waiting thread.  The worker never
+// builds grids, copies host arrays or times kernels.  Results do not depend
+// on what else is in flight: every kernel is deterministic and requests share
+// only read-only data.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,28 +35,46 @@ using namespace charm_amd;
 
 namespace {
 
+constexpr int BYPASS_MAX = 8;                  // later requests passing one that does not fit
+constexpr size_t GROUP_MAX_POINTS = 1 << 17;   // scattered requests grouped up to this size
+constexpr int GROUP_MAX = 32;                  // requests per group launch
+
 double now_ms()
 {
     using namespace std::chrono;
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
-struct Grid;
+struct Pinned {
+    void *p = nullptr;
+    size_t n = 0;
+    bool busy = false;
+};
+
+struct Grid {
+    charm_amd_grid *g = nullptr;
+    size_t nlat = 0, nlon = 0, nring = 0;
+    size_t bytes = 0;
+    int users = 0;  // queued or in-flight requests using it (under the server lock)
+    double last = 0;
+};
 
 struct SrvReq {
     long ticket = 0;
     charm_amd_srv_request r{};
     size_t bytes = 0;
-    int state = 0;  // 0 queued, 1 in flight, 2 done, 3 failed
+    int state = 0;  // 0 queued, 1 in flight, 2 done, 3 failed, 4 cancelled
     double t_submit = 0, t_start = 0, t_done = 0;
     int stream = -1;
-    hipEvent_t ev = nullptr;
+    hipEvent_t ev = nullptr;  // completion of the request's launch unit (its group's lead)
     std::vector<void *> dev;
-    void *pinned = nullptr;
-    size_t pinned_bytes = 0, out_bytes = 0;
+    Pinned *in = nullptr, *out = nullptr;
+    size_t out_bytes = 0;
+    Grid *grid = nullptr;
+    int bypassed = 0;
+    bool cancel = false;
     char msg[256] = {0};
     int code = 0;
-    struct Grid *grid = nullptr;  // resident grid in use (not evictable meanwhile)
 };
 
 struct Model {
@@ -63,37 +89,29 @@ struct GridKey {
     bool operator<(const GridKey &o) const { return nmax < o.nmax || (nmax == o.nmax && r < o.r); }
 };
 
-struct Grid {
-    charm_amd_grid *g = nullptr;
-    size_t nlat = 0, nlon = 0;
-    size_t bytes = 0;
-    int users = 0;
-    double last = 0;
-};
-
 }  // namespace
 
 struct charm_amd_server {
     size_t budget = 0, in_use = 0;
     int max_inflight = 4;
-    std::mutex mu;
+    std::mutex mu;  // queue, accounting, streams, requests
     std::condition_variable cv_work, cv_done;
     std::thread worker;
     bool stop = false;
     long next_ticket = 1;
     std::vector<Model> models;
-    std::map<GridKey, Grid> grids;
     std::deque<SrvReq *> pending;
-    std::vector<SrvReq *> inflight;
+    std::vector<std::vector<SrvReq *>> inflight;  // launch units (a request or a group)
     std::map<long, SrvReq *> all;
     std::vector<hipStream_t> streams;
     std::vector<bool> stream_busy;
-    struct Pinned {
-        void *p;
-        size_t n;
-        bool busy;
-    };
-    std::vector<Pinned> pinned;
+    int fail_next = 0;  // simulated device errors still to inject
+    int fail_code = 0;
+    charm_amd_srv_stats stats{};
+    std::mutex grid_mu;  // grid cache (lock order: grid_mu, then mu)
+    std::map<GridKey, Grid *> grids;
+    std::mutex pin_mu;  // pinned staging pool
+    std::vector<Pinned *> pinned;
 };
 
 namespace {
@@ -102,93 +120,106 @@ size_t gl_bytes(unsigned long nmax)
 {
     const size_t nlat = nmax + 1, nlon = 2 * (nmax + 1), nring = (nlat + 1) / 2;
     // latitude tables + seed table (8 B mantissa + 4 B exponent per (m, ring))
-    return nring * 32 + nlat * 16 + nlon * 8 + (size_t)std::max<unsigned long>(nmax, 1) * nring * 12;
+    return nring * 32 + nlat * 24 + nlon * 8 + (size_t)std::max<unsigned long>(nmax, 1) * nring * 12;
 }
 
-// device bytes a request needs while in flight (grid tables counted apart)
+// device bytes a request holds while in flight: its inputs and outputs in
+// HBM, the model truncated to its degree, and what the library's call
+// allocates internally (grid tables are counted with the grid)
 size_t footprint(const charm_amd_srv_request &r)
 {
     const size_t P = tri_len(r.nmax);
-    switch (r.kind) {
-    case CHARM_AMD_SRV_SHS_SCTR: {
-        const size_t nw = (r.npoint + 63) / 64;
-        const size_t nchunk = std::min<size_t>(64, std::max<size_t>(1, 4096 / std::max<size_t>(nw, 1)));
-        return r.npoint * 8 * (6 + nchunk) + P * 16;
-    }
-    case CHARM_AMD_SRV_SHS_GRID: {
-        const size_t nlat = r.grid_nmax + 1, nlon = 2 * nlat;
-        return nlat * nlon * 8 + nlat * (nlon / 2 + 1) * 16 + P * 16;
-    }
-    default: {
-        const size_t nlat = r.grid_nmax + 1, nlon = 2 * nlat, nring = (nlat + 1) / 2;
-        const size_t ntile = (nring + 191) / 192 + 1;
-        return nlat * nlon * 8 + nlat * (nlon / 2 + 1) * 16 + ntile * P * 16 + P * 16;
-    }
-    }
+    if (r.kind == CHARM_AMD_SRV_SHS_SCTR)
+        return r.npoint * 4 * sizeof(double) + P * 16 + sctr_call_scratch(r.npoint, r.nmax);
+    const size_t nlat = r.grid_nmax + 1, nlon = 2 * nlat, nring = (nlat + 1) / 2;
+    return nlat * nlon * sizeof(double) + P * 16 +
+           grid_call_scratch(r.kind == CHARM_AMD_SRV_SHA_GRID, nring, nlat, nlon, r.nmax);
 }
 
-void *pinned_get(charm_amd_server *s, size_t n)
+Pinned *pinned_get(charm_amd_server *s, size_t n)
 {
-    for (auto &p : s->pinned)
-        if (!p.busy && p.n >= n) {
-            p.busy = true;
-            return p.p;
+    n = std::max<size_t>(n, 64);
+    std::lock_guard<std::mutex> lk(s->pin_mu);
+    for (Pinned *p : s->pinned)
+        if (!p->busy && p->n >= n && p->n <= 2 * n) {
+            p->busy = true;
+            return p;
         }
-    void *p = nullptr;
-    if (hipHostMalloc(&p, std::max<size_t>(n, 64), hipHostMallocDefault) != hipSuccess) return nullptr;
-    s->pinned.push_back({p, std::max<size_t>(n, 64), true});
+    Pinned *p = new Pinned();
+    if (hipHostMalloc(&p->p, n, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        delete p;
+        return nullptr;
+    }
+    p->n = n;
+    p->busy = true;
+    s->pinned.push_back(p);
     return p;
 }
 
-void pinned_put(charm_amd_server *s, void *q)
+void pinned_put(charm_amd_server *s, Pinned *p)
 {
-    for (auto &p : s->pinned)
-        if (p.p == q) p.busy = false;
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(s->pin_mu);
+    p->busy = false;
 }
 
-// grid cache: returns a resident grid (creating it if needed), or nullptr
-Grid *get_grid(charm_amd_server *s, unsigned long nmax, double r, charm_err *err)
+// A resident grid for (nmax, r), built if needed (in the calling thread, a
+// submitter, never the worker).  Idle grids are evicted, least recently used
+// first, until the new one fits the budget.  The returned grid has one more
+// user (released when its request finishes).
+Grid *acquire_grid(charm_amd_server *s, unsigned long nmax, double r, charm_err *err)
 {
+    std::lock_guard<std::mutex> glk(s->grid_mu);
     GridKey k{nmax, r};
     auto it = s->grids.find(k);
     if (it != s->grids.end()) {
-        it->second.last = now_ms();
-        return &it->second;
+        std::lock_guard<std::mutex> lk(s->mu);
+        it->second->users++;
+        it->second->last = now_ms();
+        return it->second;
     }
-    // evict idle grids (least recently used first) until the new one fits
     const size_t need = gl_bytes(nmax);
-    while (s->in_use + need > s->budget) {
-        auto victim = s->grids.end();
-        for (auto g = s->grids.begin(); g != s->grids.end(); ++g)
-            if (g->second.users == 0 && (victim == s->grids.end() || g->second.last < victim->second.last))
-                victim = g;
-        if (victim == s->grids.end()) return nullptr;
-        s->in_use -= victim->second.bytes;
-        charm_amd_grid_free(victim->second.g);
-        s->grids.erase(victim);
+    for (;;) {
+        Grid *victim = nullptr;
+        GridKey vk{0, 0.0};
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (s->in_use + need <= s->budget) break;
+            for (auto &kv : s->grids)
+                if (kv.second->users == 0 && (!victim || kv.second->last < victim->last)) {
+                    victim = kv.second;
+                    vk = kv.first;
+                }
+            if (!victim) break;  // built anyway: requests queue until memory frees
+            s->in_use -= victim->bytes;
+        }
+        s->grids.erase(vk);
+        charm_amd_grid_free(victim->g);
+        delete victim;
     }
     charm_point *pnt = charm_crd_point_gl(nmax, r);
     if (!pnt) {
         CHARM_AMD_ERR(err, CHARM_EMEM, "Failed to build the Gauss--Legendre grid.");
         return nullptr;
     }
-    Grid g;
-    g.g = charm_amd_grid_create(pnt, err);
-    g.nlat = pnt->nlat;
-    g.nlon = pnt->nlon;
+    Grid *g = new Grid();
+    g->g = charm_amd_grid_create(pnt, err);
+    g->nlat = pnt->nlat;
+    g->nlon = pnt->nlon;
+    g->nring = (pnt->nlat + 1) / 2;
     charm_crd_point_free(pnt);
-    if (!g.g) return nullptr;
-    g.bytes = need;
-    g.last = now_ms();
+    if (!g->g) {
+        delete g;
+        return nullptr;
+    }
+    g->bytes = need;
+    g->last = now_ms();
+    std::lock_guard<std::mutex> lk(s->mu);
     s->in_use += need;
-    return &(s->grids[k] = g);
-}
-
-bool dalloc(SrvReq *q, hipStream_t st, size_t bytes, void **p)
-{
-    if (hipMallocAsync(p, std::max<size_t>(bytes, 16), st) != hipSuccess) return false;
-    q->dev.push_back(*p);
-    return true;
+    g->users = 1;
+    s->grids[k] = g;
+    return g;
 }
 
 void fail(SrvReq *q, int code, const char *msg)
@@ -198,108 +229,165 @@ void fail(SrvReq *q, int code, const char *msg)
     q->state = 3;
 }
 
-// launch a request's work on its stream (called by the worker, lock held)
-void start(charm_amd_server *s, SrvReq *q)
+bool dalloc(SrvReq *q, hipStream_t st, size_t bytes, void **p)
 {
-    const charm_amd_srv_request &r = q->r;
-    hipStream_t st = s->streams[q->stream];
+    if (hipMallocAsync(p, std::max<size_t>(bytes, 16), st) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    q->dev.push_back(*p);
+    return true;
+}
+
+bool groupable(const SrvReq *q)
+{
+    return q->r.kind == CHARM_AMD_SRV_SHS_SCTR && q->r.npoint > 0 &&
+           q->r.npoint <= GROUP_MAX_POINTS;
+}
+
+// Enqueue a launch unit (one request, or a group of small scattered requests)
+// on stream k.  Called by the worker without the server lock: the requests
+// are out of the queue and belong to the worker.
+void start(charm_amd_server *s, std::vector<SrvReq *> &unit, int k)
+{
+    hipStream_t st = s->streams[k];
     charm_err *err = charm_err_init();
-    q->t_start = now_ms();
-    const Model *md = (r.model >= 0 && r.model < (int)s->models.size()) ? &s->models[r.model] : nullptr;
-    const size_t P = tri_len(r.nmax);
-    bool ok = true;
-    if (r.kind == CHARM_AMD_SRV_SHS_SCTR) {
-        const size_t n = r.npoint;
-        void *lat, *lon, *rr, *f, *cs;
-        ok = md && r.nmax <= md->nmax && dalloc(q, st, n * 8, &lat) && dalloc(q, st, n * 8, &lon) &&
-             dalloc(q, st, n * 8, &rr) && dalloc(q, st, n * 8, &f) && dalloc(q, st, P * 16, &cs);
-        if (ok) {
-            q->out_bytes = n * 8;
-            q->pinned = pinned_get(s, 3 * n * 8);
-            ok = q->pinned != nullptr;
+    std::vector<SctrJob> jobs;
+    std::vector<SrvReq *> sct;
+    std::vector<Model> mds;  // copies: models may be added concurrently
+    int inject = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (SrvReq *q : unit) {
+            q->stream = k;
+            q->t_start = now_ms();
+            const int mi = q->r.model;
+            mds.push_back(mi >= 0 && mi < (int)s->models.size() ? s->models[mi] : Model());
         }
-        if (ok) {
-            double *pin = (double *)q->pinned;
-            std::memcpy(pin, r.lat, n * 8);
-            std::memcpy(pin + n, r.lon, n * 8);
-            std::memcpy(pin + 2 * n, r.r, n * 8);
+        if (s->fail_next > 0) {
+            s->fail_next--;
+            inject = s->fail_code ? s->fail_code : CHARM_EFUNCARG;
+        }
+    }
+    bool ok = inject == 0;
+    for (size_t u = 0; ok && u < unit.size(); u++) {
+        SrvReq *q = unit[u];
+        const charm_amd_srv_request &r = q->r;
+        const Model *md = &mds[u];
+        const size_t P = tri_len(r.nmax);
+        if (r.kind == CHARM_AMD_SRV_SHS_SCTR) {
+            const size_t n = r.npoint;
+            void *lat, *lon, *rr, *f, *cs;
+            ok = dalloc(q, st, n * 8, &lat) && dalloc(q, st, n * 8, &lon) &&
+                 dalloc(q, st, n * 8, &rr) && dalloc(q, st, n * 8, &f) && dalloc(q, st, P * 16, &cs);
+            if (!ok) break;
+            const double *pin = (const double *)q->in->p;
             ok = hipMemcpyAsync(lat, pin, n * 8, hipMemcpyHostToDevice, st) == hipSuccess &&
                  hipMemcpyAsync(lon, pin + n, n * 8, hipMemcpyHostToDevice, st) == hipSuccess &&
                  hipMemcpyAsync(rr, pin + 2 * n, n * 8, hipMemcpyHostToDevice, st) == hipSuccess;
             charm_amd_cs_truncate(md->cs, md->nmax, (double *)cs, r.nmax, st, err);
-            charm_amd_shs_sctr_dev(n, (double *)lat, (double *)lon, (double *)rr, r.nmax, md->mu, md->R,
-                                   (double *)cs, (double *)f, st, err);
-            ok = ok && charm_err_isempty(err) &&
-                 hipMemcpyAsync(pin, f, n * 8, hipMemcpyDeviceToHost, st) == hipSuccess;
-        }
-    } else if (r.kind == CHARM_AMD_SRV_SHS_GRID) {
-        Grid *g = md ? get_grid(s, r.grid_nmax, r.r_grid, err) : nullptr;
-        void *f, *cs;
-        ok = g && md && r.nmax <= md->nmax && r.nmax <= r.grid_nmax;
-        if (g) {
-            g->users++;
-            q->grid = g;
-        }
-        if (ok) ok = dalloc(q, st, g->nlat * g->nlon * 8, &f) && dalloc(q, st, P * 16, &cs);
-        if (ok) {
-            q->out_bytes = g->nlat * g->nlon * 8;
-            q->pinned = pinned_get(s, q->out_bytes);
-            ok = q->pinned != nullptr;
-        }
-        if (ok) {
+            ok = ok && charm_err_isempty(err);
+            jobs.push_back({n, (const double *)lat, (const double *)lon, (const double *)rr, r.nmax,
+                            md->mu, md->R, (const double *)cs, (double *)f});
+            sct.push_back(q);
+            q->out_bytes = n * 8;
+        } else if (r.kind == CHARM_AMD_SRV_SHS_GRID) {
+            Grid *g = q->grid;
+            void *f, *cs;
+            ok = dalloc(q, st, g->nlat * g->nlon * 8, &f) && dalloc(q, st, P * 16, &cs);
+            if (!ok) break;
             charm_amd_cs_truncate(md->cs, md->nmax, (double *)cs, r.nmax, st, err);
             charm_amd_shs_grid_dev(g->g, r.nmax, md->mu, md->R, (double *)cs, (double *)f, st, err);
+            q->out_bytes = g->nlat * g->nlon * 8;
             ok = charm_err_isempty(err) &&
-                 hipMemcpyAsync(q->pinned, f, q->out_bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
-        }
-    } else {
-        Grid *g = get_grid(s, r.grid_nmax, r.r_grid, err);
-        void *f, *cs;
-        ok = g && r.nmax <= r.grid_nmax;
-        if (g) {
-            g->users++;
-            q->grid = g;
-        }
-        if (ok) ok = dalloc(q, st, g->nlat * g->nlon * 8, &f) && dalloc(q, st, P * 16, &cs);
-        if (ok) {
-            q->out_bytes = P * 16;
-            q->pinned = pinned_get(s, std::max(q->out_bytes, g->nlat * g->nlon * 8));
-            ok = q->pinned != nullptr;
-        }
-        if (ok) {
-            std::memcpy(q->pinned, r.f_in, g->nlat * g->nlon * 8);
-            ok = hipMemcpyAsync(f, q->pinned, g->nlat * g->nlon * 8, hipMemcpyHostToDevice, st) ==
+                 hipMemcpyAsync(q->out->p, f, q->out_bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
+        } else {
+            Grid *g = q->grid;
+            void *f, *cs;
+            ok = dalloc(q, st, g->nlat * g->nlon * 8, &f) && dalloc(q, st, P * 16, &cs);
+            if (!ok) break;
+            ok = hipMemcpyAsync(f, q->in->p, g->nlat * g->nlon * 8, hipMemcpyHostToDevice, st) ==
                  hipSuccess;
             charm_amd_sha_grid_dev(g->g, r.nmax, r.mu, r.R, (double *)f, (double *)cs, st, err);
+            q->out_bytes = P * 16;
             ok = ok && charm_err_isempty(err) &&
-                 hipMemcpyAsync(q->pinned, cs, q->out_bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
+                 hipMemcpyAsync(q->out->p, cs, q->out_bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
         }
     }
-    if (ok) ok = hipEventRecord(q->ev, st) == hipSuccess;
-    if (!ok) {
-        (void)hipStreamSynchronize(st);
-        fail(q, charm_err_isempty(err) ? CHARM_EMEM : err->code,
-             charm_err_isempty(err) ? "Failed to admit the request on the device." : err->msg);
-    } else {
-        q->state = 1;
+    if (ok && !jobs.empty()) {
+        // one request: the plain device call; several: one persistent launch
+        if (jobs.size() == 1) {
+            const SctrJob &j = jobs[0];
+            charm_amd_shs_sctr_dev(j.npoint, j.lat, j.lon, j.r, j.nmax, j.mu, j.R, j.cs, j.f, st,
+                                   err);
+            ok = charm_err_isempty(err);
+        } else {
+            ok = run_sctr_jobs(jobs.data(), (int)jobs.size(), st) == hipSuccess;
+        }
+        for (size_t j = 0; ok && j < sct.size(); j++)
+            ok = hipMemcpyAsync(sct[j]->out->p, jobs[j].f, sct[j]->out_bytes,
+                                hipMemcpyDeviceToHost, st) == hipSuccess;
+    }
+    if (ok) ok = hipEventRecord(unit.front()->ev, st) == hipSuccess;
+    if (!ok) (void)hipStreamSynchronize(st);
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (SrvReq *q : unit) {
+        if (ok)
+            q->state = 1;
+        else if (inject)
+            fail(q, inject, "Simulated device error (charm_amd_server_inject_error).");
+        else
+            fail(q, charm_err_isempty(err) ? CHARM_EMEM : err->code,
+                 charm_err_isempty(err) ? "Failed to run the request on the device." : err->msg);
+    }
+    s->stats.launches++;
+    if (unit.size() > 1) {
+        s->stats.grouped += unit.size();
+        s->stats.max_group = std::max<unsigned long>(s->stats.max_group, unit.size());
     }
     charm_err_free(err);
 }
 
-void finish(charm_amd_server *s, SrvReq *q)
+// Completion of a launch unit, with the server lock held
+void finish(charm_amd_server *s, std::vector<SrvReq *> &unit)
 {
-    hipStream_t st = s->streams[q->stream];
-    if (q->state == 1 && q->pinned && q->r.out) std::memcpy(q->r.out, q->pinned, q->out_bytes);
-    for (void *p : q->dev) (void)hipFreeAsync(p, st);
-    q->dev.clear();
-    if (q->pinned) pinned_put(s, q->pinned);
-    q->pinned = nullptr;
-    if (q->grid) q->grid->users--;
-    q->grid = nullptr;
-    s->in_use -= q->bytes;
-    s->stream_busy[q->stream] = false;
-    if (q->state == 1) q->state = 2;
-    q->t_done = now_ms();
+    hipStream_t st = s->streams[unit.front()->stream];
+    for (SrvReq *q : unit) {
+        for (void *p : q->dev) (void)hipFreeAsync(p, st);
+        q->dev.clear();
+        pinned_put(s, q->in);
+        q->in = nullptr;
+        if (q->grid) q->grid->users--;
+        q->grid = nullptr;
+        s->in_use -= q->bytes;
+        if (q->state == 1) {
+            if (q->cancel) {
+                q->state = 4;
+                s->stats.cancelled++;
+            } else {
+                q->state = 2;
+                s->stats.completed++;
+            }
+        } else {
+            s->stats.failed++;
+        }
+        if (q->state != 2) {
+            pinned_put(s, q->out);
+            q->out = nullptr;
+        }
+        q->t_done = now_ms();
+    }
+    s->stream_busy[unit.front()->stream] = false;
+}
+
+// Admission test of a request of `bytes`, with the server lock held; the
+// library's cached scratch counts against the budget, so it is trimmed first
+bool fits(charm_amd_server *s, size_t bytes)
+{
+    if (s->in_use + bytes > s->budget) return false;
+    const size_t cached = scratch_cached_bytes();
+    if (s->in_use + bytes + cached > s->budget) scratch_trim(s->budget - s->in_use - bytes);
+    return true;
 }
 
 void worker_loop(charm_amd_server *s)
@@ -308,11 +396,12 @@ void worker_loop(charm_amd_server *s)
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
         bool progress = false;
-        // retire finished requests
+        // retire finished launch units
         for (size_t i = 0; i < s->inflight.size();) {
-            SrvReq *q = s->inflight[i];
-            if (q->state == 3 || hipEventQuery(q->ev) == hipSuccess) {
-                finish(s, q);
+            auto &unit = s->inflight[i];
+            SrvReq *lead = unit.front();
+            if (lead->state == 3 || hipEventQuery(lead->ev) == hipSuccess) {
+                finish(s, unit);
                 s->inflight.erase(s->inflight.begin() + i);
                 progress = true;
             } else {
@@ -320,23 +409,57 @@ void worker_loop(charm_amd_server *s)
             }
         }
         if (progress) s->cv_done.notify_all();
-        // admit in arrival order while the budget and a stream allow
-        while (!s->pending.empty() && (int)s->inflight.size() < s->max_inflight) {
-            SrvReq *q = s->pending.front();
-            if (s->in_use + q->bytes > s->budget && !s->inflight.empty()) break;
-            int k = -1;
-            for (int j = 0; j < (int)s->streams.size(); j++)
-                if (!s->stream_busy[j]) {
-                    k = j;
-                    break;
-                }
-            if (k < 0) break;
-            s->pending.pop_front();
-            s->stream_busy[k] = true;
-            q->stream = k;
+        // admit: in arrival order; a request that does not fit yet lets later
+        // ones pass, at most BYPASS_MAX of them; small scattered requests that
+        // fit together form one group launch (one group per pass)
+        std::vector<std::vector<SrvReq *>> launch;
+        std::vector<int> launch_stream;
+        std::vector<SrvReq *> skipped;
+        int nfree = 0;
+        for (bool b : s->stream_busy) nfree += !b;
+        for (auto it = s->pending.begin(); it != s->pending.end() && nfree > 0;) {
+            SrvReq *q = *it;
+            bool starving = false;
+            for (SrvReq *p : skipped) starving = starving || p->bypassed >= BYPASS_MAX;
+            if (starving) break;
+            // with nothing in flight a request runs whatever the budget says
+            // (its own bytes were checked against it at submission)
+            if (!fits(s, q->bytes) && !(s->inflight.empty() && launch.empty())) {
+                skipped.push_back(q);
+                ++it;
+                continue;
+            }
+            std::vector<SrvReq *> unit{q};
             s->in_use += q->bytes;
-            s->inflight.push_back(q);
-            start(s, q);
+            q->state = 1;  // out of the queue: cancelling now discards the result
+            it = s->pending.erase(it);
+            const bool group = groupable(q);
+            if (group)
+                for (auto jt = it; jt != s->pending.end() && (int)unit.size() < GROUP_MAX;) {
+                    SrvReq *p = *jt;
+                    if (groupable(p) && fits(s, p->bytes)) {
+                        s->in_use += p->bytes;
+                        p->state = 1;
+                        unit.push_back(p);
+                        jt = s->pending.erase(jt);
+                    } else {
+                        ++jt;
+                    }
+                }
+            for (SrvReq *p : skipped) p->bypassed += (int)unit.size();
+            int k = 0;
+            while (s->stream_busy[k]) k++;
+            s->stream_busy[k] = true;
+            nfree--;
+            launch.push_back(unit);
+            launch_stream.push_back(k);
+            if (group) break;  // the group scan may have erased what `it` points at
+        }
+        if (!launch.empty()) {
+            lk.unlock();
+            for (size_t u = 0; u < launch.size(); u++) start(s, launch[u], launch_stream[u]);
+            lk.lock();
+            for (auto &unit : launch) s->inflight.push_back(unit);
             progress = true;
         }
         if (s->stop && s->pending.empty() && s->inflight.empty()) break;
@@ -347,6 +470,12 @@ void worker_loop(charm_amd_server *s)
                 s->cv_work.wait_for(lk, std::chrono::microseconds(50));
         }
     }
+}
+
+void free_req(SrvReq *q)
+{
+    if (q->ev) (void)hipEventDestroy(q->ev);
+    delete q;
 }
 
 }  // namespace
@@ -387,21 +516,27 @@ int charm_amd_server_add_model(charm_amd_server *s, const charm_shc *shcs, unsig
             cs[2 * q] = shcs->c[m][n - m];
             cs[2 * q + 1] = (m == 0) ? 0.0 : shcs->s[m][n - m];
         }
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (s->in_use + P * 16 > s->budget) {
-        CHARM_AMD_ERR(err, CHARM_EMEM, "The model does not fit the HBM budget.");
-        return -1;
-    }
     Model md;
     md.nmax = nmax;
     md.mu = shcs->mu;
     md.R = shcs->r;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->in_use + P * 16 > s->budget) {
+            CHARM_AMD_ERR(err, CHARM_EMEM, "The model does not fit the HBM budget.");
+            return -1;
+        }
+        s->in_use += P * 16;
+    }
     if (hipMalloc((void **)&md.cs, P * 16) != hipSuccess ||
         hipMemcpy(md.cs, cs.data(), P * 16, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->in_use -= P * 16;
         CHARM_AMD_ERR(err, CHARM_EMEM, "Failed to make the model resident.");
         return -1;
     }
-    s->in_use += P * 16;
+    std::lock_guard<std::mutex> lk(s->mu);
     s->models.push_back(md);
     return (int)s->models.size() - 1;
 }
@@ -414,25 +549,84 @@ long charm_amd_server_submit(charm_amd_server *s, const charm_amd_srv_request *r
         CHARM_AMD_ERR(err, CHARM_EFUNCARG, "Invalid request.");
         return -1;
     }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        const bool shs = r->kind != CHARM_AMD_SRV_SHA_GRID;
+        const Model *md = (r->model >= 0 && r->model < (int)s->models.size()) ? &s->models[r->model]
+                                                                              : nullptr;
+        const bool bad = (shs && (!md || r->nmax > md->nmax)) ||
+                         (r->kind == CHARM_AMD_SRV_SHS_SCTR &&
+                          (!r->lat || !r->lon || !r->r || r->npoint >= (size_t)INT32_MAX)) ||
+                         (r->kind != CHARM_AMD_SRV_SHS_SCTR && r->nmax > r->grid_nmax) ||
+                         (r->kind == CHARM_AMD_SRV_SHA_GRID && !r->f_in);
+        if (bad) {
+            CHARM_AMD_ERR(err, CHARM_EFUNCARG, "Invalid request.");
+            return -1;
+        }
+    }
     SrvReq *q = new SrvReq();
     q->r = *r;
+    q->t_submit = now_ms();
     q->bytes = footprint(*r);
-    if (hipEventCreateWithFlags(&q->ev, hipEventDisableTiming) != hipSuccess) {
-        delete q;
-        CHARM_AMD_ERR(err, CHARM_EMEM, "Failed to create an event.");
-        return -1;
-    }
-    std::lock_guard<std::mutex> lk(s->mu);
     if (q->bytes > s->budget) {
-        (void)hipEventDestroy(q->ev);
         delete q;
         CHARM_AMD_ERR(err, CHARM_EMEM, "The request exceeds the HBM budget.");
         return -1;
     }
+    if (hipEventCreateWithFlags(&q->ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        q->ev = nullptr;
+        free_req(q);
+        CHARM_AMD_ERR(err, CHARM_EMEM, "Failed to create an event.");
+        return -1;
+    }
+    // host work in the submitting thread: inputs into pinned staging, output
+    // staging, the resident grid
+    bool ok = true;
+    if (r->kind == CHARM_AMD_SRV_SHS_SCTR) {
+        const size_t n = r->npoint;
+        q->in = pinned_get(s, 3 * n * sizeof(double));
+        q->out = pinned_get(s, n * sizeof(double));
+        ok = q->in && q->out;
+        if (ok) {
+            double *pin = (double *)q->in->p;
+            std::memcpy(pin, r->lat, n * 8);
+            std::memcpy(pin + n, r->lon, n * 8);
+            std::memcpy(pin + 2 * n, r->r, n * 8);
+        }
+    } else {
+        q->grid = acquire_grid(s, r->grid_nmax, r->r_grid, err);
+        ok = q->grid != nullptr;
+        if (ok) {
+            const size_t gb = q->grid->nlat * q->grid->nlon * sizeof(double);
+            if (r->kind == CHARM_AMD_SRV_SHA_GRID) {
+                q->in = pinned_get(s, gb);
+                q->out = pinned_get(s, tri_len(r->nmax) * 16);
+                ok = q->in && q->out;
+                if (ok) std::memcpy(q->in->p, r->f_in, gb);
+            } else {
+                q->out = pinned_get(s, gb);
+                ok = q->out != nullptr;
+            }
+        }
+    }
+    if (!ok) {
+        if (charm_err_isempty(err))
+            CHARM_AMD_ERR(err, CHARM_EMEM, "Failed to stage the request in host memory.");
+        pinned_put(s, q->in);
+        pinned_put(s, q->out);
+        if (q->grid) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            q->grid->users--;
+        }
+        free_req(q);
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
     q->ticket = s->next_ticket++;
-    q->t_submit = now_ms();
     s->all[q->ticket] = q;
     s->pending.push_back(q);
+    s->stats.submitted++;
     s->cv_work.notify_one();
     return q->ticket;
 }
@@ -447,16 +641,65 @@ int charm_amd_server_wait(charm_amd_server *s, long ticket, double *latency_ms, 
     }
     SrvReq *q = it->second;
     s->cv_done.wait(lk, [q] { return q->state >= 2 && q->t_done > 0; });
+    s->all.erase(it);
+    lk.unlock();
     if (latency_ms) *latency_ms = q->t_done - q->t_submit;
     int rc = 0;
-    if (q->state == 3) {
+    if (q->state == 2) {
+        // the result leaves pinned staging in the waiting thread
+        std::memcpy(q->r.out, q->out->p, q->out_bytes);
+        pinned_put(s, q->out);
+        q->out = nullptr;
+    } else if (q->state == 4) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG, "The request was cancelled.");
+        rc = -1;
+    } else {
         CHARM_AMD_ERR(err, q->code ? q->code : CHARM_EFUNCARG, q->msg);
         rc = -1;
     }
-    (void)hipEventDestroy(q->ev);
-    s->all.erase(it);
-    delete q;
+    free_req(q);
     return rc;
+}
+
+int charm_amd_server_cancel(charm_amd_server *s, long ticket, charm_err *err)
+{
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) {
+        CHARM_AMD_ERR(err, CHARM_EFUNCARG, "Unknown ticket.");
+        return -1;
+    }
+    SrvReq *q = it->second;
+    if (q->state >= 2) return 1;  // already completed
+    if (q->state == 0) {
+        auto pq = std::find(s->pending.begin(), s->pending.end(), q);
+        if (pq != s->pending.end()) s->pending.erase(pq);
+        pinned_put(s, q->in);
+        pinned_put(s, q->out);
+        q->in = q->out = nullptr;
+        if (q->grid) q->grid->users--;
+        q->grid = nullptr;
+        q->state = 4;
+        q->t_done = now_ms();
+        s->stats.cancelled++;
+        s->cv_done.notify_all();
+        return 0;
+    }
+    q->cancel = true;  // in flight: the result is discarded when it completes
+    return 0;
+}
+
+void charm_amd_server_inject_error(charm_amd_server *s, int count, int code)
+{
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->fail_next = std::max(0, count);
+    s->fail_code = code;
+}
+
+void charm_amd_server_get_stats(charm_amd_server *s, charm_amd_srv_stats *st)
+{
+    std::lock_guard<std::mutex> lk(s->mu);
+    *st = s->stats;
 }
 
 size_t charm_amd_server_in_use(charm_amd_server *s)
@@ -475,13 +718,16 @@ void charm_amd_server_free(charm_amd_server *s)
     }
     if (s->worker.joinable()) s->worker.join();
     (void)hipDeviceSynchronize();
-    for (auto &kv : s->all) {
-        (void)hipEventDestroy(kv.second->ev);
-        delete kv.second;
+    for (auto &kv : s->all) free_req(kv.second);
+    for (auto &g : s->grids) {
+        charm_amd_grid_free(g.second->g);
+        delete g.second;
     }
-    for (auto &g : s->grids) charm_amd_grid_free(g.second.g);
     for (auto &m : s->models) (void)hipFree(m.cs);
-    for (auto &p : s->pinned) (void)hipHostFree(p.p);
+    for (Pinned *p : s->pinned) {
+        (void)hipHostFree(p->p);
+        delete p;
+    }
     for (auto st : s->streams) (void)hipStreamDestroy(st);
     delete s;
 }

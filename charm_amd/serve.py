@@ -1,8 +1,12 @@
 """Request serving over the C ABI's server (charm_amd_server_*).
 
-A worker thread in the library admits requests in arrival order under an HBM
-budget, overlaps them on the device (one stream per request in flight) and
+A worker thread in the library admits requests under an HBM budget (in
+arrival order; a request that does not fit yet lets at most 8 later ones
+pass), overlaps them on the device (one stream per launch in flight), runs
+small scattered-point requests admitted together as one persistent launch and
 completes them asynchronously; models and Gauss-Legendre grids stay resident.
+Submitting copies a request's inputs, so the caller may reuse its arrays at
+once; the output array is filled by :meth:`Ticket.wait`.
 """
 import ctypes as _ct
 import threading as _th
@@ -19,6 +23,11 @@ _dp = _ct.POINTER(_ct.c_double)
 SHS_SCTR = 0
 SHS_GRID = 1
 SHA_GRID = 2
+
+
+class _SrvStats(_ct.Structure):
+    _fields_ = [(n, _ct.c_ulong) for n in ("submitted", "completed", "failed", "cancelled",
+                                           "launches", "grouped", "max_group")]
 
 
 class _SrvRequest(_ct.Structure):
@@ -39,13 +48,16 @@ def _p(a):
 
 
 class Ticket:
-    def __init__(self, server, tid, out, keep):
-        self.server, self.id, self.out, self._keep = server, tid, out, keep
+    def __init__(self, server, tid, out):
+        self.server, self.id, self.out = server, tid, out
         self.latency_ms = None
+        self._done = False
 
     def wait(self):
-        """Block until done; returns the output array."""
-        if self.latency_ms is None:
+        """Block until done; returns the output array.  A failed or cancelled
+        request raises :class:`charm_amd.CHarmError` with the library's code."""
+        if not self._done:
+            self._done = True
             f = lib().charm_amd_server_wait
             f.restype = _ct.c_int
             f.argtypes = [_vp, _ct.c_long, _ct.POINTER(_ct.c_double), _ct.POINTER(_err._Err)]
@@ -53,12 +65,25 @@ class Ticket:
             err = _err.init()
             try:
                 f(self.server._h, self.id, _ct.byref(lat), err)
+                self.latency_ms = lat.value
                 _err.handler(err, True)
             finally:
                 _err.free(err)
-            self.latency_ms = lat.value
-            self._keep = None
         return self.out
+
+    def cancel(self):
+        """Cancel: a queued request is dropped, an in-flight one has its
+        result discarded.  Returns False if the request had already completed."""
+        f = lib().charm_amd_server_cancel
+        f.restype = _ct.c_int
+        f.argtypes = [_vp, _ct.c_long, _ct.POINTER(_err._Err)]
+        err = _err.init()
+        try:
+            rc = f(self.server._h, self.id, err)
+            _err.handler(err, True)
+        finally:
+            _err.free(err)
+        return rc == 0
 
 
 class Server:
@@ -89,7 +114,7 @@ class Server:
             _err.free(err)
         return mid
 
-    def _submit(self, req, out, keep):
+    def _submit(self, req, out):
         f = lib().charm_amd_server_submit
         f.restype = _ct.c_long
         f.argtypes = [_vp, _ct.POINTER(_SrvRequest), _ct.POINTER(_err._Err)]
@@ -99,20 +124,20 @@ class Server:
             _err.handler(err, True)
         finally:
             _err.free(err)
-        return Ticket(self, tid, out, keep)
+        return Ticket(self, tid, out)
 
     def shs_sctr(self, model, nmax, lat, lon, r):
         lat, lon, r = (_np.ascontiguousarray(a, dtype=_np.float64) for a in (lat, lon, r))
         out = _np.empty(lat.size)
         req = _SrvRequest(SHS_SCTR, int(model), int(nmax), lat.size, _p(lat), _p(lon), _p(r),
                           0, 0.0, None, 0.0, 0.0, _p(out))
-        return self._submit(req, out, (lat, lon, r))
+        return self._submit(req, out)
 
     def shs_grid(self, model, nmax, grid_nmax, r_grid):
         out = _np.empty((grid_nmax + 1, 2 * (grid_nmax + 1)))
         req = _SrvRequest(SHS_GRID, int(model), int(nmax), 0, None, None, None, int(grid_nmax),
                           float(r_grid), None, 0.0, 0.0, _p(out))
-        return self._submit(req, out, ())
+        return self._submit(req, out)
 
     def sha_grid(self, f_grid, nmax, r_grid, mu, R):
         f_grid = _np.ascontiguousarray(f_grid, dtype=_np.float64)
@@ -121,7 +146,25 @@ class Server:
         out = _np.empty((P, 2))
         req = _SrvRequest(SHA_GRID, -1, int(nmax), 0, None, None, None, int(grid_nmax),
                           float(r_grid), _p(f_grid), float(mu), float(R), _p(out))
-        return self._submit(req, out, (f_grid,))
+        return self._submit(req, out)
+
+    def inject_error(self, count, code):
+        """Make the next ``count`` launches fail with CHARM error ``code``, as a
+        device error would (fault injection from the host)."""
+        f = lib().charm_amd_server_inject_error
+        f.restype = None
+        f.argtypes = [_vp, _ct.c_int, _ct.c_int]
+        f(self._h, int(count), int(code))
+
+    def stats(self):
+        """Counters: submitted, completed, failed, cancelled, launches,
+        grouped (requests that ran in a group of several), max_group."""
+        f = lib().charm_amd_server_get_stats
+        f.restype = None
+        f.argtypes = [_vp, _ct.POINTER(_SrvStats)]
+        st = _SrvStats()
+        f(self._h, _ct.byref(st))
+        return {n: getattr(st, n) for n, _ in _SrvStats._fields_}
 
     def in_use(self):
         f = lib().charm_amd_server_in_use

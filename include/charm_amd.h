@@ -380,12 +380,16 @@ void charm_amd_batch_run(charm_amd_batch *b, void *stream, charm_err *err);
 int charm_amd_batch_uses_graph(const charm_amd_batch *b);
 void charm_amd_batch_free(charm_amd_batch *b);
 
-/* Serving: a worker thread admits requests in arrival order under an HBM
- * byte budget and a limit of requests in flight, runs each on its own stream
- * (so requests overlap on the device), and completes them asynchronously.
+/* Serving (no counterpart in CHarm): a worker thread admits requests under an
+ * HBM byte budget and a limit of launches in flight -- in arrival order, a
+ * request that does not fit yet letting at most 8 later ones pass -- and runs
+ * each on its own stream, so that requests overlap on the device; small
+ * scattered-point requests admitted together run as one persistent launch.
  * Models and Gauss-Legendre grids (with their seed tables) stay resident.
- * Host arrays passed in a request must stay valid until charm_amd_server_wait
- * returns for it. */
+ * charm_amd_server_submit copies a request's input arrays (lat, lon, r,
+ * f_in) before it returns and builds its grid if needed; the output array
+ * must stay valid until charm_amd_server_wait, which copies the result into
+ * it, returns. */
 enum { CHARM_AMD_SRV_SHS_SCTR = 0, CHARM_AMD_SRV_SHS_GRID = 1, CHARM_AMD_SRV_SHA_GRID = 2 };
 
 typedef struct {
@@ -411,6 +415,23 @@ int charm_amd_server_add_model(charm_amd_server *s, const charm_shc *shcs, unsig
 long charm_amd_server_submit(charm_amd_server *s, const charm_amd_srv_request *r, charm_err *err);
 /* Waits for a ticket; *latency_ms = completion time minus submission time. */
 int charm_amd_server_wait(charm_amd_server *s, long ticket, double *latency_ms, charm_err *err);
+/* Cancels a ticket: a queued request is dropped, an in-flight one has its
+ * result discarded; charm_amd_server_wait then reports CHARM_EFUNCARG
+ * ("cancelled").  Returns 0, 1 if the request had already completed (its
+ * result stays available), -1 for an unknown ticket. */
+int charm_amd_server_cancel(charm_amd_server *s, long ticket, charm_err *err);
+/* Fault injection from the host: the next `count` launches fail as a device
+ * error would, with CHARM error code `code` (CHARM_EMEM for an allocation
+ * failure, CHARM_EFUNCARG for a failed kernel); their requests report it in
+ * charm_amd_server_wait and the server carries on. */
+void charm_amd_server_inject_error(charm_amd_server *s, int count, int code);
+typedef struct {
+    unsigned long submitted, completed, failed, cancelled;
+    unsigned long launches;   /* launch units started (a request or a group) */
+    unsigned long grouped;    /* requests that ran in a group of several */
+    unsigned long max_group;  /* largest group */
+} charm_amd_srv_stats;
+void charm_amd_server_get_stats(charm_amd_server *s, charm_amd_srv_stats *st);
 /* HBM bytes currently accounted to models, grids and requests in flight. */
 size_t charm_amd_server_in_use(charm_amd_server *s);
 void charm_amd_server_free(charm_amd_server *s);
