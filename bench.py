@@ -78,20 +78,83 @@ def base_line(metric, value, steps, warmup, ms, config):
             "data": "synthetic (seeded Kaula-like coefficients)", "config": config}
 
 
-def cpu_baseline(N, C, S, lat, lon, R, GM, rows, threads):
-    """The oracle (scalar C port of CHarm's algorithm) on a bounded sample of
-    the same workload: SHS of `rows` GL rows at degree N on the host."""
-    os.environ["OMP_NUM_THREADS"] = str(threads)
+def cpu_threads():
+    """Host threads for the CPU baseline: OMP_NUM_THREADS when set (the host's
+    share of cores), else the physical cores this process may use."""
+    from oracle.cpu_gl import physical_cores
+
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else physical_cores()
+
+
+def cpu_baseline(N, C, S, GM, R, reps):
+    """The repo's CPU restatement of CHarm's GL synthesis and analysis
+    (oracle/cpu_gl.c: symmetry fold, 32-ring blocks in AVX-512 or AVX2
+    loops, OpenMP over the host's cores, scipy.fft along the parallels) on
+    the full round trip at degree N: median of `reps` runs after a warm-up."""
     import numpy as np
+    from oracle.cpu_gl import CpuGL, cpu_model, physical_cores
     from oracle.oracle import Oracle
 
-    orc = Oracle("d")
-    ring_idx = np.linspace(0, len(lat) - 1, rows).round().astype(np.int64)
-    r = np.full(len(lat), R)
-    t0 = time.perf_counter()
-    orc.shs_grid(N, C, S, GM, R, lat, r, lon, gl=True, rings=ring_idx)
-    dt = time.perf_counter() - t0
-    return rows * (N + 1) ** 2 / dt, dt
+    threads = cpu_threads()
+    cg = CpuGL(threads)
+    lat, w, lon = Oracle("d").gl_grid(N)
+    t_shs, t_sha = [], []
+    for rep in range(reps + 1):
+        t0 = time.perf_counter()
+        f = cg.shs(N, C, S, GM, R, R, lat, len(lon))
+        t1 = time.perf_counter()
+        Co, So = cg.sha(N, f, lat, w, R, GM, R)
+        t2 = time.perf_counter()
+        if rep:
+            t_shs.append(t1 - t0)
+            t_sha.append(t2 - t1)
+    err = float(max(np.abs(Co - C).max(), np.abs(So - S).max()) / np.abs(C).max())
+    ts, ta = float(np.median(t_shs)), float(np.median(t_sha))
+    units = float(N + 1) ** 2 * len(lat)
+    return {"value": 2.0 * units / (ts + ta), "unit": UNIT, "cores": threads, "kind": "port",
+            "label": "repo CPU restatement, not CHarm",
+            "sample": "full SHS+SHA round trip at nmax=%d on the GL grid (%dx%d), median of %d "
+                      "after a warm-up: oracle/cpu_gl.c (ring pairs folded, 32-ring blocks in "
+                      "%s loops, OpenMP, subnormals flushed as CHarm's -ffast-math build) + "
+                      "scipy.fft rows, %d threads" % (N, len(lat), len(lon), reps,
+                                                       cg.isa.upper(), threads),
+            "cpu_model": cpu_model(), "physical_cores_visible": physical_cores(),
+            "shs_s": ts, "sha_s": ta, "shs_units_per_s": units / ts,
+            "sha_units_per_s": units / ta, "roundtrip_rel_err": err}
+
+
+# CHarm's published round-trip errors (docs/src/benchmarks.rst:60-65, read
+# off its plot, about +-15%; 2160 and 10800 interpolated log-log in N)
+PUBLISHED_EPS = {1000: (8.5e-13, 5.3e-14), 2160: (4.0e-12, 1.38e-13), 10800: (4.9e-11, 9.1e-13)}
+
+
+def accuracy_protocol(N):
+    """CHarm's accuracy protocol (bench/bench.c:192-300): C, S ~ U[-1, 1),
+    mu = R = r = 1, GL synthesis and analysis at degree N; eps_max and
+    eps_rms of the recovered coefficients."""
+    import numpy as np
+
+    from charm_amd.device import DevArray, DeviceGrid, Stream
+    from charm_amd.workloads import uniform_coeffs
+
+    C, S = uniform_coeffs(N, N)
+    grid = DeviceGrid(N, 1.0)
+    st = Stream()
+    cs = DevArray.from_numpy(np.stack([C, S], axis=1))
+    f = DevArray((grid.nlat, grid.nlon))
+    out = DevArray(cs.shape)
+    grid.shs(cs, N, 1.0, 1.0, f, st)
+    grid.sha(f, N, 1.0, 1.0, out, st)
+    st.synchronize()
+    o = out.numpy()
+    d = np.concatenate([o[:, 0] - C, o[:, 1] - S])
+    res = {"nmax": N, "eps_max": float(np.abs(d).max()),
+           "eps_rms": float(np.sqrt(2.0 * np.sum(d * d) / ((N + 1) * (N + 2)))),
+           "coefficients": "U[-1,1) seed %d, mu=R=r=1" % N}
+    if N in PUBLISHED_EPS:
+        res["charm_published_eps_max"], res["charm_published_eps_rms"] = PUBLISHED_EPS[N]
+    return res
 
 
 def gl_roundtrip(args, N, with_sctr, with_cpu):
@@ -174,7 +237,8 @@ def gl_roundtrip(args, N, with_sctr, with_cpu):
         "sha_units_per_s": units / (sha_ms * 1e-3),
         "kernel_avg_ms": avg,
         "roundtrip_rel_err": rt_err,
-        "roofline": {"bound": "mfma", "compute": "vector FP64 (VALU)", "kernel": dom,
+        "accuracy_protocol": accuracy_protocol(N),
+        "roofline": {"bound": "valu", "compute": "vector FP64 (VALU)", "kernel": dom,
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,
                      "ring_pairs_per_lane": last_ring_k(lib, dom),
@@ -188,14 +252,7 @@ def gl_roundtrip(args, N, with_sctr, with_cpu):
     if with_sctr:
         line.update(sctr_measure(N, cs, args.sctr_points, stream))
     if with_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        pnt = grid.pnt
-        v, dt = cpu_baseline(N, C, S, pnt.lat, pnt.lon, R_EARTH, GM, args.cpu_rows, threads)
-        line["cpu_baseline"] = {
-            "value": v, "unit": UNIT, "cores": threads, "kind": "port",
-            "sample": "SHS of %d of the %d GL rows at nmax=%d (Legendre sums without the "
-                      "symmetry fold + direct DFT per row), oracle in double, %.1f s wall"
-                      % (args.cpu_rows, nlat, N, dt)}
+        line["cpu_baseline"] = cpu_baseline(N, C, S, GM, R_EARTH, args.cpu_reps)
     return line
 
 
@@ -292,7 +349,7 @@ def grad_bench(args):
                           "grad_combine_kernel": cmb_avg},
         "laplace_trace_rel": trace_err,
         "combine_GBps": cmb_bytes / (cmb_avg * 1e-3) / 1e9,
-        "roofline": {"bound": "mfma", "compute": "vector FP64 (VALU)", "kernel": "shs_grid_kernel",
+        "roofline": {"bound": "valu", "compute": "vector FP64 (VALU)", "kernel": "shs_grid_kernel",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": TRAFFIC_GL2160.get(("shs_grid_kernel", last_ring_k(lib, "shs_grid_kernel"))),
@@ -310,7 +367,7 @@ def grad_bench(args):
     line.update({"sctr_grad1_npoint": npt, "sctr_grad1_s": gs,
                  "sctr_grad1_units_per_s": (N + 1) ** 2 * npt / gs})
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         os.environ["OMP_NUM_THREADS"] = str(threads)
         from oracle.oracle import Oracle
 
@@ -321,6 +378,7 @@ def grad_bench(args):
         dt = time.perf_counter() - t0
         line["cpu_baseline"] = {
             "value": (N + 1) ** 2 * ns / dt, "unit": UNIT, "cores": threads, "kind": "port",
+            "label": "repo CPU restatement (the scalar oracle), not CHarm",
             "sample": "grad1 at %d of the scattered points at nmax=%d (CHarm's per-term "
                       "derivative recurrences), oracle in double, %.1f s wall; compare "
                       "sctr_grad1_units_per_s" % (ns, N, dt)}
@@ -406,7 +464,7 @@ def gfm_bench(args):
                       "nmax": N, "parallelism": "single GPU"})
     line["c00"] = float(pot.c[0])
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         os.environ["OMP_NUM_THREADS"] = str(threads)
         from oracle.oracle import Oracle
 
@@ -417,23 +475,37 @@ def gfm_bench(args):
         dt2 = time.perf_counter() - t0
         line["cpu_baseline"] = {
             "value": units(n2, pmax) / dt2, "unit": UNIT, "cores": threads, "kind": "port",
+            "label": "repo CPU restatement (the scalar oracle), not CHarm",
             "sample": "the same modelling for a degree-%d shape (grids up to degree %d), "
                       "oracle SHS/SHA in double plus numpy powers, %.2f s wall" % (n2, pmax * n2, dt2)}
     return line
 
 
-def batch_bench(args):
+def _timed(fn, reps):
+    fn()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    return (time.perf_counter() - t0) / reps
+
+
+def grid_batch(mix, reps, label):
+    """One mix of GL requests [(is_synthesis, N, seed)], each with its own
+    Kaula coefficients and grid: device-resident per-call, one batch (eager
+    and replayed from a HIP graph), and the plain host-array API per call
+    (charm_shs_point / charm_sha_point) with grids, models and analysis inputs
+    built before the timer, as CHarm's own benchmark times one call
+    (bench/bench.c:225-279)."""
     import numpy as np
 
-    import charm_amd  # noqa: F401
     from charm_amd import crd, sha, shc, shs
     from charm_amd.device import SHA_GRID, SHS_GRID, Batch, DevArray, DeviceGrid, Stream
-    from charm_amd.workloads import GM, R_EARTH, batch_mix, kaula_coeffs
+    from charm_amd.workloads import GM, R_EARTH, kaula_coeffs
 
     st = Stream()
     reqs, host = [], []
     units = 0.0
-    for is_shs, N, seed in batch_mix():
+    for is_shs, N, seed in mix:
         kind = SHS_GRID if is_shs else SHA_GRID
         C, S = kaula_coeffs(N, seed)
         g = DeviceGrid(N, R_EARTH)
@@ -443,10 +515,14 @@ def batch_bench(args):
             g.shs(cs, N, GM, R_EARTH, f, st)
             cs = DevArray(cs.shape)
         reqs.append((kind, N, GM, R_EARTH, g, cs, f))
-        host.append((kind, N, C, S))
         units += float(N + 1) ** 2 * g.nlat
     st.synchronize()
-    print("# batch: %d requests set up" % len(reqs), file=sys.stderr, flush=True)
+    for (kind, N, _, _, g, _, f), (is_shs, _, seed) in zip(reqs, mix):
+        pnt = crd.PointGridGL(N, R_EARTH)
+        C, S = kaula_coeffs(N, seed)
+        host.append((kind, N, pnt, shc.Shc.from_arrays(N, C, S, GM, R_EARTH),
+                     None if kind == SHS_GRID else f.numpy()))
+    print("# %s: %d requests set up" % (label, len(reqs)), file=sys.stderr, flush=True)
 
     def per_call():
         for kind, N, mu, R, g, cs, f in reqs:
@@ -456,72 +532,62 @@ def batch_bench(args):
                 g.sha(f, N, mu, R, cs, st)
         st.synchronize()
 
-    def timed(fn, reps):
-        fn()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            fn()
-        return (time.perf_counter() - t0) / reps
-
     def outputs():
         return [(r[6].numpy() if r[0] == SHS_GRID else r[5].numpy()) for r in reqs]
 
-    t_call = timed(per_call, args.steps)
+    def host_api():
+        for kind, N, pnt, m, fh in host:
+            if kind == SHS_GRID:
+                shs.point(pnt, m, N)
+            else:
+                sha.point(pnt, fh, N, GM, R_EARTH)
+
+    t_call = _timed(per_call, reps)
     ref = outputs()
     b_eager = Batch(reqs, use_graph=False)
-    t_batch = timed(lambda: (b_eager.run(st), st.synchronize()), args.steps)
+    t_batch = _timed(lambda: (b_eager.run(st), st.synchronize()), reps)
     b_graph = Batch(reqs, use_graph=True)
-    t_graph = timed(lambda: (b_graph.run(st), st.synchronize()), args.steps)
-    err = 0.0
-    for x, y in zip(ref, outputs()):
-        err = max(err, float(np.abs(y - x).max() / np.abs(x).max()))
-    print("# batch: timed per-call and batched", file=sys.stderr, flush=True)
-    # the plain drop-in API on host arrays (grid set-up and PCIe included)
-    t0 = time.perf_counter()
-    for kind, N, C, S in host:
-        g = crd.PointGridGL(N, R_EARTH)
-        m = shc.Shc.from_arrays(N, C, S, GM, R_EARTH)
-        fh = shs.point(g, m, N)
-        if kind == SHA_GRID:
-            sha.point(g, fh, N, GM, R_EARTH)
-    t_host = time.perf_counter() - t0
+    t_graph = _timed(lambda: (b_graph.run(st), st.synchronize()), reps)
+    # synthesis results are bitwise those of the calls alone; analysis sums its
+    # ring tiles in another grouping (the batch's ring pairs per lane)
+    got = outputs()
+    same = all(np.array_equal(x, y) for x, y, r in zip(ref, got, reqs) if r[0] == SHS_GRID)
+    sha_diff = max([float(np.abs(y - x).max() / np.abs(x).max())
+                    for x, y, r in zip(ref, got, reqs) if r[0] == SHA_GRID] + [0.0])
+    t_host = _timed(host_api, 1)
+    print("# %s: timed per-call, batched and host API" % label, file=sys.stderr, flush=True)
     best = min(t_batch, t_graph)
-    line = base_line("(nmax+1)^2*nlat/s FP64 batched SHS/SHA, 64 requests nmax~U{90..2700}",
-                     units / best, args.steps, 1, 1e3 * best,
-                     {"workload": "64 GL requests (32 SHS, 32 SHA), nmax~U{90..2700} seed 64, "
-                                  "own coefficients and grid each", "requests": 64,
-                      "parallelism": "single GPU"})
-    line.update({"per_call_units_per_s": units / t_call, "batch_units_per_s": units / t_batch,
-                 "batch_graph_units_per_s": units / t_graph, "graph_used": b_graph.uses_graph,
-                 "speedup_vs_per_call_dev": t_call / best,
-                 "host_api_per_call_s": t_host,
-                 "speedup_vs_host_api_per_call": t_host / best,
-                 "batch_vs_alone_max_rel_diff": err})
-    line.update(batch_sctr_bench(args))
-    return line
+    return units, best, {
+        "requests": len(reqs), "units": units,
+        "per_call_dev_units_per_s": units / t_call, "batch_units_per_s": units / t_batch,
+        "batch_graph_units_per_s": units / t_graph, "graph_used": b_graph.uses_graph,
+        "speedup_vs_per_call_dev": t_call / best,
+        "host_api_per_call_s": t_host, "speedup_vs_host_api_per_call": t_host / best,
+        "shs_bitwise_equal_to_alone": bool(same), "sha_max_rel_diff_vs_alone": sha_diff}
 
 
-def batch_sctr_bench(args):
-    """32 scattered synthesis requests, nmax ~ U{90..2700}, 1e3..1e5 points
-    each (log-uniform), r = R + U(0, 9000 m): one call per request vs one
-    batch.  The batch sorts each point set once, at creation."""
+def sctr_batch(degs, npts, seed0, reps, label):
+    """Scattered synthesis requests (degree degs[i], npts[i] points with
+    r = R + U(0, 9000 m), own coefficients): device-resident per-call, one
+    batch replayed from a HIP graph (the batch sorts each point set once, at
+    creation), and the plain host-array API per call with the point
+    structures and models built before the timer."""
     import numpy as np
 
+    from charm_amd import crd, shc, shs
     from charm_amd.device import SHS_SCTR, Batch, DevArray, Stream, shs_sctr
     from charm_amd.workloads import GM, R_EARTH, kaula_coeffs, scattered_points
 
-    rng = np.random.default_rng(65)
-    degs = rng.integers(90, 2701, 32)
-    npts = np.round(10.0 ** rng.uniform(3, 5, 32)).astype(int)
     st = Stream()
-    reqs = []
+    reqs, host = [], []
     units = 0.0
     for i, (N, P) in enumerate(zip(degs, npts)):
         N, P = int(N), int(P)
-        C, S = kaula_coeffs(N, 2000 + i)
-        pts = tuple(DevArray.from_numpy(a) for a in scattered_points(P, 3000 + i))
-        reqs.append((SHS_SCTR, N, GM, R_EARTH, pts, DevArray.from_numpy(np.stack([C, S], axis=1)),
-                     DevArray(P)))
+        C, S = kaula_coeffs(N, seed0 + i)
+        pts = scattered_points(P, seed0 + 1000 + i)
+        reqs.append((SHS_SCTR, N, GM, R_EARTH, tuple(DevArray.from_numpy(a) for a in pts),
+                     DevArray.from_numpy(np.stack([C, S], axis=1)), DevArray(P)))
+        host.append((N, crd.PointSctr(*pts), shc.Shc.from_arrays(N, C, S, GM, R_EARTH)))
         units += float(N + 1) ** 2 * P
 
     def per_call():
@@ -529,24 +595,58 @@ def batch_sctr_bench(args):
             shs_sctr(*pts, cs, N, mu, R, f, st)
         st.synchronize()
 
-    def timed(fn, reps):
-        fn()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            fn()
-        return (time.perf_counter() - t0) / reps
+    def host_api():
+        for N, pnt, m in host:
+            shs.point(pnt, m, N)
 
-    t_call = timed(per_call, args.steps)
+    t_call = _timed(per_call, reps)
     ref = [r[6].numpy() for r in reqs]
     b = Batch(reqs, use_graph=True)
-    t_batch = timed(lambda: (b.run(st), st.synchronize()), args.steps)
+    t_batch = _timed(lambda: (b.run(st), st.synchronize()), reps)
     same = all(np.array_equal(x, r[6].numpy()) for x, r in zip(ref, reqs))
-    print("# batch: timed scattered per-call and batched", file=sys.stderr, flush=True)
-    return {"sctr_batch": {"requests": 32, "points": int(npts.sum()),
-                           "per_call_units_per_s": units / t_call,
-                           "batch_graph_units_per_s": units / t_batch,
-                           "speedup_vs_per_call_dev": t_call / t_batch,
-                           "graph_used": b.uses_graph, "bitwise_equal_to_alone": bool(same)}}
+    t_host = _timed(host_api, 1)
+    print("# %s: timed scattered per-call, batched and host API" % label, file=sys.stderr,
+          flush=True)
+    return {"requests": len(reqs), "points": int(np.sum(npts)),
+            "per_call_dev_units_per_s": units / t_call,
+            "batch_graph_units_per_s": units / t_batch,
+            "speedup_vs_per_call_dev": t_call / t_batch,
+            "host_api_per_call_s": t_host, "speedup_vs_host_api_per_call": t_host / t_batch,
+            "graph_used": b.uses_graph, "bitwise_equal_to_alone": bool(same)}
+
+
+def batch_bench(args):
+    """C4a: the 64-request mix, nmax ~ U{90..2700}; beside it a mix of 256
+    small grids (nmax ~ U{16..256}), 32 scattered sets (1e3..1e5 points,
+    nmax ~ U{90..2700}) and 256 small scattered sets (100..2000 points, nmax
+    ~ U{16..360})."""
+    import numpy as np
+
+    from charm_amd.workloads import batch_mix
+
+    units, best, res = grid_batch(batch_mix(), args.steps, "batch")
+    line = base_line("(nmax+1)^2*nlat/s FP64 batched SHS/SHA, 64 requests nmax~U{90..2700}",
+                     units / best, args.steps, 1, 1e3 * best,
+                     {"workload": "64 GL requests (32 SHS, 32 SHA), nmax~U{90..2700} seed 64, "
+                                  "own coefficients and grid each", "requests": 64,
+                      "parallelism": "single GPU"})
+    line.update(res)
+    rng = np.random.default_rng(256)
+    small = [(i % 2 == 0, int(N), 5000 + i) for i, N in enumerate(rng.integers(16, 257, 256))]
+    line["small_grids"] = grid_batch(small, args.steps, "small grids")[2]
+    line["small_grids"]["mix"] = "256 GL requests (128 SHS, 128 SHA), nmax~U{16..256} seed 256"
+    rng = np.random.default_rng(65)
+    degs = rng.integers(90, 2701, 32)
+    npts = np.round(10.0 ** rng.uniform(3, 5, 32)).astype(int)
+    line["sctr_batch"] = sctr_batch(degs, npts, 2000, args.steps, "sctr batch")
+    line["sctr_batch"]["mix"] = "32 scattered sets, nmax~U{90..2700}, 1e3..1e5 points seed 65"
+    rng = np.random.default_rng(257)
+    degs = rng.integers(16, 361, 256)
+    npts = rng.integers(100, 2001, 256)
+    line["small_sctr_batch"] = sctr_batch(degs, npts, 7000, args.steps, "small sctr batch")
+    line["small_sctr_batch"]["mix"] = ("256 scattered sets, nmax~U{16..360}, 100..2000 points "
+                                       "seed 257")
+    return line
 
 
 def multi_bench(args):
@@ -666,11 +766,52 @@ def serve_bench(args):
             tickets.append(srv.shs_grid(mid, n, a, R_EARTH))
         else:
             tickets.append(srv.sha_grid(sha_in[i], n, R_EARTH, GM, R_EARTH))
-    for t in tickets:
-        t.wait()
+    outs = [t.wait().copy() for t in tickets]
     wall = time.perf_counter() - t0
     lat = np.array([t.latency_ms for t in tickets])
+    st_poisson = srv.stats()
+    print("# serve: Poisson run done", file=sys.stderr, flush=True)
+
+    def submit(i):
+        kind, n, a = reqs[i]
+        if kind == "sctr":
+            return srv.shs_sctr(mid, n, *a)
+        if kind == "shs":
+            return srv.shs_grid(mid, n, a, R_EARTH)
+        return srv.sha_grid(sha_in[i], n, R_EARTH, GM, R_EARTH)
+
+    # every request again, one at a time: its result must equal the served one
+    t0 = time.perf_counter()
+    alone = [submit(i).wait().copy() for i in range(nreq)]
+    t_seq = time.perf_counter() - t0
+    same = all(np.array_equal(x, y) for x, y in zip(outs, alone))
+    # closed loop: everything submitted at once
+    t0 = time.perf_counter()
+    ts = [submit(i) for i in range(nreq)]
+    sat = [t.wait().copy() for t in ts]
+    t_sat = time.perf_counter() - t0
+    same = same and all(np.array_equal(x, y) for x, y in zip(sat, alone))
+    st_all = srv.stats()
     srv.close()
+    # the plain host-array API per call, grids and point structures built
+    # before the timer
+    from charm_amd import crd, sha, shs
+
+    model = shc.Shc.from_arrays(NM, C, S, GM, R_EARTH)
+    calls = []
+    for i, (kind, n, a) in enumerate(reqs):
+        if kind == "sctr":
+            calls.append((shs.point, (crd.PointSctr(*a), model, n)))
+        elif kind == "shs":
+            calls.append((shs.point, (crd.PointGridGL(a, R_EARTH), model, n)))
+        else:
+            calls.append((sha.point, (crd.PointGridGL(a, R_EARTH), sha_in[i], n, GM, R_EARTH)))
+    calls[0][0](*calls[0][1])
+    t0 = time.perf_counter()
+    for fn, a in calls:
+        fn(*a)
+    t_host = time.perf_counter() - t0
+    print("# serve: sequential, closed-loop and host API runs done", file=sys.stderr, flush=True)
     line = base_line("(nmax+1)^2*nlat/s FP64 served SHS/SHA mix, p50/p99 latency", units / wall,
                      nreq, 2, 1e3 * wall / nreq,
                      {"workload": "Poisson arrivals (rate %.1f/s) on a resident nmax=%d model: 60%% "
@@ -681,8 +822,16 @@ def serve_bench(args):
                       "parallelism": "single GPU"})
     line.update({"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                  "mean_ms": float(lat.mean()), "wall_s": wall,
+                 "served_equal_to_alone": bool(same),
+                 "closed_loop": {"wall_s": t_sat, "units_per_s": units / t_sat,
+                                 "speedup_vs_one_at_a_time_served": t_seq / t_sat,
+                                 "speedup_vs_host_api_per_call": t_host / t_sat},
+                 "one_at_a_time_served_s": t_seq, "host_api_per_call_s": t_host,
+                 "server_stats_poisson": st_poisson, "server_stats_total": st_all,
                  "note": "units of scattered requests count (nmax+1)^2 per point; results "
-                         "copied back to host memory inside the latency"})
+                         "copied back to host memory inside the latency; the Poisson value is "
+                         "bounded by the arrival rate, the closed loop is the server's "
+                         "throughput"})
     return line
 
 
@@ -694,7 +843,7 @@ def main():
     ap.add_argument("--workload", default="gl2160",
                     choices=("gl2160", "gl10800", "batch", "multi", "grad", "io", "gfm",
                              "serve"))
-    ap.add_argument("--cpu-rows", type=int, default=1024)
+    ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sctr", action="store_true")
     ap.add_argument("--sctr-points", type=int, default=1000000)

@@ -291,6 +291,24 @@ bool epoch_fraction(const char *date, double &t, std::string &why)
     return true;
 }
 
+// Runs the body of a C ABI function; an exception (a failed allocation for a
+// corrupt or huge file, say) becomes a CHARM error instead of leaving the
+// extern "C" function
+template <class R, class F>
+R guarded(charm_err *err, R fail, F &&body)
+{
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        set_err(err, CHARM_EMEM, "Memory allocation failure.");
+    } catch (const std::exception &e) {
+        set_err(err, CHARM_EFILEIO, e.what());
+    } catch (...) {
+        set_err(err, CHARM_EFILEIO, "Unexpected failure.");
+    }
+    return fail;
+}
+
 }  // namespace
 }  // namespace charm_amd
 
@@ -302,7 +320,7 @@ extern "C" {
 // bin: nmax (unsigned long), mu, r, then C and S, each order m = 0..nmax with
 // degrees m..nmax (src/shc/shc_write_bin.c:18-187, shc_read_bin.c:26-256)
 // ---------------------------------------------------------------------------
-unsigned long charm_shc_read_bin(const char *pathname, unsigned long nmax, charm_shc *shcs,
+static unsigned long shc_read_bin_impl(const char *pathname, unsigned long nmax, charm_shc *shcs,
                                  charm_err *err)
 {
     if (!nmax_only(nmax, shcs) && !shcs) {
@@ -325,44 +343,59 @@ unsigned long charm_shc_read_bin(const char *pathname, unsigned long nmax, charm
         std::fclose(f);
         return nmax_file;
     }
-    if (std::fread(&mu, sizeof mu, 1, f) != 1 || std::fread(&r, sizeof r, 1, f) != 1) {
-        set_err(err, CHARM_EFILEIO, "Failed to read the scaling parameter or the radius.");
+    if (std::fread(&mu, sizeof mu, 1, f) != 1) {
+        set_err(err, CHARM_EFILEIO, "Failed to read the scaling parameter.");
+        std::fclose(f);
+        return nmax_file;
+    }
+    if (std::fread(&r, sizeof r, 1, f) != 1) {
+        set_err(err, CHARM_EFILEIO, "Failed to read the radius of the reference sphere.");
         std::fclose(f);
         return nmax_file;
     }
     if (nmax > nmax_file) {
         set_err(err, CHARM_EFUNCARG,
-                "The maximum harmonic degree of the file is smaller than \"nmax\".");
+                "Not enough coefficients in the input file for the maximum harmonic degree "
+                "\"nmax\".");
         std::fclose(f);
         return nmax_file;
     }
     if (shcs->nmax < nmax) {
-        set_err(err, CHARM_EFUNCARG, "\"shcs->nmax\" cannot be smaller than \"nmax\".");
+        set_err(err, CHARM_EFUNCARG,
+                "Too low maximum degree \"shcs->nmax\" to read coefficients up to degree "
+                "\"nmax\".");
         std::fclose(f);
         return nmax_file;
     }
     shcs->mu = mu;
     shcs->r = r;
     reset_coeffs(shcs);
-    // both tables of the file in one read, then column copies
-    const size_t per = tri_len(nmax_file);
-    std::vector<double> buf(2 * per);
-    if (std::fread(buf.data(), sizeof(double), buf.size(), f) != buf.size()) {
-        set_err(err, CHARM_EFILEIO, "Failed to read the coefficients.");
-        std::fclose(f);
-        return nmax_file;
+    // column by column: order m holds degrees m..nmax_file; the degrees above
+    // nmax and the orders above nmax are skipped
+    for (int cs = 0; cs < 2; cs++) {
+        double **dst = cs ? shcs->s : shcs->c;
+        bool ok = true;
+        for (unsigned long m = 0; m <= nmax_file && ok; m++) {
+            if (m <= nmax) {
+                const size_t nread = nmax + 1 - m;
+                ok = std::fread(dst[m], sizeof(double), nread, f) == nread;
+                if (ok && nmax < nmax_file)
+                    ok = std::fseek(f, (long)((nmax_file - nmax) * sizeof(double)), SEEK_CUR) == 0;
+            } else {
+                ok = std::fseek(f, (long)((nmax_file + 1 - m) * sizeof(double)), SEEK_CUR) == 0;
+            }
+        }
+        if (!ok) {
+            set_err(err, CHARM_EFILEIO, cs ? "Failed to read the \"S\" coefficients."
+                                           : "Failed to read the \"C\" coefficients.");
+            break;
+        }
     }
     std::fclose(f);
-    for (int cs = 0; cs < 2; cs++) {
-        const double *src = buf.data() + cs * per;
-        double **dst = cs ? shcs->s : shcs->c;
-        for (unsigned long m = 0; m <= nmax; m++)
-            std::memcpy(dst[m], src + col_off(nmax_file, m), (nmax + 1 - m) * sizeof(double));
-    }
     return nmax_file;
 }
 
-void charm_shc_write_bin(const charm_shc *shcs, unsigned long nmax, const char *pathname,
+static void shc_write_bin_impl(const charm_shc *shcs, unsigned long nmax, const char *pathname,
                          charm_err *err)
 {
     if (nmax > shcs->nmax) {
@@ -495,13 +528,13 @@ static unsigned long read_lines_format(const char *pathname, unsigned long nmax,
     return nmax_file;
 }
 
-unsigned long charm_shc_read_tbl(const char *pathname, unsigned long nmax, charm_shc *shcs,
+static unsigned long shc_read_tbl_impl(const char *pathname, unsigned long nmax, charm_shc *shcs,
                                  charm_err *err)
 {
     return read_lines_format(pathname, nmax, shcs, err, false);
 }
 
-unsigned long charm_shc_read_dov(const char *pathname, unsigned long nmax, charm_shc *shcs,
+static unsigned long shc_read_dov_impl(const char *pathname, unsigned long nmax, charm_shc *shcs,
                                  charm_err *err)
 {
     return read_lines_format(pathname, nmax, shcs, err, true);
@@ -511,7 +544,7 @@ unsigned long charm_shc_read_dov(const char *pathname, unsigned long nmax, charm
 // mtx: row i holds C_{i,0..i}, then S_{j,i+1} for j = i+1..nmax_file
 // (src/shc/shc_read_mtx.c, shc_write_mtx.c)
 // ---------------------------------------------------------------------------
-unsigned long charm_shc_read_mtx(const char *pathname, unsigned long nmax, charm_shc *shcs,
+static unsigned long shc_read_mtx_impl(const char *pathname, unsigned long nmax, charm_shc *shcs,
                                  charm_err *err)
 {
     if (!nmax_only(nmax, shcs) && !shcs) {
@@ -582,7 +615,7 @@ unsigned long charm_shc_read_mtx(const char *pathname, unsigned long nmax, charm
 // ---------------------------------------------------------------------------
 // gfc (ICGEM format, src/shc/shc_read_gfc.c:455-1409)
 // ---------------------------------------------------------------------------
-unsigned long charm_shc_read_gfc(const char *pathname, unsigned long nmax, const char *epoch,
+static unsigned long shc_read_gfc_impl(const char *pathname, unsigned long nmax, const char *epoch,
                                  charm_shc *shcs, charm_err *err)
 {
     if (!nmax_only(nmax, shcs) && !shcs) {
@@ -982,19 +1015,19 @@ static void write_pairs(const charm_shc *shcs, unsigned long nmax, const char *f
     close_text_out(f, ok, err);
 }
 
-void charm_shc_write_tbl(const charm_shc *shcs, unsigned long nmax, const char *formatting,
+static void shc_write_tbl_impl(const charm_shc *shcs, unsigned long nmax, const char *formatting,
                          int ordering, const char *pathname, charm_err *err)
 {
     write_pairs(shcs, nmax, formatting, ordering, pathname, err, false);
 }
 
-void charm_shc_write_dov(const charm_shc *shcs, unsigned long nmax, const char *formatting,
+static void shc_write_dov_impl(const charm_shc *shcs, unsigned long nmax, const char *formatting,
                          int ordering, const char *pathname, charm_err *err)
 {
     write_pairs(shcs, nmax, formatting, ordering, pathname, err, true);
 }
 
-void charm_shc_write_mtx(const charm_shc *shcs, unsigned long nmax, const char *formatting,
+static void shc_write_mtx_impl(const charm_shc *shcs, unsigned long nmax, const char *formatting,
                          const char *pathname, charm_err *err)
 {
     FILE *f = open_text_out(shcs, nmax, pathname, formatting, err);
@@ -1008,6 +1041,61 @@ void charm_shc_write_mtx(const charm_shc *shcs, unsigned long nmax, const char *
         ok = ok && std::fprintf(f, "\n") > 0;
     }
     close_text_out(f, ok, err);
+}
+
+unsigned long charm_shc_read_bin(const char *pathname, unsigned long nmax, charm_shc *shcs,
+                                 charm_err *err)
+{
+    return guarded(err, CHARM_SHC_NMAX_ERROR, [&] { return shc_read_bin_impl(pathname, nmax, shcs, err); });
+}
+
+unsigned long charm_shc_read_tbl(const char *pathname, unsigned long nmax, charm_shc *shcs,
+                                 charm_err *err)
+{
+    return guarded(err, CHARM_SHC_NMAX_ERROR, [&] { return shc_read_tbl_impl(pathname, nmax, shcs, err); });
+}
+
+unsigned long charm_shc_read_dov(const char *pathname, unsigned long nmax, charm_shc *shcs,
+                                 charm_err *err)
+{
+    return guarded(err, CHARM_SHC_NMAX_ERROR, [&] { return shc_read_dov_impl(pathname, nmax, shcs, err); });
+}
+
+unsigned long charm_shc_read_mtx(const char *pathname, unsigned long nmax, charm_shc *shcs,
+                                 charm_err *err)
+{
+    return guarded(err, CHARM_SHC_NMAX_ERROR, [&] { return shc_read_mtx_impl(pathname, nmax, shcs, err); });
+}
+
+unsigned long charm_shc_read_gfc(const char *pathname, unsigned long nmax, const char *epoch,
+                                 charm_shc *shcs, charm_err *err)
+{
+    return guarded(err, CHARM_SHC_NMAX_ERROR,
+                   [&] { return shc_read_gfc_impl(pathname, nmax, epoch, shcs, err); });
+}
+
+void charm_shc_write_bin(const charm_shc *shcs, unsigned long nmax, const char *pathname,
+                         charm_err *err)
+{
+    guarded(err, 0, [&] { shc_write_bin_impl(shcs, nmax, pathname, err); return 0; });
+}
+
+void charm_shc_write_tbl(const charm_shc *shcs, unsigned long nmax, const char *formatting,
+                         int ordering, const char *pathname, charm_err *err)
+{
+    guarded(err, 0, [&] { shc_write_tbl_impl(shcs, nmax, formatting, ordering, pathname, err); return 0; });
+}
+
+void charm_shc_write_dov(const charm_shc *shcs, unsigned long nmax, const char *formatting,
+                         int ordering, const char *pathname, charm_err *err)
+{
+    guarded(err, 0, [&] { shc_write_dov_impl(shcs, nmax, formatting, ordering, pathname, err); return 0; });
+}
+
+void charm_shc_write_mtx(const charm_shc *shcs, unsigned long nmax, const char *formatting,
+                         const char *pathname, charm_err *err)
+{
+    guarded(err, 0, [&] { shc_write_mtx_impl(shcs, nmax, formatting, pathname, err); return 0; });
 }
 
 }  // extern "C"

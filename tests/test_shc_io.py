@@ -239,3 +239,38 @@ def test_large_model_round_trip(tmp_path):
         m.to_file(ftype, path, ordering=shc.WRITE_M)
         x = shc.Shc.from_file(ftype, path)
         assert np.array_equal(x.c, C) and np.array_equal(x.s, S), ftype
+
+
+def test_bin_reads_columns_and_survives_corrupt_headers(tmp_path):
+    """charm_shc_read_bin reads order by order (shc_read_bin.c:203-256): a
+    low-degree read of a file whose header claims a huge degree touches only
+    what it needs, a truncated file reports CHARM_EFILEIO with CHarm's
+    message, and nothing escapes the C ABI as an exception."""
+    import struct
+
+    from charm_amd.workloads import kaula_coeffs
+
+    C, S = kaula_coeffs(40, 3)
+    m = shc.Shc.from_arrays(40, C, S, 2.0, 3.0)
+    path = str(tmp_path / "m.bin")
+    m.to_file("bin", path)
+    low = shc.Shc.from_file("bin", path, 7)
+    ref = shc.Shc.from_file("bin", path)
+    for n in range(8):
+        for mm in range(n + 1):
+            assert low.get_coeffs(n, mm) == ref.get_coeffs(n, mm)
+    # header claiming degree 2^40 with 41 degrees of data behind it
+    raw = open(path, "rb").read()
+    huge = tmp_path / "huge.bin"
+    huge.write_bytes(struct.pack("<Q", 2 ** 40) + raw[8:])
+    assert shc.Shc.nmax_from_file("bin", str(huge)) == 2 ** 40
+    msg = _err_code("bin", str(huge), 5, shc.Shc.from_zeros(5))
+    assert "Failed to read the \"C\" coefficients" in msg
+    # truncated inside the S table
+    cut = tmp_path / "cut.bin"
+    cut.write_bytes(raw[:-100])
+    assert "Failed to read the \"S\" coefficients" in _err_code("bin", str(cut), 40,
+                                                                shc.Shc.from_zeros(40))
+    short = tmp_path / "short.bin"
+    short.write_bytes(raw[:12])
+    assert "scaling parameter" in _err_code("bin", str(short), 3, shc.Shc.from_zeros(3))
