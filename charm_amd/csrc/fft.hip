@@ -230,11 +230,10 @@ __device__ __forceinline__ cplx row_out(const RowArgs &a, const cplx *x, int k, 
 
 // f (2L reals per row) -> spec (L+1 complex per row), F_k = sum_j f_j e^{-2 pi i jk/2L}
 template <int MODE>
-__global__ __launch_bounds__(1024) void r2c_kernel(RowArgs a)
+__device__ __forceinline__ void r2c_row(const RowArgs &a, size_t row)
 {
     cplx *x = lds_rows;
     const int L = a.L;
-    const size_t row = blockIdx.x;
     const cplx *z = (const cplx *)(a.f + row * a.ldf);
     const fft::TwSplit tw = lds_twiddles(a.tw, a.plan.M);
     row_dft<MODE>(a, x, tw, [&](int n) { return z[n]; });
@@ -250,6 +249,12 @@ __global__ __launch_bounds__(1024) void r2c_kernel(RowArgs a)
         const cplx o = fft::rot90<-1>(fft::scale(fft::sub(zk, zr), 0.5));
         out[k] = fft::add(e, fft::mul(a.pt[k], o));
     }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(1024) void r2c_kernel(RowArgs a)
+{
+    r2c_row<MODE>(a, blockIdx.x);
 }
 
 // Input n of the inverse transform of a spectrum row: conj Z_n with
@@ -272,11 +277,10 @@ __device__ __forceinline__ cplx c2r_input(const RowArgs &a, const cplx *X, int n
 // spec (L+1 complex per row) -> f (2L reals per row),
 // f_j = sum_{m=0}^{2L-1} X_m e^{+2 pi i jm/2L} over the Hermitian extension
 template <int MODE>
-__global__ __launch_bounds__(1024) void c2r_kernel(RowArgs a)
+__device__ __forceinline__ void c2r_row(const RowArgs &a, size_t row)
 {
     cplx *x = lds_rows;
     const int L = a.L;
-    const size_t row = blockIdx.x;
     const cplx *X = a.spec + row * a.lds;
     const fft::TwSplit tw = lds_twiddles(a.tw, a.plan.M);
     row_dft<MODE>(a, x, tw, [&](int n) { return c2r_input(a, X, n); });
@@ -287,6 +291,37 @@ __global__ __launch_bounds__(1024) void c2r_kernel(RowArgs a)
         const cplx y = row_out<MODE>(a, x, n, v0);
         f[n] = fft::mk(2.0 * y.x, -2.0 * y.y);
     }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(1024) void c2r_kernel(RowArgs a)
+{
+    c2r_row<MODE>(a, blockIdx.x);
+}
+
+// Rows of many transforms of one mode in one launch: block b runs row
+// b - row0[j] of job j, the last job with row0[j] <= b (row0 ascending,
+// row0[0] = 0).  For batches of small grids, whose rows would otherwise take
+// one launch per grid.
+template <int MODE, bool C2R>
+__global__ __launch_bounds__(1024) void rows_group_kernel(const RowArgs *__restrict__ jobs,
+                                                          const unsigned *__restrict__ row0,
+                                                          int njob)
+{
+    const unsigned b = blockIdx.x;
+    int lo = 0, hi = njob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (row0[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const RowArgs &a = jobs[lo];
+    if (C2R)
+        c2r_row<MODE>(a, b - row0[lo]);
+    else
+        r2c_row<MODE>(a, b - row0[lo]);
 }
 
 // Spectrum of the convolution filter in the order the forward passes of
@@ -541,7 +576,13 @@ hipError_t set_lds_limits()
                          (const void *)split_a_kernel<fft::DFT_RADER, false>,
                          (const void *)split_a_kernel<fft::DFT_DIRECT, true>,
                          (const void *)split_a_kernel<fft::DFT_BLUESTEIN, true>,
-                         (const void *)split_a_kernel<fft::DFT_RADER, true>};
+                         (const void *)split_a_kernel<fft::DFT_RADER, true>,
+                         (const void *)rows_group_kernel<fft::DFT_DIRECT, false>,
+                         (const void *)rows_group_kernel<fft::DFT_BLUESTEIN, false>,
+                         (const void *)rows_group_kernel<fft::DFT_RADER, false>,
+                         (const void *)rows_group_kernel<fft::DFT_DIRECT, true>,
+                         (const void *)rows_group_kernel<fft::DFT_BLUESTEIN, true>,
+                         (const void *)rows_group_kernel<fft::DFT_RADER, true>};
     for (const void *fn : fns) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e != hipSuccess) return e;
@@ -802,6 +843,87 @@ int launch_row_fft(int c2r, size_t nlon, size_t nrow, double *f, size_t ldf, voi
         ROW_LAUNCH(r2c_kernel)
     }
 #undef ROW_LAUNCH
+    return (int)hipGetLastError();
+}
+
+bool row_group_supported(size_t nlon)
+{
+    return nlon >= 2 && nlon % 2 == 0 && nlon / 2 <= (size_t)INT32_MAX / 16 &&
+           single_row_plan(nlon);
+}
+
+// Device descriptors of grouped row transforms: one part per transform mode
+// (one launch each).  Plans are made here (this synchronises the stream).
+int row_group_build(const RowJob *jobs, int njob, void *stream, std::vector<RowGroupPart> *parts,
+                    std::vector<void *> *owned)
+{
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<RowArgs> args[3];
+    std::vector<unsigned> row0[3];
+    unsigned nrow[3] = {0, 0, 0};
+    int threads[3] = {64, 64, 64};
+    size_t lds[3] = {0, 0, 0};
+    for (int j = 0; j < njob; j++) {
+        const RowJob &q = jobs[j];
+        if (!row_group_supported(q.nlon) || q.nrow == 0 || q.nrow > (size_t)INT32_MAX)
+            return (int)hipErrorInvalidValue;
+        Plan p;
+        if (int e = get_plan((int)(q.nlon / 2), s, &p)) return e;
+        RowArgs a = row_args(p);
+        a.f = q.f;
+        a.ldf = q.ldf;
+        a.spec = (cplx *)q.spec;
+        a.lds = q.lds;
+        const int md = p.mp.mode;
+        if ((size_t)nrow[md] + q.nrow > (size_t)INT32_MAX) return (int)hipErrorInvalidValue;
+        args[md].push_back(a);
+        row0[md].push_back(nrow[md]);
+        nrow[md] += (unsigned)q.nrow;
+        threads[md] = std::max(threads[md], p.threads);
+        lds[md] = std::max(lds[md], lds_bytes(p.mp.M));
+    }
+    if (hipError_t e = set_lds_limits()) return (int)e;
+    for (int md = 0; md < 3; md++) {
+        if (args[md].empty()) continue;
+        RowGroupPart part;
+        part.mode = md;
+        part.njob = (int)args[md].size();
+        part.nrow = nrow[md];
+        part.threads = threads[md];
+        part.lds = lds[md];
+        void *da = nullptr, *dr = nullptr;
+        hipError_t e = hipMalloc(&da, args[md].size() * sizeof(RowArgs));
+        if (e != hipSuccess) return (int)e;
+        owned->push_back(da);
+        if ((e = hipMalloc(&dr, row0[md].size() * sizeof(unsigned))) != hipSuccess) return (int)e;
+        owned->push_back(dr);
+        if ((e = hipMemcpyAsync(da, args[md].data(), args[md].size() * sizeof(RowArgs),
+                                hipMemcpyHostToDevice, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(dr, row0[md].data(), row0[md].size() * sizeof(unsigned),
+                                hipMemcpyHostToDevice, s)) != hipSuccess ||
+            (e = hipStreamSynchronize(s)) != hipSuccess)
+            return (int)e;
+        part.args = da;
+        part.row0 = (const unsigned *)dr;
+        parts->push_back(part);
+    }
+    return 0;
+}
+
+int row_group_launch(int c2r, const std::vector<RowGroupPart> &parts, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    for (const RowGroupPart &p : parts) {
+        const RowArgs *a = (const RowArgs *)p.args;
+        const dim3 grid(p.nrow), block((unsigned)p.threads);
+#define GROUP_LAUNCH(MODE)                                                                             if (c2r)                                                                                               hipLaunchKernelGGL((rows_group_kernel<MODE, true>), grid, block, p.lds, s, a, p.row0,                                 p.njob);                                                                    else                                                                                                   hipLaunchKernelGGL((rows_group_kernel<MODE, false>), grid, block, p.lds, s, a, p.row0,                                p.njob);
+        switch (p.mode) {
+        case fft::DFT_DIRECT: GROUP_LAUNCH(fft::DFT_DIRECT) break;
+        case fft::DFT_BLUESTEIN: GROUP_LAUNCH(fft::DFT_BLUESTEIN) break;
+        default: GROUP_LAUNCH(fft::DFT_RADER) break;
+        }
+#undef GROUP_LAUNCH
+    }
     return (int)hipGetLastError();
 }
 

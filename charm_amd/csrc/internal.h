@@ -291,6 +291,51 @@ size_t row_fft_scratch(size_t n, size_t nrow);
 int launch_row_fft(int c2r, size_t n, size_t nrow, double *f, size_t ldf, void *spec, size_t lds,
                    void *scratch, void *stream);
 int prepare_row_fft(size_t n, void *stream);
+// Row transforms of many small grids in a few launches (one per transform
+// mode): the rows of job j are those of one launch_row_fft call
+struct RowJob {
+    size_t nlon, nrow;
+    double *f;
+    size_t ldf;
+    void *spec;
+    size_t lds;
+};
+struct RowGroupPart {
+    int mode = 0, njob = 0, threads = 64;
+    unsigned nrow = 0;
+    size_t lds = 0;
+    const void *args = nullptr;        // device RowArgs[njob]
+    const unsigned *row0 = nullptr;    // device first row of each job
+};
+// Per-request finishing steps of a batch, one launch for all requests
+struct ScaleJob {
+    int nmax;
+    const void *cs;  // double2 coefficients, m-major triangle of degree nmax
+    void *out;       // the same times s_nm
+};
+struct ShaReduceJob {
+    int nmax, ntile;
+    const void *part;     // [ntile][tri_len(nmax)] double2
+    const double *scale;  // [nmax + 1] or nullptr
+    double *cs;           // interleaved (C, S) triangle of degree nmax
+};
+struct SctrReduceJob {
+    size_t npoint;
+    int nchunk;
+    const double *part;  // [nchunk][npoint]
+    const int *perm;
+    double mur;
+    double *f;
+};
+int launch_scale_cs_group(const ScaleJob *jobs, int njob, int nmax_max, const double *sn, int nt,
+                          void *stream);
+int launch_sha_reduce_group(const ShaReduceJob *jobs, int njob, int nmax_max, const double *sn,
+                            int nt, void *stream);
+int launch_sctr_reduce_group(const SctrReduceJob *jobs, int njob, size_t npoint_max, void *stream);
+bool row_group_supported(size_t nlon);
+int row_group_build(const RowJob *jobs, int njob, void *stream, std::vector<RowGroupPart> *parts,
+                    std::vector<void *> *owned);
+int row_group_launch(int c2r, const std::vector<RowGroupPart> &parts, void *stream);
 
 // Polar optimisation threshold for a transform of degree nmax from the
 // globals: max(a1, a2 nmax), or -1 (off) when a2 < 0

@@ -269,6 +269,24 @@ __global__ void scale_cs_kernel(int nmax, const double2 *__restrict__ cs, const 
     }
 }
 
+// scale_cs_kernel for many coefficient sets in one launch: set blockIdx.z
+__global__ void scale_cs_group_kernel(const ScaleJob *__restrict__ jobs,
+                                      const double *__restrict__ sn, int nt)
+{
+    const ScaleJob q = jobs[blockIdx.z];
+    const int m = blockIdx.y;
+    if (m > q.nmax) return;
+    const double2 *src = (const double2 *)q.cs + dcol_off(q.nmax, m) - m;
+    double2 *dst = (double2 *)q.out + dcol_off(q.nmax, m) - m;
+    const double *s = sn + dcol_off(nt, m) - m;
+    for (int n = m + blockIdx.x * blockDim.x + threadIdx.x; n <= q.nmax;
+         n += gridDim.x * blockDim.x) {
+        const double2 v = src[n];
+        const double f = s[n];
+        dst[n] = make_double2(v.x * f, v.y * f);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Sectorial seeds P_mm, m = 1..nmax, as X-numbers, one lane per ring
 // (src/leg/leg_func_prepare.c:18-69).  Row m-1 of the table holds P_mm.
@@ -1247,6 +1265,38 @@ __global__ void sha_reduce_kernel(int nmax, int ntile, const double2 *__restrict
     }
 }
 
+// sha_reduce_kernel for many analyses in one launch (analysis blockIdx.z;
+// output at its own degree, interleaved C, S)
+__global__ void sha_reduce_group_kernel(const ShaReduceJob *__restrict__ jobs,
+                                        const double *__restrict__ sn, int nt)
+{
+    const ShaReduceJob q = jobs[blockIdx.z];
+    const int m = blockIdx.y, nmax = q.nmax;
+    if (m > nmax) return;
+    const size_t P = tri_len_dev(nmax);
+    const size_t col = dcol_off(nmax, m) - m;
+    const double2 *part = (const double2 *)q.part;
+    for (int n = m + blockIdx.x * blockDim.x + threadIdx.x; n <= nmax;
+         n += gridDim.x * blockDim.x) {
+        double vc = 0.0, vs = 0.0;
+        for (int tl = 0; tl < q.ntile; tl++) {
+            const double2 p = part[(size_t)tl * P + col + n];
+            vc += p.x;
+            vs += p.y;
+        }
+        const double f = sn[dcol_off(nt, m) + (n - m)];
+        vc *= f;
+        vs *= f;
+        if (q.scale) {
+            vc *= q.scale[n];
+            vs *= q.scale[n];
+        }
+        if (m == 0) vs = 0.0;
+        q.cs[2 * (col + n)] = vc;
+        q.cs[2 * (col + n) + 1] = vs;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // SHS at scattered points (src/shs/shs_point_sctr.c:294-687,
 // shs_sctr_mulc.c:36-57).  K points per lane (points sorted by latitude, so
@@ -1533,6 +1583,17 @@ __global__ void sctr_reduce_kernel(size_t npoint, int nchunk, const double *__re
     f[perm ? (size_t)perm[i] : i] = mur * s;
 }
 
+// sctr_reduce_kernel for many point sets in one launch (set blockIdx.y)
+__global__ void sctr_reduce_group_kernel(const SctrReduceJob *__restrict__ jobs)
+{
+    const SctrReduceJob q = jobs[blockIdx.y];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= q.npoint) return;
+    double s = q.part[i];
+    for (int c = 1; c < q.nchunk; c++) s += q.part[(size_t)c * q.npoint + i];
+    q.f[q.perm ? (size_t)q.perm[i] : i] = q.mur * s;
+}
+
 // points in the order of perm: t, u from lat; lon and r gathered
 __global__ void gather_points_kernel(size_t n, const int *__restrict__ perm,
                                      const double *__restrict__ lat, const double *__restrict__ lon,
@@ -1659,6 +1720,32 @@ int launch_scale_cs(int nmax, const void *cs, const double *sn, int nt, void *ou
     const unsigned gx = (unsigned)((nmax + 1 + 255) / 256);
     hipLaunchKernelGGL(scale_cs_kernel, dim3(gx, (unsigned)nmax + 1), dim3(256), 0, S(stream), nmax,
                        (const double2 *)cs, sn, nt, (double2 *)out);
+    return (int)hipGetLastError();
+}
+
+int launch_scale_cs_group(const ScaleJob *jobs, int njob, int nmax_max, const double *sn, int nt,
+                          void *stream)
+{
+    if (njob <= 0) return 0;
+    const dim3 grid((unsigned)((nmax_max + 1 + 255) / 256), (unsigned)nmax_max + 1, (unsigned)njob);
+    hipLaunchKernelGGL(scale_cs_group_kernel, grid, dim3(256), 0, S(stream), jobs, sn, nt);
+    return (int)hipGetLastError();
+}
+
+int launch_sha_reduce_group(const ShaReduceJob *jobs, int njob, int nmax_max, const double *sn,
+                            int nt, void *stream)
+{
+    if (njob <= 0) return 0;
+    const dim3 grid((unsigned)((nmax_max + 1 + 255) / 256), (unsigned)nmax_max + 1, (unsigned)njob);
+    hipLaunchKernelGGL(sha_reduce_group_kernel, grid, dim3(256), 0, S(stream), jobs, sn, nt);
+    return (int)hipGetLastError();
+}
+
+int launch_sctr_reduce_group(const SctrReduceJob *jobs, int njob, size_t npoint_max, void *stream)
+{
+    if (njob <= 0) return 0;
+    const dim3 grid((unsigned)((npoint_max + 255) / 256), (unsigned)njob);
+    hipLaunchKernelGGL(sctr_reduce_group_kernel, grid, dim3(256), 0, S(stream), jobs);
     return (int)hipGetLastError();
 }
 

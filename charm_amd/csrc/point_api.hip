@@ -2277,6 +2277,16 @@ struct charm_amd_batch {
     int n_sct_items = 0;
     int grid_sct = 0;
     int grid_shs = 0, grid_sha = 0;
+    // the per-request steps around the grouped kernels, each in one launch:
+    // row transforms of the grids with single-row plans (the others run per
+    // request), coefficient scaling and the reductions
+    std::vector<charm_amd::RowGroupPart> r2c_group, c2r_group;
+    std::vector<size_t> r2c_alone, c2r_alone;  // indices into sha / shs
+    charm_amd::ScaleJob *d_scale_shs = nullptr, *d_scale_sct = nullptr;
+    charm_amd::ShaReduceJob *d_red = nullptr;
+    charm_amd::SctrReduceJob *d_sred = nullptr;
+    int nmax_shs = 0, nmax_sha = 0, nmax_sct = 0;
+    size_t npoint_max = 0;
     bool use_graph = false;
     hipGraphExec_t exec = nullptr;
     hipStream_t cap = nullptr;
@@ -2316,36 +2326,35 @@ const int *polar_rings_owned(charm_amd_batch *b, const GridGeom &g, unsigned lon
 void batch_enqueue(charm_amd_batch *b, hipStream_t s)
 {
     HIPCHK((hipError_t)launch_zero_ints(b->d_counter, 3, s));
-    for (const auto &j : b->sha) {
-        const charm_amd_request &q = b->req[j.r];
-        rows_r2c(j.nlon, j.nlat, q.f, j.spec, j.nfc, s);
+    if (!b->sha.empty()) {
+        HIPCHK((hipError_t)row_group_launch(0, b->r2c_group, s));
+        for (size_t i : b->r2c_alone) {
+            const auto &j = b->sha[i];
+            rows_r2c(j.nlon, j.nlat, b->req[j.r].f, j.spec, j.nfc, s);
+        }
+        HIPCHK((hipError_t)launch_sha_batch(b->d_sha, b->d_sha_items, b->n_sha_items,
+                                            b->d_counter + 1, b->grid_sha, s));
+        HIPCHK((hipError_t)launch_sha_reduce_group(b->d_red, (int)b->sha.size(), b->nmax_sha, b->sn,
+                                                   b->nt, s));
     }
-    HIPCHK((hipError_t)launch_sha_batch(b->d_sha, b->d_sha_items, b->n_sha_items,
-                                        b->d_counter + 1, b->grid_sha, s));
-    for (const auto &j : b->sha) {
-        const charm_amd_request &q = b->req[j.r];
-        HIPCHK((hipError_t)launch_sha_reduce((int)q.nmax, j.ntile, j.part, b->sn, b->nt,
-                                             (int)q.nmax, j.scale, q.cs, q.cs + 1, 2, s));
+    if (!b->shs.empty()) {
+        HIPCHK((hipError_t)launch_scale_cs_group(b->d_scale_shs, (int)b->shs.size(), b->nmax_shs,
+                                                 b->sn, b->nt, s));
+        HIPCHK((hipError_t)launch_shs_batch(b->d_shs, b->d_shs_items, b->n_shs_items, b->d_counter,
+                                            b->grid_shs, s));
+        HIPCHK((hipError_t)row_group_launch(1, b->c2r_group, s));
+        for (size_t i : b->c2r_alone) {
+            const auto &j = b->shs[i];
+            rows_c2r(j.nlon, j.nlat, j.spec, j.nfc, b->req[j.r].f, s);
+        }
     }
-    for (const auto &j : b->shs) {
-        const charm_amd_request &q = b->req[j.r];
-        HIPCHK((hipError_t)launch_scale_cs((int)q.nmax, q.cs, b->sn, b->nt, j.part, s));
-    }
-    HIPCHK((hipError_t)launch_shs_batch(b->d_shs, b->d_shs_items, b->n_shs_items, b->d_counter,
-                                        b->grid_shs, s));
-    for (const auto &j : b->shs) {
-        const charm_amd_request &q = b->req[j.r];
-        rows_c2r(j.nlon, j.nlat, j.spec, j.nfc, q.f, s);
-    }
-    for (const auto &j : b->sct) {
-        const charm_amd_request &q = b->req[j.r];
-        HIPCHK((hipError_t)launch_scale_cs((int)q.nmax, q.cs, b->sn, b->nt, j.cs2, s));
-    }
-    HIPCHK((hipError_t)launch_sctr_batch(b->d_sct, b->d_sct_items, b->n_sct_items, b->d_counter + 2,
-                                         b->grid_sct, s));
-    for (const auto &j : b->sct) {
-        const charm_amd_request &q = b->req[j.r];
-        HIPCHK((hipError_t)launch_sctr_reduce(q.npoint, j.nchunk, j.part, j.perm, q.mu / q.R, q.f, s));
+    if (!b->sct.empty()) {
+        HIPCHK((hipError_t)launch_scale_cs_group(b->d_scale_sct, (int)b->sct.size(), b->nmax_sct,
+                                                 b->sn, b->nt, s));
+        HIPCHK((hipError_t)launch_sctr_batch(b->d_sct, b->d_sct_items, b->n_sct_items,
+                                             b->d_counter + 2, b->grid_sct, s));
+        HIPCHK((hipError_t)launch_sctr_reduce_group(b->d_sred, (int)b->sct.size(), b->npoint_max,
+                                                    s));
     }
 }
 
@@ -2528,6 +2537,56 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
                     for (int tl = 0; tl < a.ntile; tl++)
                         sha_items.push_back({(long)(nmax + 1 - 4 * qd), make_int2(idx, (tl << 16) | qd)});
             }
+        }
+        // descriptors of the grouped per-request steps
+        {
+            std::vector<RowJob> rj;
+            for (size_t i = 0; i < b->sha.size(); i++) {
+                const auto &j = b->sha[i];
+                if (row_group_supported(j.nlon))
+                    rj.push_back({j.nlon, j.nlat, b->req[j.r].f, j.nlon, j.spec, j.nfc});
+                else
+                    b->r2c_alone.push_back(i);
+            }
+            if (!rj.empty())
+                HIPCHK((hipError_t)row_group_build(rj.data(), (int)rj.size(), s, &b->r2c_group,
+                                                   &b->owned));
+            rj.clear();
+            for (size_t i = 0; i < b->shs.size(); i++) {
+                const auto &j = b->shs[i];
+                if (row_group_supported(j.nlon))
+                    rj.push_back({j.nlon, j.nlat, b->req[j.r].f, j.nlon, j.spec, j.nfc});
+                else
+                    b->c2r_alone.push_back(i);
+            }
+            if (!rj.empty())
+                HIPCHK((hipError_t)row_group_build(rj.data(), (int)rj.size(), s, &b->c2r_group,
+                                                   &b->owned));
+            std::vector<ScaleJob> sj;
+            for (const auto &j : b->shs) {
+                const charm_amd_request &q = b->req[j.r];
+                sj.push_back({(int)q.nmax, q.cs, j.part});
+                b->nmax_shs = std::max(b->nmax_shs, (int)q.nmax);
+            }
+            if (!sj.empty()) b->d_scale_shs = dupload(b, sj, s);
+            sj.clear();
+            std::vector<SctrReduceJob> sr;
+            for (const auto &j : b->sct) {
+                const charm_amd_request &q = b->req[j.r];
+                sj.push_back({(int)q.nmax, q.cs, j.cs2});
+                sr.push_back({q.npoint, j.nchunk, j.part, j.perm, q.mu / q.R, q.f});
+                b->nmax_sct = std::max(b->nmax_sct, (int)q.nmax);
+                b->npoint_max = std::max(b->npoint_max, q.npoint);
+            }
+            if (!sj.empty()) b->d_scale_sct = dupload(b, sj, s);
+            if (!sr.empty()) b->d_sred = dupload(b, sr, s);
+            std::vector<ShaReduceJob> rd;
+            for (const auto &j : b->sha) {
+                const charm_amd_request &q = b->req[j.r];
+                rd.push_back({(int)q.nmax, j.ntile, j.part, j.scale, q.cs});
+                b->nmax_sha = std::max(b->nmax_sha, (int)q.nmax);
+            }
+            if (!rd.empty()) b->d_red = dupload(b, rd, s);
         }
         auto sort_items = [](std::vector<Item> &v) {
             std::stable_sort(v.begin(), v.end(),
