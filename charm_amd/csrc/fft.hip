@@ -393,13 +393,14 @@ __global__ void twiddle_table_kernel(int M, cplx *tw)
 // runs the length-L2 LDS transform and applies the twiddle; pass B (one
 // thread per row and k2) runs the L1-point DFTs and the real/complex packing.
 // ---------------------------------------------------------------------------
-constexpr int SPLIT_R[] = {2, 3, 4, 5, 7, 8, 9, 16};
+constexpr int SPLIT_R[] = {2, 3, 4, 5, 7, 8, 9, 16, 11, 13};
 
 struct SplitArgs {
     RowArgs sub;       // the length-L2 plan
     int L, L1, L2;
     const cplx *ptf;   // [L+1] exp(-i pi k / L)
     const cplx *twf;   // [L]   exp(-2 pi i j / L)
+    const cplx *w1;    // [L1]  exp(-2 pi i q / L1) (short factors without a codelet)
     cplx *Y;           // [row][n1][k2]
     double *f;
     size_t ldf;
@@ -457,7 +458,10 @@ __device__ __forceinline__ void split_column(const SplitArgs &a, size_t row, int
     const cplx *y = a.Y + row * (size_t)R1 * a.L2 + k2;
 #pragma unroll
     for (int n1 = 0; n1 < R1; n1++) v[n1] = y[(size_t)n1 * a.L2];
-    fft::dft<R1, -1>(v);
+    if constexpr (R1 == 11 || R1 == 13)
+        fft::dft_table<R1>(v, a.w1);
+    else
+        fft::dft<R1, -1>(v);
 }
 
 template <int R1, bool C2R>
@@ -709,7 +713,7 @@ int split_factor(int L)
 
 struct SplitPlan {
     int L1 = 0;
-    cplx *pt = nullptr, *tw = nullptr;
+    cplx *pt = nullptr, *tw = nullptr, *w1 = nullptr;
 };
 std::map<int, SplitPlan> g_split;
 
@@ -731,6 +735,8 @@ int get_split(int L, hipStream_t s, SplitPlan *out)
     if ((e = hipMalloc((void **)&sp.tw, L * sizeof(cplx))) != hipSuccess) return (int)e;
     hipLaunchKernelGGL(full_tables_kernel, dim3((L + 1 + 255) / 256), dim3(256), 0, s, L, sp.pt,
                        sp.tw);
+    if ((e = hipMalloc((void **)&sp.w1, sp.L1 * sizeof(cplx))) != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(twiddle_table_kernel, dim3(1), dim3(64), 0, s, sp.L1, sp.w1);
     if ((e = hipGetLastError()) != hipSuccess) return (int)e;
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return (int)e;
     std::lock_guard<std::mutex> lk(g_fft_mu);
@@ -779,6 +785,7 @@ int launch_row_fft(int c2r, size_t nlon, size_t nrow, double *f, size_t ldf, voi
         a.L2 = L2;
         a.ptf = sp.pt;
         a.twf = sp.tw;
+        a.w1 = sp.w1;
         a.Y = (cplx *)scratch;
         a.f = f;
         a.ldf = ldf;
@@ -814,6 +821,7 @@ int launch_row_fft(int c2r, size_t nlon, size_t nrow, double *f, size_t ldf, voi
         break;
         switch (sp.L1) {
             SPLIT_B(2) SPLIT_B(3) SPLIT_B(4) SPLIT_B(5) SPLIT_B(7) SPLIT_B(8) SPLIT_B(9) SPLIT_B(16)
+            SPLIT_B(11) SPLIT_B(13)
         default: return (int)hipErrorInvalidValue;
         }
 #undef SPLIT_B

@@ -265,6 +265,41 @@ FFT_HD void dft(cplx *v)
     if (R == 16) dft16<SIGN>(v);
 }
 
+// R-point DFT from a table w[q] = exp(SIGN 2 pi i q / R) (primes without a
+// codelet); pairs k, R - k share the cosine sums of the pair sums and the sine
+// sums of the pair differences, as in dft5 and dft7 (R odd)
+template <int R>
+FFT_HD void dft_table(cplx *v, const cplx *w)
+{
+    static_assert(R % 2 == 1, "odd R");
+    cplx t[R / 2 + 1], d[R / 2 + 1];
+#pragma unroll
+    for (int j = 1; j <= R / 2; j++) {
+        t[j] = add(v[j], v[R - j]);
+        d[j] = sub(v[j], v[R - j]);
+    }
+    cplx out[R];
+    cplx s0 = v[0];
+#pragma unroll
+    for (int j = 1; j <= R / 2; j++) s0 = add(s0, t[j]);
+    out[0] = s0;
+#pragma unroll
+    for (int k = 1; k <= R / 2; k++) {
+        // m = v0 + sum_j t_j cos(2 pi jk/R); n = i sign sum_j d_j sin(2 pi jk/R)
+        cplx m = v[0], n = mk(0.0, 0.0);
+#pragma unroll
+        for (int j = 1; j <= R / 2; j++) {
+            const cplx wk = w[(j * k) % R];
+            m = axpy(wk.x, t[j], m);
+            n = add(n, mk(-wk.y * d[j].y, wk.y * d[j].x));
+        }
+        out[k] = add(m, n);
+        out[R - k] = sub(m, n);
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++) v[k] = out[k];
+}
+
 // exp(-2 pi i q / M) for 0 <= q < M from two short tables, hi[q / 64] =
 // exp(-2 pi i 64 (q / 64) / M) and lo[q mod 64] = exp(-2 pi i (q mod 64) / M):
 // ceil(M / 64) + 64 entries (a few KB, kept in LDS beside the row), one

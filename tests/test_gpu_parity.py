@@ -363,11 +363,12 @@ def test_device_grid_one_slot_rescale_roundtrip():
     assert err <= TOL
 
 
-@pytest.mark.parametrize("nlon", (20000, 21602, 24570))
+@pytest.mark.parametrize("nlon", (20000, 21602, 24570, 10802, 19682))
 def test_split_row_lengths_synthesis(orc, nlon):
     """Rows longer than one LDS transform (L = 10000 = 2 x 5000, 10801 =
-    7 x 1543, 12285 = 3 x 4095): the two-pass c2r against the oracle's direct
-    longitude sums."""
+    7 x 1543, 12285 = 3 x 4095), or with no single LDS plan (L = 5401 =
+    11 x 491 and 9841 = 13 x 757, split with the table DFTs of 11 and 13
+    points): the two-pass c2r against the oracle's direct longitude sums."""
     N = 60
     C, S = kaula_coeffs(N, 17)
     lat = np.array([1.2, 0.4, -0.05, -0.9])
@@ -378,13 +379,14 @@ def test_split_row_lengths_synthesis(orc, nlon):
     assert relmax(f, ref) <= TOL, relmax(f, ref)
 
 
-def test_split_row_lengths_roundtrip():
-    """DH2 grid of nmax 4999 (10000 x 20000): two-pass c2r and r2c."""
+@pytest.mark.parametrize("N", [4999, 5400])
+def test_split_row_lengths_roundtrip(N):
+    """DH2 grid of nmax 4999 (10000 x 20000) and GL grid of nmax 5400
+    (5401 x 10802, rows split 11 x 491): two-pass c2r and r2c."""
     from charm_amd.device import DevArray, Stream
 
-    N = 4999
     C, S = kaula_coeffs(N, 5)
-    g = crd.PointGridDH2(N, R_EARTH)
+    g = crd.PointGridDH2(N, R_EARTH) if N == 4999 else crd.PointGridGL(N, R_EARTH)
     m = model(N, C, S, GM, R_EARTH)
     f = shs.point(g, m, N)
     out = sha.point(g, f, N, GM, R_EARTH)
