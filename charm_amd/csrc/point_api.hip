@@ -614,7 +614,9 @@ struct DevGrid {
 // of waves, so small grids want few rings per lane; large grids amortise the
 // per-degree work of a wave (column staging, analysis's cross-lane reduction)
 // over more rings.  In ring pairs n: synthesis K = 1 up to 450, 2 up to 2500,
-// 3 beyond; analysis K = 2 up to 450, 4 beyond.
+// 3 beyond; analysis K = 2 up to 450, 4 up to 1200, 6 up to 2000, 8 beyond
+// (analysis at K = 6 and 8 runs 2 waves per SIMD instead of 3, which pays
+// from about nmax 3000 on: 8-10% faster than K = 4 at nmax 5400 to 10800).
 //
 // With CHARM_AMD_TUNE=1 an autotuner refines the rule at run time: the first
 // large transform of a kind in a ring-count bucket (a quarter octave) times
@@ -629,18 +631,10 @@ std::map<std::pair<int, int>, int> g_tuned;
 std::atomic<int> g_last_k[2];
 thread_local bool t_no_tune = false;
 
-int forced_k(const char *env, int kmin, int kmax)
-{
-    const char *v = std::getenv(env);
-    if (!v) return 0;
-    int k = std::atoi(v);
-    return (k >= kmin && k <= kmax) ? k : 0;
-}
-
 int fitted_k(int kind, size_t nring)
 {
     if (kind == 0) return nring <= 450 ? 1 : (nring <= 2500 ? 2 : 3);
-    return nring <= 450 ? 2 : 4;
+    return nring <= 450 ? 2 : (nring <= 1200 ? 4 : (nring <= 2000 ? 6 : 8));
 }
 
 bool tuning_on()
@@ -649,11 +643,26 @@ bool tuning_on()
     return v && v[0] == '1' && !t_no_tune;
 }
 
-template <class Launch>
-int tuned_k(int kind, int nring, unsigned long nmax, int kmin, int kmax, hipStream_t s,
-            Launch launch)
+// Ring pairs per lane that each grid kernel is built for
+constexpr int SHS_KS[] = {1, 2, 3};
+constexpr int SHA_KS[] = {2, 3, 4, 6, 8};
+
+bool valid_k(int kind, int k)
 {
-    if (int k = forced_k(kind == 0 ? "CHARM_AMD_SHS_K" : "CHARM_AMD_SHA_K", kmin, kmax)) return k;
+    for (int v : (kind == 0 ? std::vector<int>(std::begin(SHS_KS), std::end(SHS_KS))
+                            : std::vector<int>(std::begin(SHA_KS), std::end(SHA_KS))))
+        if (v == k) return true;
+    return false;
+}
+
+template <class Launch>
+int tuned_k(int kind, int nring, unsigned long nmax, hipStream_t s, Launch launch)
+{
+    const char *env = kind == 0 ? "CHARM_AMD_SHS_K" : "CHARM_AMD_SHA_K";
+    if (const char *v = std::getenv(env)) {
+        const int k = std::atoi(v);
+        if (valid_k(kind, k)) return k;
+    }
     const int rule = fitted_k(kind, (size_t)nring);
     const double work = (double)nring * (double)(nmax + 1) * (double)(nmax + 1);
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
@@ -667,35 +676,34 @@ int tuned_k(int kind, int nring, unsigned long nmax, int kmin, int kmax, hipStre
         auto it = g_tuned.find(key);
         if (it != g_tuned.end()) return it->second;
     }
+    const std::vector<int> ks = kind == 0 ? std::vector<int>(std::begin(SHS_KS), std::end(SHS_KS))
+                                          : std::vector<int>(std::begin(SHA_KS), std::end(SHA_KS));
     int best = rule;
     float best_ms = 0.0f;
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
-    float kms[8];
-    for (int k = kmin; k <= kmax; k++) {
-        launch(k);  // warm-up (code load, caches)
-        kms[k - kmin] = 0.0f;
-    }
+    std::vector<float> kms(ks.size(), 0.0f);
+    for (int k : ks) launch(k);  // warm-up (code load, caches)
     for (int round = 0; round < 3; round++) {
-        for (int k = kmin; k <= kmax; k++) {
+        for (size_t i = 0; i < ks.size(); i++) {
             HIPCHK(hipEventRecord(a, s));
-            launch(k);
+            launch(ks[i]);
             HIPCHK(hipEventRecord(b, s));
             HIPCHK(hipEventSynchronize(b));
             float ms = 0.0f;
             HIPCHK(hipEventElapsedTime(&ms, a, b));
-            if (round == 0 || ms < kms[k - kmin]) kms[k - kmin] = ms;
+            if (round == 0 || ms < kms[i]) kms[i] = ms;
         }
     }
-    for (int k = kmin; k <= kmax; k++) {
-        const float ms = kms[k - kmin];
+    for (size_t i = 0; i < ks.size(); i++) {
+        const float ms = kms[i];
         if (std::getenv("CHARM_AMD_TUNE_LOG"))
             std::fprintf(stderr, "charm_amd tune: kind %d nring %d nmax %lu K=%d %.3f ms\n", kind,
-                         nring, nmax, k, (double)ms);
-        if (k == kmin || ms < best_ms) {
+                         nring, nmax, ks[i], (double)ms);
+        if (i == 0 || ms < best_ms) {
             best_ms = ms;
-            best = k;
+            best = ks[i];
         }
     }
     (void)hipEventDestroy(a);
@@ -765,7 +773,7 @@ void shs_grid_lumped(const GridGeom &g, const DevGrid &d, bool fft, unsigned lon
     a.ldo = ld;
     a.mcount = (int)ld;
     a.fft_layout = fft ? 1 : 0;
-    a.k = tuned_k(0, (int)g.nring, nmax, 1, 3, stream, [&](int k) {
+    a.k = tuned_k(0, (int)g.nring, nmax, stream, [&](int k) {
         ShsGridArgs y = a;
         y.k = k;
         y.ntile = ring_tiles(g.nring, k);
@@ -1467,7 +1475,7 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *w_dev, double cw
     const bool fuse = nslot == 1 && nmax_out == nmax;
     DevBuf part((size_t)nslot * tri_len(nmax) * 2 * sizeof(double), s);
     a.part = part.p;
-    a.k = tuned_k(1, (int)g.nring, nmax, 2, 4, s, [&](int k) {
+    a.k = tuned_k(1, (int)g.nring, nmax, s, [&](int k) {
         ShaArgs y = a;
         y.k = k;
         group(y);

@@ -127,9 +127,15 @@ def test_polar_scattered_synthesis(polar, npoint):
     assert np.any(full != ref[:4096])
 
 
-@pytest.mark.parametrize("blocks", [(8, 4, 8), (1, 8, 8)])
-def test_polar_analysis(polar, blocks):
-    N = 200
+@pytest.mark.parametrize("blocks,K", [((8, 4, 8), None), ((1, 8, 8), None), ((8, 4, 8), 8),
+                                      ((8, 4, 8), 6)])
+def test_polar_analysis(polar, blocks, K, monkeypatch):
+    """Analysis with the block rule, also with 6 and 8 ring pairs per lane
+    (at nmax 1100, 551 ring pairs: a full tile of 512 and one of 39 rings for
+    K = 8, a full tile of 384 and one of 167 rings for K = 6)."""
+    N = 200 if K is None else 1100
+    if K is not None:
+        monkeypatch.setenv("CHARM_AMD_SHA_K", str(K))
     polar.set(A1, A2, *blocks)
     g = crd.PointGridGL(N, R_EARTH)
     rng = np.random.default_rng(35)
