@@ -1199,7 +1199,7 @@ __device__ void sha_finish_quad(const ShaArgs &A, int quad, int ngrp)
 }
 
 template <int K>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 3 ? 4 : (K <= 4 ? 3 : 2))))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 3 ? 4 : (K <= 5 ? 3 : 2))))
 void sha_kernel(ShaArgs A)
 {
     __shared__ __align__(16) ShaSmem<K> sm;
@@ -1818,6 +1818,7 @@ int launch_sha(const ShaArgs &a, void *stream)
     switch (a.k) {
     case 2: hipLaunchKernelGGL((sha_kernel<2>), dim3(nblk), dim3(256), 0, S(stream), a); break;
     case 3: hipLaunchKernelGGL((sha_kernel<3>), dim3(nblk), dim3(256), 0, S(stream), a); break;
+    case 5: hipLaunchKernelGGL((sha_kernel<5>), dim3(nblk), dim3(256), 0, S(stream), a); break;
     case 6: hipLaunchKernelGGL((sha_kernel<6>), dim3(nblk), dim3(256), 0, S(stream), a); break;
     case 8: hipLaunchKernelGGL((sha_kernel<8>), dim3(nblk), dim3(256), 0, S(stream), a); break;
     default: hipLaunchKernelGGL((sha_kernel<4>), dim3(nblk), dim3(256), 0, S(stream), a); break;

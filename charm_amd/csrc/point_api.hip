@@ -613,10 +613,11 @@ struct DevGrid {
 // charm_amd.tune`, which reruns that sweep): few rings leave the chip short
 // of waves, so small grids want few rings per lane; large grids amortise the
 // per-degree work of a wave (column staging, analysis's cross-lane reduction)
-// over more rings.  In ring pairs n: synthesis K = 1 up to 450, 2 up to 2500,
-// 3 beyond; analysis K = 2 up to 450, 4 up to 1200, 6 up to 2000, 8 beyond
-// (analysis at K = 6 and 8 runs 2 waves per SIMD instead of 3, which pays
-// from about nmax 3000 on: 8-10% faster than K = 4 at nmax 5400 to 10800).
+// over more rings.  In ring pairs n: synthesis K = 1 up to 450, 2 up to 900,
+// 3 beyond; analysis K = 2 up to 450, 4 up to 800, 5 up to 1600, 6 up to
+// 1900, 8 beyond (K = 5 keeps 3 waves per SIMD, 162 VGPRs; K = 6 and 8 run 2
+// waves per SIMD, which pays from about nmax 3500 on: 8-10% faster than
+// K = 4 at nmax 5400 to 10800).
 //
 // With CHARM_AMD_TUNE=1 an autotuner refines the rule at run time: the first
 // large transform of a kind in a ring-count bucket (a quarter octave) times
@@ -633,8 +634,11 @@ thread_local bool t_no_tune = false;
 
 int fitted_k(int kind, size_t nring)
 {
-    if (kind == 0) return nring <= 450 ? 1 : (nring <= 2500 ? 2 : 3);
-    return nring <= 450 ? 2 : (nring <= 1200 ? 4 : (nring <= 2000 ? 6 : 8));
+    if (kind == 0) return nring <= 450 ? 1 : (nring <= 900 ? 2 : 3);
+    if (nring <= 450) return 2;
+    if (nring <= 800) return 4;
+    if (nring <= 1600) return 5;
+    return nring <= 1900 ? 6 : 8;
 }
 
 bool tuning_on()
@@ -645,7 +649,7 @@ bool tuning_on()
 
 // Ring pairs per lane that each grid kernel is built for
 constexpr int SHS_KS[] = {1, 2, 3};
-constexpr int SHA_KS[] = {2, 3, 4, 6, 8};
+constexpr int SHA_KS[] = {2, 3, 4, 5, 6, 8};
 
 bool valid_k(int kind, int k)
 {

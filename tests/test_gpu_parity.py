@@ -295,17 +295,18 @@ def test_device_grid_radius_differs_from_R():
     assert np.array_equal(f.numpy(), f_host)
 
 
-@pytest.mark.parametrize("K", (2, 3, 6, 8))
+@pytest.mark.parametrize("K", (2, 3, 5, 6, 8))
 def test_device_grid_skips_all_zero_waves(K, monkeypatch):
     """Repeated transforms on one device grid skip the waves (order, ring
     tile) whose every term the first transform found to be exactly zero.  At
     nmax = 1600 several hundred waves near the poles are such.  Synthesis:
     every call equals the host entry point (which never skips) bit for bit.
     Analysis on a fresh grid: the first call records the flags without
-    skipping, later calls skip and must equal it bit for bit.  K = 6 and 8
+    skipping, later calls skip and must equal it bit for bit.  K = 5, 6 and 8
     (analysis only; synthesis keeps its rule): 801 ring pairs are two full
-    tiles and one of 33 rings (K = 6), or one full tile and one of 289 rings,
-    run at 6 pairs per lane (K = 8)."""
+    tiles and one of 161 rings (K = 5), two full tiles and one of 33 rings
+    (K = 6), or one full tile and one of 289 rings, run at 6 pairs per lane
+    (K = 8)."""
     from charm_amd.device import DevArray, DeviceGrid, Stream
 
     monkeypatch.setenv("CHARM_AMD_SHS_K", str(K))
