@@ -90,9 +90,12 @@ struct ThreadState {
     std::map<std::tuple<int, size_t, size_t, size_t, hipStream_t>, hipfftHandle> plans;
     void *pinned = nullptr;
     size_t pinned_size = 0;
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};  // last copy out of each staging half
     ~ThreadState()
     {
         for (auto &kv : plans) hipfftDestroy(kv.second);
+        for (hipEvent_t e : stage_ev)
+            if (e) (void)hipEventDestroy(e);
         if (pinned) (void)hipHostFree(pinned);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -498,19 +501,6 @@ bool fft_applicable(const charm_point *pnt, unsigned long nmax)
            nearly_equal(pnt->lon[nlon - 1] + dl, 2.0 * PI, charm_glob_threshold);
 }
 
-// Pack C, S of degree shcs->nmax into (C, S) pairs of degree nmax; S_n0 = 0.
-void pack_coeffs(const charm_shc *shcs, unsigned long nmax, double *cs)
-{
-    size_t q = 0;
-    for (unsigned long m = 0; m <= nmax; m++) {
-        const double *c = shcs->c[m], *s = shcs->s[m];
-        for (unsigned long n = m; n <= nmax; n++, q++) {
-            cs[2 * q] = c[n - m];
-            cs[2 * q + 1] = (m == 0) ? 0.0 : s[n - m];
-        }
-    }
-}
-
 int ring_tiles(size_t nring, int K)
 {
     return (int)((nring + 64 * (size_t)K - 1) / (64 * (size_t)K));
@@ -884,14 +874,38 @@ void grid_setup(const charm_point *pnt, const charm_shc *shcs, unsigned long nma
     }
 }
 
-// Coefficients of degree nmax as (C, S) pairs on the device
+// Coefficients of degree nmax as (C, S) pairs on the device, packed into two
+// pinned staging halves of at most 2^20 pairs (16 MiB) each and copied half
+// by half; a half is refilled once its previous copy has completed.  A large
+// model so pins at most 32 MiB of host memory (pinning the whole triangle,
+// 933 MB at nmax 10800, made the first call of a process at that degree take
+// twice as long as the next ones).
 void upload_coeffs(const charm_shc *shcs, unsigned long nmax, hipStream_t s, DevBuf &cs)
 {
+    constexpr size_t STAGE_PAIRS = size_t(1) << 20;
     const size_t P = tri_len(nmax);
-    double *hcs = (double *)pinned_buffer(P * 2 * sizeof(double));
-    pack_coeffs(shcs, nmax, hcs);
     cs.alloc(P * 2 * sizeof(double), s);
-    h2d(cs.as<double>(), hcs, 2 * P, s);
+    const size_t half = std::min(P, STAGE_PAIRS);
+    double *h = (double *)pinned_buffer(2 * half * 2 * sizeof(double));
+    ThreadState &st = tls();
+    for (hipEvent_t &e : st.stage_ev)
+        if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    unsigned long m = 0, n = 0;  // the next pair to pack
+    bool used[2] = {false, false};
+    int which = 0;
+    for (size_t q0 = 0; q0 < P; q0 += half, which ^= 1) {
+        const size_t cnt = std::min(half, P - q0);
+        double *buf = h + (size_t)which * half * 2;
+        if (used[which]) HIPCHK(hipEventSynchronize(st.stage_ev[which]));
+        for (size_t i = 0; i < cnt; i++) {
+            buf[2 * i] = shcs->c[m][n - m];
+            buf[2 * i + 1] = (m == 0) ? 0.0 : shcs->s[m][n - m];
+            if (++n > nmax) n = ++m;
+        }
+        h2d(cs.as<double>() + 2 * q0, buf, 2 * cnt, s);
+        HIPCHK(hipEventRecord(st.stage_ev[which], s));
+        used[which] = true;
+    }
 }
 
 void shs_grid_host(const charm_point *pnt, const charm_shc *shcs, unsigned long nmax, double *f,
@@ -1026,11 +1040,8 @@ void shs_sctr_host(const charm_point *pnt, const charm_shc *shcs, unsigned long 
     for (size_t i = 0; i < npoint && unit; i++)
         unit = nearly_equal(pnt->r[i], shcs->r, charm_glob_threshold);
     Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
-    const size_t P = tri_len(nmax);
-    double *hcs = (double *)pinned_buffer(P * 2 * sizeof(double));
-    pack_coeffs(shcs, nmax, hcs);
-    DevBuf cs(P * 2 * sizeof(double), s);
-    h2d(cs.as<double>(), hcs, 2 * P, s);
+    DevBuf cs;
+    upload_coeffs(shcs, nmax, s, cs);
     DevBuf dlat(npoint * sizeof(double), s), dl(npoint * sizeof(double), s);
     DevBuf dr(npoint * sizeof(double), s);
     h2d(dlat.as<double>(), pnt->lat, npoint, s);
