@@ -380,13 +380,16 @@ void finish(charm_amd_server *s, std::vector<SrvReq *> &unit)
     s->stream_busy[unit.front()->stream] = false;
 }
 
-// Admission test of a request of `bytes`, with the server lock held; the
-// library's cached scratch counts against the budget, so it is trimmed first
-bool fits(charm_amd_server *s, size_t bytes)
+// Admission test of a request of `bytes`, with the server lock held.  The
+// library's cached scratch counts against the budget: when it would not fit
+// beside the admitted requests, *trim_to receives the size to trim the cache
+// to, which the worker does after releasing the lock (trimming waits for the
+// buffers' last uses and frees device memory)
+bool fits(charm_amd_server *s, size_t bytes, size_t *trim_to)
 {
     if (s->in_use + bytes > s->budget) return false;
-    const size_t cached = scratch_cached_bytes();
-    if (s->in_use + bytes + cached > s->budget) scratch_trim(s->budget - s->in_use - bytes);
+    const size_t room = s->budget - s->in_use - bytes;
+    if (scratch_cached_bytes() > room) *trim_to = std::min(*trim_to, room);
     return true;
 }
 
@@ -415,6 +418,7 @@ void worker_loop(charm_amd_server *s)
         std::vector<std::vector<SrvReq *>> launch;
         std::vector<int> launch_stream;
         std::vector<SrvReq *> skipped;
+        size_t trim_to = SIZE_MAX;
         int nfree = 0;
         for (bool b : s->stream_busy) nfree += !b;
         for (auto it = s->pending.begin(); it != s->pending.end() && nfree > 0;) {
@@ -424,7 +428,7 @@ void worker_loop(charm_amd_server *s)
             if (starving) break;
             // with nothing in flight a request runs whatever the budget says
             // (its own bytes were checked against it at submission)
-            if (!fits(s, q->bytes) && !(s->inflight.empty() && launch.empty())) {
+            if (!fits(s, q->bytes, &trim_to) && !(s->inflight.empty() && launch.empty())) {
                 skipped.push_back(q);
                 ++it;
                 continue;
@@ -437,7 +441,7 @@ void worker_loop(charm_amd_server *s)
             if (group)
                 for (auto jt = it; jt != s->pending.end() && (int)unit.size() < GROUP_MAX;) {
                     SrvReq *p = *jt;
-                    if (groupable(p) && fits(s, p->bytes)) {
+                    if (groupable(p) && fits(s, p->bytes, &trim_to)) {
                         s->in_use += p->bytes;
                         p->state = 1;
                         unit.push_back(p);
@@ -457,6 +461,7 @@ void worker_loop(charm_amd_server *s)
         }
         if (!launch.empty()) {
             lk.unlock();
+            if (trim_to != SIZE_MAX) scratch_trim(trim_to);
             for (size_t u = 0; u < launch.size(); u++) start(s, launch[u], launch_stream[u]);
             lk.lock();
             for (auto &unit : launch) s->inflight.push_back(unit);

@@ -11,6 +11,8 @@
   dda_n = sqrt(sum_m (dC_nm^2 + dS_nm^2)), compared degree by degree with the
   oracle's (CHarm's algorithm in double) on the same input.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -59,6 +61,29 @@ def test_charm_protocol_roundtrip(N):
     print("protocol N=%d: eps_max %.3e eps_rms %.3e (CHarm published %.2e / %.2e)"
           % (N, e_max, e_rms, p_max, p_rms))
     assert e_max <= 2.0 * p_max and e_rms <= 2.0 * p_rms
+
+
+@pytest.mark.parametrize("N", [1000, 2160])
+def test_protocol_equals_charm_algorithm_in_double(N):
+    """The protocol's errors are those of CHarm's algorithm in double: the
+    CPU restatement of CHarm's GL transforms (oracle/cpu_gl.c: folded ring
+    pairs, CHarm's recurrences and X-numbers, FFT rows) on the same
+    coefficients gives eps_max and eps_rms within 1.3x of the GPU's (measured:
+    3.39e-12 / 1.324e-13 against 3.45e-12 / 1.318e-13 at N = 2160; at N =
+    10800, 7.35e-11 / 8.60e-13 against 7.60e-11 / 8.65e-13)."""
+    from oracle.cpu_gl import CpuGL
+
+    C, S = uniform_coeffs(N, N)
+    Co, So, grid = roundtrip_dev(N, C, S, 1.0, 1.0)
+    g_max, g_rms = eps(Co, So, C, S, N)
+    p = grid.pnt
+    cg = CpuGL(max(1, min(16, len(os.sched_getaffinity(0)))))
+    f = cg.shs(N, C, S, 1.0, 1.0, 1.0, p.lat, len(p.lon))
+    Cc, Sc = cg.sha(N, f, p.lat, p.w, 1.0, 1.0, 1.0)
+    c_max, c_rms = eps(Cc, Sc, C, S, N)
+    print("N=%d GPU eps_max %.3e eps_rms %.3e; CHarm's algorithm on the CPU %.3e / %.3e"
+          % (N, g_max, g_rms, c_max, c_rms))
+    assert g_max <= 1.3 * c_max and g_rms <= 1.3 * c_rms
 
 
 @pytest.mark.parametrize("N,kind", [(1000, "uniform"), (720, "kaula")])
