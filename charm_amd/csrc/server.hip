@@ -38,6 +38,7 @@ namespace {
 constexpr int BYPASS_MAX = 8;                  // later requests passing one that does not fit
 constexpr size_t GROUP_MAX_POINTS = 1 << 17;   // scattered requests grouped up to this size
 constexpr int GROUP_MAX = 32;                  // requests per group launch
+constexpr size_t PINNED_CAP = size_t(4) << 30;  // pinned staging kept for reuse
 
 double now_ms()
 {
@@ -136,6 +137,9 @@ size_t footprint(const charm_amd_srv_request &r)
            grid_call_scratch(r.kind == CHARM_AMD_SRV_SHA_GRID, nring, nlat, nlon, r.nmax);
 }
 
+// A pinned staging buffer of at least n bytes from the pool (reused when an
+// idle one is at most twice as large).  Idle buffers are released, largest
+// first, while a new one would take the pool beyond PINNED_CAP bytes.
 Pinned *pinned_get(charm_amd_server *s, size_t n)
 {
     n = std::max<size_t>(n, 64);
@@ -145,6 +149,18 @@ Pinned *pinned_get(charm_amd_server *s, size_t n)
             p->busy = true;
             return p;
         }
+    size_t total = 0;
+    for (Pinned *p : s->pinned) total += p->n;
+    while (total + n > PINNED_CAP) {
+        auto big = s->pinned.end();
+        for (auto it = s->pinned.begin(); it != s->pinned.end(); ++it)
+            if (!(*it)->busy && (big == s->pinned.end() || (*it)->n > (*big)->n)) big = it;
+        if (big == s->pinned.end()) break;  // all busy: the pool grows
+        total -= (*big)->n;
+        (void)hipHostFree((*big)->p);
+        delete *big;
+        s->pinned.erase(big);
+    }
     Pinned *p = new Pinned();
     if (hipHostMalloc(&p->p, n, hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
