@@ -478,7 +478,17 @@ void worker_loop(charm_amd_server *s)
         if (!launch.empty()) {
             lk.unlock();
             if (trim_to != SIZE_MAX) scratch_trim(trim_to);
-            for (size_t u = 0; u < launch.size(); u++) start(s, launch[u], launch_stream[u]);
+            for (size_t u = 0; u < launch.size(); u++) {
+                try {
+                    start(s, launch[u], launch_stream[u]);
+                } catch (...) {  // a failed host allocation: the unit fails, the worker goes on
+                    std::lock_guard<std::mutex> g(s->mu);
+                    for (SrvReq *q : launch[u]) {
+                        q->stream = launch_stream[u];
+                        fail(q, CHARM_EMEM, "Failed to start the request on the host.");
+                    }
+                }
+            }
             lk.lock();
             for (auto &unit : launch) s->inflight.push_back(unit);
             progress = true;
@@ -522,8 +532,22 @@ charm_amd_server *charm_amd_server_create(size_t hbm_budget_bytes, int max_infli
     return s;
 }
 
+static int server_add_model(charm_amd_server *s, const charm_shc *shcs, unsigned long nmax,
+                            charm_err *err);
+
 int charm_amd_server_add_model(charm_amd_server *s, const charm_shc *shcs, unsigned long nmax,
                                charm_err *err)
+{
+    try {
+        return server_add_model(s, shcs, nmax, err);
+    } catch (...) {
+        CHARM_AMD_ERR(err, CHARM_EMEM, "Failed to add the model (host memory).");
+        return -1;
+    }
+}
+
+static int server_add_model(charm_amd_server *s, const charm_shc *shcs, unsigned long nmax,
+                            charm_err *err)
 {
     if (!s || !shcs || nmax > shcs->nmax) {
         CHARM_AMD_ERR(err, CHARM_EFUNCARG, "Invalid model.");
@@ -562,7 +586,19 @@ int charm_amd_server_add_model(charm_amd_server *s, const charm_shc *shcs, unsig
     return (int)s->models.size() - 1;
 }
 
+static long server_submit(charm_amd_server *s, const charm_amd_srv_request *r, charm_err *err);
+
 long charm_amd_server_submit(charm_amd_server *s, const charm_amd_srv_request *r, charm_err *err)
+{
+    try {
+        return server_submit(s, r, err);
+    } catch (...) {
+        CHARM_AMD_ERR(err, CHARM_EMEM, "Failed to queue the request (host memory).");
+        return -1;
+    }
+}
+
+static long server_submit(charm_amd_server *s, const charm_amd_srv_request *r, charm_err *err)
 {
     if (!s || !r || !r->out ||
         (r->kind != CHARM_AMD_SRV_SHS_SCTR && r->kind != CHARM_AMD_SRV_SHS_GRID &&
