@@ -861,7 +861,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); "
-              "multi-GPU sharding is described in DESIGN.md, not implemented." % args.gpus,
+              "sharding across GPUs is designed, not implemented." % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "gl2160":
