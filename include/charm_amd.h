@@ -99,6 +99,54 @@ charm_shc *charm_shc_init(unsigned long nmax, double mu, double r, double *c, do
 /* src/shc/shc.h.in:602 */
 void charm_shc_free(charm_shc *shcs);
 
+/* Coefficient utilities (host code).  The ranges are degrees nmin..nmax of
+ * every order; mu and R must match where two structures meet (within
+ * charm_glob_threshold); a bad argument sets CHARM_EFUNCARG in err. */
+/* src/shc/shc.h.in:555-587: copy of degrees nmin..nmax into a new zeroed
+ * structure of degree nmax_shcs_out; NULL on a bad range */
+charm_shc *charm_shc_copy(const charm_shc *shcs, unsigned long nmin, unsigned long nmax,
+                          unsigned long nmax_shcs_out);
+/* src/shc/shc.h.in:1320-1363: rop = op1 + op2 on degrees nmin..nmax
+ * (nmax <= rop->nmax), operands zero padded beyond their nmax; in place
+ * allowed */
+void charm_shc_add(charm_shc *rop, charm_shc *op1, charm_shc *op2, unsigned long nmin,
+                   unsigned long nmax, charm_err *err);
+/* src/shc/shc.h.in:1371-1380 */
+void charm_shc_sub(charm_shc *rop, charm_shc *op1, charm_shc *op2, unsigned long nmin,
+                   unsigned long nmax, charm_err *err);
+/* src/shc/shc.h.in:1388-1401 */
+void charm_shc_mul(charm_shc *rop, charm_shc *op1, charm_shc *op2, unsigned long nmin,
+                   unsigned long nmax, charm_err *err);
+/* src/shc/shc.h.in:1474-1491 (nmax <= op2->nmax; no zero divisor except
+ * S_{n,0}) */
+void charm_shc_div(charm_shc *rop, charm_shc *op1, charm_shc *op2, unsigned long nmin,
+                   unsigned long nmax, charm_err *err);
+/* src/shc/shc.h.in:1408-1434: degree n times a[n - nmin] */
+void charm_shc_mul_degree_wise(charm_shc *shcs, const double *a, unsigned long nmin,
+                               unsigned long nmax, charm_err *err);
+/* src/shc/shc.h.in:1441-1466: order m times a[m] (nmax + 1 factors) */
+void charm_shc_mul_order_wise(charm_shc *shcs, const double *a, unsigned long nmin,
+                              unsigned long nmax, charm_err *err);
+/* src/shc/shc.h.in:1499-1507 */
+void charm_shc_div_degree_wise(charm_shc *shcs, const double *a, unsigned long nmin,
+                               unsigned long nmax, charm_err *err);
+/* src/shc/shc.h.in:1515-1523 */
+void charm_shc_div_order_wise(charm_shc *shcs, const double *a, unsigned long nmin,
+                              unsigned long nmax, charm_err *err);
+/* src/shc/shc.h.in:1545-1571: dv[n] = sum_m C_nm^2 + S_nm^2, n = 0..nmax */
+void charm_shc_dv(const charm_shc *shcs, unsigned long nmax, double *dv, charm_err *err);
+/* src/shc/shc.h.in:1578-1603: sqrt of dv */
+void charm_shc_da(const charm_shc *shcs, unsigned long nmax, double *da, charm_err *err);
+/* src/shc/shc.h.in:1610-1639: degree variances of shcs1 - shcs2 */
+void charm_shc_ddv(const charm_shc *shcs1, const charm_shc *shcs2, unsigned long nmax,
+                   double *ddv, charm_err *err);
+/* src/shc/shc.h.in:1646-1675 */
+void charm_shc_dda(const charm_shc *shcs1, const charm_shc *shcs2, unsigned long nmax,
+                   double *dda, charm_err *err);
+/* src/shc/shc.h.in:1693-1722: to munew and rnew, degree n times
+ * (mu / munew) (R / rnew)^n */
+void charm_shc_rescale(charm_shc *shcs, double munew, double rnew, charm_err *err);
+
 /* Coefficient files (src/shc/shc.h.in:60-108, 618-1330).  The readers return
  * the maximum degree stored in the file (CHARM_SHC_NMAX_ERROR on error); with
  * nmax = CHARM_SHC_NMAX_MODEL and shcs = NULL they only read that degree.

@@ -2,8 +2,9 @@
 
 The fixture files under ``tests/golden/charm_data/{double,quad}`` are data
 files copied from CHarm 0.4.9's ``data/tests`` folder (outputs of CHarm built
-in double and in quad precision).  The input model is CHarm's
-``data/input/EGM96-degree10-mtx.txt``.
+in double and in quad precision).  The input models are CHarm's
+``data/input/EGM96-degree10-mtx.txt`` and, for the degree variance tests,
+``data/input/Earth2014-RET2014-degree4-mtx.txt``.
 """
 import os
 
