@@ -70,7 +70,8 @@ struct ShsGridArgs {
 struct ShaArgs {
     int nmax;
     int nring;         // ring pairs ("north" rings, incl. the equator)
-    int nlat;
+    int nlat;          // ring i mirrors to row nlat - 1 - i
+    int nrow;          // rows of spec and cw
     int ntile;
     const double *t;   // [nring]
     const double *sx;

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 
 #include "internal.h"
 
@@ -853,42 +854,90 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
     const int m = m0 + wave;
     const int nmax = A.nmax;
     double *lds = sm.buf;
-    // each tile's transpose reads are issued before the next tile's
-    // arithmetic and summed after it, which hides the LDS latency (an
-    // immediate sum frees 32 VGPRs but measured slower for every K)
-    constexpr bool PIPE = true;
+    // K = 5 and K = 8: each tile's transpose reads are issued before the next
+    // tile's arithmetic and summed after it, which hides the LDS latency at
+    // the 3 (K = 5) or 2 (K = 8) waves per SIMD these kernels run at.  Other
+    // K: an immediate sum, whose 32 fewer VGPRs let K <= 4 run at 4 waves per
+    // SIMD and K = 6 at 3 (nmax 2160, K = 4: 0.55 ms against 0.58 ms
+    // pipelined; K = 6 with its slot reduction: 0.534 ms against 0.542 ms
+    // pipelined at 2 waves).
+    constexpr bool PIPE = K == 5 || K >= 8;
 
-    // stage the weighted spectra A = cw Re F, B = -cw Im F of the tile's rows
-    // for orders m0..m0+3 (64-byte runs per row)
-    double2 *stage = (double2 *)lds;
-    const double2 *spec = (const double2 *)A.spec;
-    for (int ei = threadIdx.x; ei < 2 * 64 * K * 4; ei += 256) {
-        const int col = ei & 3, rowsel = ei >> 2;
-        const int ns = rowsel / (64 * K), rl = rowsel % (64 * K);
-        const int i = rbase + rl;
-        const int mm = m0 + col;
-        double2 v = make_double2(0.0, 0.0);
-        if (i < A.nring && mm <= nmax) {
-            int row = i;
-            if (ns) row = A.mirror[i] ? A.nlat - 1 - i : -1;
-            if (row >= 0) {
-                const double2 F = spec[(size_t)row * A.lds + mm];
-                const double cw = A.cws * A.cw[row];
-                v = make_double2(cw * F.x, -cw * F.y);
-            }
-        }
-        stage[ei] = v;
-    }
-    __syncthreads();
-
+    // the wave's per-ring data (sin of the latitude, the sectorial seed of
+    // order m, the polar optimisation's first dropped order) and its
+    // all-zero flag, requested first and from indices clamped into the
+    // arrays, so that these loads and the staging's overlap
+    const bool col = m <= nmax;  // wave-uniform
     int ring[K];
-    double t[K], ampC[K], ammC[K], ampS[K], ammS[K];
+    double t[K], sxs[K];
+    int sxe[K], pm[K];
     bool valid[K];
+    int rc[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
         ring[k] = rbase + k * 64 + lane;
         valid[k] = ring[k] < A.nring;
-        t[k] = valid[k] ? A.t[ring[k]] : 0.0;
+        rc[k] = min(ring[k], A.nring - 1);
+        t[k] = A.t[rc[k]];
+        pm[k] = A.pm0 != nullptr ? A.pm0[rc[k]] : INT_MAX;
+    }
+    {
+        // seed row m - 1, clamped (the tables hold at least one row); the
+        // values are used for 0 < m <= nmax only
+        const size_t srow = (size_t)max(0, min(m - 1, nmax - 1)) * A.sld;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            sxs[k] = A.sx[srow + rc[k]];
+            sxe[k] = A.six[srow + rc[k]];
+        }
+    }
+    // (read from the mirror flags when there is no table, so that the load
+    // needs no branch)
+    const int dead_v = (A.dead_mode == 2 ? A.dead + (size_t)min(m, nmax) * A.ntile + tile : A.mirror)[0];
+
+    // stage the weighted spectra A = cw Re F, B = -cw Im F of the tile's rows
+    // for orders m0..m0+3 (64-byte runs per row): thread (r, c) = (tid / 4,
+    // tid mod 4) takes order m0 + c of the rings rbase + r + 64k and of their
+    // mirrors.  The loads are unconditional, from indices clamped into the
+    // arrays, so that a thread issues all of them before it waits for any;
+    // what lies outside the tile, above nmax or has no mirror is dropped
+    // afterwards.  The southern row nlat-1-i is clamped to the last row (on
+    // Driscoll--Healy grids the north pole's would be row nrow, the missing
+    // south pole).
+    double2 *stage = (double2 *)lds;
+    {
+        const double2 *spec = (const double2 *)A.spec;
+        const int c = threadIdx.x & 3, r = threadIdx.x >> 2;
+        const int mm = m0 + c, mc = min(mm, nmax);
+        double2 F[2 * K];
+        double w[2 * K];
+        int mir[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int ic = min(rbase + r + 64 * k, A.nring - 1);
+            const int rows[2] = {ic, min(A.nlat - 1 - ic, A.nrow - 1)};
+            mir[k] = A.mirror[ic];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                F[h * K + k] = spec[(size_t)rows[h] * A.lds + mc];
+                w[h * K + k] = A.cw[rows[h]];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 2 * K; j++) {
+            const int k = j % K;
+            const bool in = rbase + r + 64 * k < A.nring && mm <= nmax && (j < K || mir[k] != 0);
+            const double cw = A.cws * w[j];
+            stage[(j * 64 + r) * 4 + c] =
+                in ? make_double2(cw * F[j].x, -cw * F[j].y) : make_double2(0.0, 0.0);
+        }
+    }
+    __syncthreads();
+
+    double ampC[K], ammC[K], ampS[K], ammS[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        if (!valid[k]) t[k] = 0.0;
         const double2 N = stage[((0 * 64 * K) + k * 64 + lane) * 4 + wave];
         const double2 S = stage[((1 * 64 * K) + k * 64 + lane) * 4 + wave];
         // fold (src/sha/sha_point.c:728-748); rows without a mirror hold S = 0
@@ -908,12 +957,12 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
     bool skip[K], allskip = true;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        skip[k] = !valid[k] || (A.pm0 != nullptr && m >= A.pm0[ring[k]]);
+        skip[k] = !valid[k] || m >= pm[k];
         allskip = allskip && skip[k];
     }
     // a wave whose column was found all zero by an earlier transform on this
     // grid (same degree and tiling) contributes nothing
-    const bool dead_skip = A.dead_mode == 2 && A.dead[(size_t)m * A.ntile + tile] != 0;
+    const bool dead_skip = A.dead_mode == 2 && dead_v != 0;
     if (wave_all(allskip) || dead_skip) {
         if (!acc)
             for (int n = m + lane; n <= nmax; n += 64) {
@@ -936,13 +985,9 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
     // degree m: P_00 = 1 or the sectorial seed P_mm (X-number)
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        if (m == 0) {
-            x[k] = skip[k] ? 0.0 : 1.0;
-            e[k] = 0;
-        } else {
-            x[k] = skip[k] ? 0.0 : A.sx[(size_t)(m - 1) * A.sld + ring[k]];
-            e[k] = skip[k] ? 0 : 960 * A.six[(size_t)(m - 1) * A.sld + ring[k]];
-        }
+        // P_00 = 1 at m = 0
+        x[k] = skip[k] ? 0.0 : (m == 0 ? 1.0 : sxs[k]);
+        e[k] = (skip[k] || m == 0) ? 0 : 960 * sxe[k];
         y[k] = 0.0;
         const double p0 = ldexp(x[k], e[k]);
         v0 = fma(p0, ampC[k], v0);
@@ -974,6 +1019,13 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
     // degrees m+1..nmax in tiles of 8; (n + m) is odd at every tile start
     cdouble *calc = (cdouble *)(A.al + dcol_off(A.al_nt, m) - m);
     int n = m + 1;
+    // a[] holds the alphas of the 8-degree tile at n.  Each tile requests the
+    // next tile's alphas before its arithmetic (the last whole tile requests
+    // its own again, which stays in bounds): a scalar load's wait is a wait
+    // for every LDS access issued before it as well, so a load made at the
+    // top of the tile would also make the reducer's reads complete there.
+    double a[8];
+    if (n + 7 <= nmax) load_alpha8_const(calc + n, a);
     // phase 1: 16-degree tiles (two reducer tiles each) while a lane of the
     // wave still carries an exponent; an 8-degree tile at the end of the
     // column, where 16 degrees do not fit (the plain phase has no ldexp)
@@ -994,8 +1046,9 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
         dead = dead && alldeep;
 #pragma unroll 1
         for (int h = 0; h < hlen; h += 8) {
-            double a[8];
-            load_alpha8_const(calc + n + h, a);
+            const int nx = n + h + 8;
+            double an[8];
+            load_alpha8_const(calc + (nx + 7 <= nmax ? nx : n + h), an);
             if (PIPE) rd.begin();
             if (alldeep) {
                 // every term of the tile is exactly zero
@@ -1033,6 +1086,8 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
                     rd.finish();
                 }
             }
+#pragma unroll
+            for (int j = 0; j < 8; j++) a[j] = an[j];
         }
         int ex[K], d[K];
         bool need = false;
@@ -1064,8 +1119,8 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
     if (A.dead_mode == 1 && lane == 0) A.dead[(size_t)m * A.ntile + tile] = dead ? 1 : 0;
     // phase 2: every lane in plain doubles
     for (; n + 7 <= nmax; n += 8) {
-        double a[8];
-        load_alpha8_const(calc + n, a);
+        double an[8];
+        load_alpha8_const(calc + (n + 15 <= nmax ? n + 8 : n), an);
         if (PIPE) rd.begin();
 #pragma unroll
         for (int j = 0; j < 8; j += 2) {
@@ -1090,6 +1145,8 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
             rd.begin();
             rd.finish();
         }
+#pragma unroll
+        for (int j = 0; j < 8; j++) a[j] = an[j];
     }
     // last, partial tile
     if (n <= nmax) {
@@ -1199,7 +1256,7 @@ __device__ void sha_finish_quad(const ShaArgs &A, int quad, int ngrp)
 }
 
 template <int K>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 3 ? 4 : (K <= 5 ? 3 : 2))))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K <= 4 ? 4 : (K <= 6 ? 3 : 2))))
 void sha_kernel(ShaArgs A)
 {
     __shared__ __align__(16) ShaSmem<K> sm;

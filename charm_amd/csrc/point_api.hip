@@ -598,16 +598,19 @@ struct DevGrid {
 // ---------------------------------------------------------------------------
 // Ring pairs per lane K of the grid kernels.
 //
-// By default a rule fitted on MI355X to the kernel times of every K at 18
-// degrees from 64 to 10800 on Gauss-Legendre grids (`python -m
-// charm_amd.tune`, which reruns that sweep): few rings leave the chip short
-// of waves, so small grids want few rings per lane; large grids amortise the
-// per-degree work of a wave (column staging, analysis's cross-lane reduction)
-// over more rings.  In ring pairs n: synthesis K = 1 up to 450, 2 up to 900,
-// 3 beyond; analysis K = 2 up to 450, 4 up to 800, 5 up to 1600, 6 up to
-// 1900, 8 beyond (K = 5 keeps 3 waves per SIMD, 162 VGPRs; K = 6 and 8 run 2
-// waves per SIMD, which pays from about nmax 3500 on: 8-10% faster than
-// K = 4 at nmax 5400 to 10800).
+// By default a rule fitted on MI355X to the kernel times of every K on
+// Gauss-Legendre grids of degree 64 to 10800 (`python -m charm_amd.tune`,
+// which reruns that sweep; analysis timed with its slot reduction): few
+// rings leave the chip short of waves, so small grids want few rings per
+// lane; large grids amortise the per-degree work of a wave (column staging,
+// analysis's cross-lane reduction) over more rings.  In ring pairs n:
+// synthesis K = 1 up to 450, 2 up to 900, 3 beyond; analysis K = 2 up to
+// 351, 3 up to 551, 5 up to 951, 6 beyond (K <= 4 run 4 waves per SIMD, K = 5
+// and 6 three, K = 8 two).  Above 1000 ring pairs K = 5, 6 and 8 are within
+// a few per cent of each other, the order set by how the rings fill the last
+// tile; K = 6 was fastest or within 4% at 13 of 16 sampled degrees (at nmax
+// 2160: 0.540 ms against 0.573 (K = 5) and 0.580 (K = 8)), 8.5% behind K = 5
+// at nmax 2500.
 //
 // With CHARM_AMD_TUNE=1 an autotuner refines the rule at run time: the first
 // large transform of a kind in a ring-count bucket (a quarter octave) times
@@ -625,10 +628,9 @@ thread_local bool t_no_tune = false;
 int fitted_k(int kind, size_t nring)
 {
     if (kind == 0) return nring <= 450 ? 1 : (nring <= 900 ? 2 : 3);
-    if (nring <= 450) return 2;
-    if (nring <= 800) return 4;
-    if (nring <= 1600) return 5;
-    return nring <= 1900 ? 6 : 8;
+    if (nring <= 351) return 2;
+    if (nring <= 551) return 3;
+    return nring <= 951 ? 5 : 6;
 }
 
 bool tuning_on()
@@ -1466,6 +1468,7 @@ void run_sha(const GridGeom &g, const DevGrid &d, const double *w_dev, double cw
     a.nmax = (int)nmax;
     a.nring = (int)g.nring;
     a.nlat = (int)g.mir_nlat;
+    a.nrow = (int)g.nlat;
     a.t = d.t;
     a.sx = d.sx;
     a.six = d.six;
@@ -2531,6 +2534,7 @@ charm_amd_batch *charm_amd_batch_create(const charm_amd_request *reqs, size_t nr
                 a.nmax = nmax;
                 a.nring = (int)g->g.nring;
                 a.nlat = (int)g->g.mir_nlat;
+                a.nrow = (int)g->g.nlat;
                 a.k = K;
                 a.ntile = ring_tiles(g->g.nring, K);
                 a.tpg = 1;

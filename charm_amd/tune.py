@@ -5,7 +5,9 @@ degrees and fits the K rule by ring count.
     python -m charm_amd.tune [nmax ...]
 
 For each degree it prints the kernel time (HIP events around the launches,
-fastest of two rounds) of synthesis at K = 1..3 and analysis at K = 2, 3, 4, 5, 6, 8, and
+fastest of two rounds) of synthesis at K = 1..3 and analysis at K = 2, 3, 4, 5, 6, 8
+(the analysis kernel plus the reduction of its partial slots, whose number
+depends on K), and
 at the end the fitted rule: for each kernel the ring-pair counts up to which
 each K is fastest.  The library's built-in rule (point_api.hip, fitted_k) was
 fitted this way on MI355X; CHARM_AMD_TUNE=1 refines it at run time per
@@ -19,16 +21,17 @@ import numpy as np
 
 DEFAULT_DEGREES = (64, 128, 200, 300, 450, 600, 800, 1000, 1300, 1700, 2160, 2700, 3500, 4500,
                    5400, 7000, 8200, 10800)
-KINDS = ((0, (1, 2, 3), "CHARM_AMD_SHS_K", "shs_grid_kernel"),
-         (1, (2, 3, 4, 5, 6, 8), "CHARM_AMD_SHA_K", "sha_kernel"))
+KINDS = ((0, (1, 2, 3), "CHARM_AMD_SHS_K", ("shs_grid_kernel",)),
+         (1, (2, 3, 4, 5, 6, 8), "CHARM_AMD_SHA_K", ("sha_kernel", "sha_reduce_kernel")))
 
 
-def _kernel_ms(lib, name):
+def _kernel_ms(lib, name, calls):
+    """Average time per transform of the named kernel (0 if it did not run)."""
     tot, cnt = _ct.c_double(), _ct.c_long()
     lib.charm_amd_prof_read.argtypes = [_ct.c_char_p, _ct.POINTER(_ct.c_double),
                                         _ct.POINTER(_ct.c_long)]
     lib.charm_amd_prof_read(name.encode(), _ct.byref(tot), _ct.byref(cnt))
-    return tot.value / max(cnt.value, 1)
+    return tot.value / calls
 
 
 def sweep(degrees=DEFAULT_DEGREES, out=sys.stdout):
@@ -55,7 +58,7 @@ def sweep(degrees=DEFAULT_DEGREES, out=sys.stdout):
             else:
                 g.sha(f, N, GM, R_EARTH, co, st)
 
-        for kind, ks, env, name in KINDS:
+        for kind, ks, env, names in KINDS:
             t = res.setdefault((kind, N), {})
             old = os.environ.get(env)
             try:
@@ -66,12 +69,13 @@ def sweep(degrees=DEFAULT_DEGREES, out=sys.stdout):
                         run(kind)
                         st.synchronize()
                         L.charm_amd_prof_reset()
-                        L.charm_amd_prof_enable(2)
+                        L.charm_amd_prof_enable(1)
                         for _ in range(reps):
                             run(kind)
                         st.synchronize()
                         L.charm_amd_prof_enable(0)
-                        t[k] = min(t.get(k, 1e30), _kernel_ms(L, name))
+                        ms = sum(_kernel_ms(L, nm, reps) for nm in names)
+                        t[k] = min(t.get(k, 1e30), ms)
             finally:
                 if old is None:
                     os.environ.pop(env, None)
