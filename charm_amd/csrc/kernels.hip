@@ -678,6 +678,12 @@ __device__ __forceinline__ void shs_tile(const ShsGridArgs &A, int tile, int qua
             return;
         }
     }
+    if constexpr (K >= 4) {
+        if (kt == 3) {
+            shs_grid_item<3, UNIT>(A, tile, rbase, quad, reinterpret_cast<ShsSmem<3> &>(sm));
+            return;
+        }
+    }
     shs_grid_item<K, UNIT>(A, tile, rbase, quad, sm);
 }
 
@@ -1852,6 +1858,7 @@ int launch_shs_grid(const ShsGridArgs &a, void *stream)
     switch (a.k) {
     case 1: shs_launch_k<1>(a, nblk, S(stream)); break;
     case 3: shs_launch_k<3>(a, nblk, S(stream)); break;
+    case 4: shs_launch_k<4>(a, nblk, S(stream)); break;
     default: shs_launch_k<2>(a, nblk, S(stream)); break;
     }
     return (int)hipGetLastError();

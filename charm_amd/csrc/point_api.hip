@@ -604,9 +604,11 @@ struct DevGrid {
 // rings leave the chip short of waves, so small grids want few rings per
 // lane; large grids amortise the per-degree work of a wave (column staging,
 // analysis's cross-lane reduction) over more rings.  In ring pairs n:
-// synthesis K = 1 up to 450, 2 up to 900, 3 beyond; analysis K = 2 up to
-// 351, 3 up to 551, 5 up to 951, 6 beyond (K <= 4 run 4 waves per SIMD, K = 5
-// and 6 three, K = 8 two).  Above 1000 ring pairs K = 5, 6 and 8 are within
+// synthesis K = 1 up to 450, 2 up to 900, 3 beyond (K = 4, 3 waves per
+// SIMD, is 0-2.5% faster from nmax 3500 on and a tuner candidate only);
+// analysis K = 2 up to 351, 3 up to 551, 5 up to 951, 6 beyond (K <= 4 run
+// 4 waves per SIMD, K = 5 and 6 three, K = 8 two).  Above 1000 ring pairs
+// K = 5, 6 and 8 are within
 // a few per cent of each other, the order set by how the rings fill the last
 // tile; K = 6 was fastest or within 4% at 13 of 16 sampled degrees (at nmax
 // 2160: 0.540 ms against 0.573 (K = 5) and 0.580 (K = 8)), 8.5% behind K = 5
@@ -640,7 +642,7 @@ bool tuning_on()
 }
 
 // Ring pairs per lane that each grid kernel is built for
-constexpr int SHS_KS[] = {1, 2, 3};
+constexpr int SHS_KS[] = {1, 2, 3, 4};
 constexpr int SHA_KS[] = {2, 3, 4, 5, 6, 8};
 
 bool valid_k(int kind, int k)

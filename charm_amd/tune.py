@@ -5,7 +5,7 @@ degrees and fits the K rule by ring count.
     python -m charm_amd.tune [nmax ...]
 
 For each degree it prints the kernel time (HIP events around the launches,
-fastest of two rounds) of synthesis at K = 1..3 and analysis at K = 2, 3, 4, 5, 6, 8
+fastest of two rounds) of synthesis at K = 1..4 and analysis at K = 2, 3, 4, 5, 6, 8
 (the analysis kernel plus the reduction of its partial slots, whose number
 depends on K), and
 at the end the fitted rule: for each kernel the ring-pair counts up to which
@@ -21,7 +21,7 @@ import numpy as np
 
 DEFAULT_DEGREES = (64, 128, 200, 300, 450, 600, 800, 1000, 1300, 1700, 2160, 2700, 3500, 4500,
                    5400, 7000, 8200, 10800)
-KINDS = ((0, (1, 2, 3), "CHARM_AMD_SHS_K", ("shs_grid_kernel",)),
+KINDS = ((0, (1, 2, 3, 4), "CHARM_AMD_SHS_K", ("shs_grid_kernel",)),
          (1, (2, 3, 4, 5, 6, 8), "CHARM_AMD_SHA_K", ("sha_kernel", "sha_reduce_kernel")))
 
 

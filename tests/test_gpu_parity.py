@@ -295,7 +295,7 @@ def test_device_grid_radius_differs_from_R():
     assert np.array_equal(f.numpy(), f_host)
 
 
-@pytest.mark.parametrize("K", (2, 3, 5, 6, 8))
+@pytest.mark.parametrize("K", (2, 3, 4, 5, 6, 8))
 def test_device_grid_skips_all_zero_waves(K, monkeypatch):
     """Repeated transforms on one device grid skip the waves (order, ring
     tile) whose every term the first transform found to be exactly zero.  At
@@ -306,7 +306,7 @@ def test_device_grid_skips_all_zero_waves(K, monkeypatch):
     (analysis only; synthesis keeps its rule): 801 ring pairs are two full
     tiles and one of 161 rings (K = 5), two full tiles and one of 33 rings
     (K = 6), or one full tile and one of 289 rings, run at 6 pairs per lane
-    (K = 8)."""
+    (K = 8); K = 4: three full tiles and one of 33 rings."""
     from charm_amd.device import DevArray, DeviceGrid, Stream
 
     monkeypatch.setenv("CHARM_AMD_SHS_K", str(K))
@@ -337,6 +337,47 @@ def test_device_grid_skips_all_zero_waves(K, monkeypatch):
             assert np.abs(o[:, 1] - S).max() <= TOL * np.abs(C).max()
         else:
             assert np.array_equal(o, first), "analysis call %d" % call
+
+
+def test_every_ring_pairs_per_lane(monkeypatch):
+    """Synthesis at every built K gives the same grid bit for bit (a ring's
+    sums do not depend on how rings are tiled), and analysis at every K
+    recovers the coefficients.  nmax = 810: 406 ring pairs, at K = 4 one full
+    tile and one of 150 rings, run at 3 pairs per lane; at K = 6 one tile of
+    384 and one of 22; at K = 8 one of 406, run at 8."""
+    from charm_amd.device import DevArray, DeviceGrid, Stream
+
+    N = 810
+    C, S = kaula_coeffs(N, 41)
+    st = Stream()
+    cs = DevArray.from_numpy(np.stack([C, S], axis=1))
+    dg = DeviceGrid(N, R_EARTH)
+    f = DevArray((dg.nlat, dg.nlon))
+    ref = None
+    for K in (1, 2, 3, 4):
+        monkeypatch.setenv("CHARM_AMD_SHS_K", str(K))
+        dg.shs(cs, N, GM, R_EARTH, f, st)
+        st.synchronize()
+        assert charm_amd.lib().charm_amd_last_ring_k(0) == K
+        if ref is None:
+            ref = f.numpy()
+        else:
+            assert np.array_equal(f.numpy(), ref), "synthesis K=%d" % K
+    out = DevArray(cs.shape)
+    first = None
+    for K in (2, 3, 4, 5, 6, 8):
+        monkeypatch.setenv("CHARM_AMD_SHA_K", str(K))
+        dg.sha(f, N, GM, R_EARTH, out, st)
+        st.synchronize()
+        assert charm_amd.lib().charm_amd_last_ring_k(1) == K
+        o = out.numpy()
+        assert np.abs(o[:, 0] - C).max() <= TOL * np.abs(C).max(), "analysis K=%d" % K
+        assert np.abs(o[:, 1] - S).max() <= TOL * np.abs(C).max(), "analysis K=%d" % K
+        if first is None:
+            first = o
+        else:
+            # the same sums, added across lanes and tiles in another order
+            assert np.abs(o - first).max() <= 1e-14 * np.abs(C).max(), "analysis K=%d" % K
 
 
 def test_device_grid_one_slot_rescale_roundtrip():
