@@ -177,28 +177,57 @@ __device__ __forceinline__ cplx wave_reduce(cplx v)
     return v;
 }
 
+// Rows are read and written by each thread in batches of IO_BATCH points:
+// the batch's global loads (inputs, index and twiddle tables) are all issued,
+// from indices clamped into the row, before any of them is waited for, so a
+// thread waits once per batch rather than once per point.
+constexpr int IO_BATCH = 4;
+
 // Loads the L inputs v(n) of a row's DFT into LDS in the order MODE needs
 // (Rader: point n >= 1 at slot(log_g n), the sum of all points per wave
-// beside the tables) and runs the transform.  get(n) gives input n.
+// beside the tables) and runs the transform.  get(n) gives input n, for any
+// 0 <= n < L.
 template <int MODE, class Get>
 __device__ __forceinline__ void row_dft(const RowArgs &a, cplx *x, const fft::TwSplit &tw, Get get)
 {
     const fft::MixPlan &p = a.plan;
     const int L = a.L, M = p.M;
+    const int bd = blockDim.x;
     if (MODE == fft::DFT_RADER) {
         cplx acc = fft::mk(0.0, 0.0);
-        for (int n = threadIdx.x; n < L; n += blockDim.x) {
-            const cplx v = get(n);
-            acc = fft::add(acc, v);
-            if (n >= 1) x[fft::slot(a.lg[n])] = v;
+        for (int n0 = threadIdx.x; n0 < L; n0 += IO_BATCH * bd) {
+            cplx v[IO_BATCH];
+            int q[IO_BATCH];
+#pragma unroll
+            for (int j = 0; j < IO_BATCH; j++) {
+                const int n = min(n0 + j * bd, L - 1);
+                v[j] = get(n);
+                q[j] = a.lg[n];
+            }
+#pragma unroll
+            for (int j = 0; j < IO_BATCH; j++) {
+                const int n = n0 + j * bd;
+                if (n < L) {
+                    acc = fft::add(acc, v[j]);
+                    if (n >= 1) x[fft::slot(q[j])] = v[j];
+                }
+            }
         }
         acc = wave_reduce(acc);
         if ((threadIdx.x & 63) == 0) wave_sums(M)[threadIdx.x >> 6] = acc;
     } else {
-        for (int n = threadIdx.x; n < M; n += blockDim.x) {
-            cplx v = fft::mk(0.0, 0.0);
-            if (n < L) v = (MODE == fft::DFT_BLUESTEIN) ? fft::mul(get(n), a.w[n]) : get(n);
-            x[fft::slot(n)] = v;
+        for (int n0 = threadIdx.x; n0 < M; n0 += IO_BATCH * bd) {
+            cplx v[IO_BATCH];
+#pragma unroll
+            for (int j = 0; j < IO_BATCH; j++) {
+                const int n = min(n0 + j * bd, L - 1);
+                v[j] = (MODE == fft::DFT_BLUESTEIN) ? fft::mul(get(n), a.w[n]) : get(n);
+            }
+#pragma unroll
+            for (int j = 0; j < IO_BATCH; j++) {
+                const int n = n0 + j * bd;
+                if (n < M) x[fft::slot(n)] = n < L ? v[j] : fft::mk(0.0, 0.0);
+            }
         }
     }
     __syncthreads();
@@ -208,14 +237,39 @@ __device__ __forceinline__ void row_dft(const RowArgs &a, cplx *x, const fft::Tw
         convolve(x, a.B, tw, p);
 }
 
-// Output k of the row's DFT after row_dft (v0 = input 0, used by Rader)
+// Output k of the row's DFT after row_dft, in two steps: row_out_at reads
+// what output k needs from the global tables (its LDS slot; Bluestein's
+// chirp factor), row_out the LDS (v0 = input 0, used by Rader).
+struct OutAt {
+    int s;
+    cplx w;
+};
+
 template <int MODE>
-__device__ __forceinline__ cplx row_out(const RowArgs &a, const cplx *x, int k, cplx v0)
+__device__ __forceinline__ OutAt row_out_at(const RowArgs &a, int k)
 {
     const int M = a.plan.M;
-    if (MODE == fft::DFT_DIRECT) return x[fft::slot(a.rev[k])];
-    if (MODE == fft::DFT_BLUESTEIN)
-        return fft::scale(fft::mul(a.w[k], x[fft::slot(k)]), 1.0 / (double)M);
+    OutAt o;
+    o.w = fft::mk(0.0, 0.0);
+    if (MODE == fft::DFT_DIRECT) {
+        o.s = fft::slot(a.rev[k]);
+    } else if (MODE == fft::DFT_BLUESTEIN) {
+        o.s = fft::slot(k);
+        o.w = a.w[k];
+    } else {
+        const int q = a.lg[k];
+        o.s = fft::slot(q == 0 ? 0 : M - q);
+    }
+    return o;
+}
+
+template <int MODE>
+__device__ __forceinline__ cplx row_out(const RowArgs &a, const cplx *x, int k, const OutAt &o,
+                                        cplx v0)
+{
+    const int M = a.plan.M;
+    if (MODE == fft::DFT_DIRECT) return x[o.s];
+    if (MODE == fft::DFT_BLUESTEIN) return fft::scale(fft::mul(o.w, x[o.s]), 1.0 / (double)M);
     // Rader: Z_0 = sum of the inputs; Z_{g^-p} = v0 + conv_p / M
     if (k == 0) {
         const cplx *ws = wave_sums(M);
@@ -224,8 +278,7 @@ __device__ __forceinline__ cplx row_out(const RowArgs &a, const cplx *x, int k, 
         for (int i = 0; i < nw; i++) s = fft::add(s, ws[i]);
         return s;
     }
-    const int q = a.lg[k];
-    return fft::axpy(1.0 / (double)M, x[fft::slot(q == 0 ? 0 : M - q)], v0);
+    return fft::axpy(1.0 / (double)M, x[o.s], v0);
 }
 
 // f (2L reals per row) -> spec (L+1 complex per row), F_k = sum_j f_j e^{-2 pi i jk/2L}
@@ -235,19 +288,35 @@ __device__ __forceinline__ void r2c_row(const RowArgs &a, size_t row)
     cplx *x = lds_rows;
     const int L = a.L;
     const cplx *z = (const cplx *)(a.f + row * a.ldf);
+    const cplx z0 = (MODE == fft::DFT_RADER) ? z[0] : fft::mk(0.0, 0.0);
     const fft::TwSplit tw = lds_twiddles(a.tw, a.plan.M);
     row_dft<MODE>(a, x, tw, [&](int n) { return z[n]; });
-    const cplx z0 = (MODE == fft::DFT_RADER) ? z[0] : fft::mk(0.0, 0.0);
     // Z_k; F_k = E_k + e^{-i pi k/L} O_k with E_k = (Z_k + conj Z_{L-k}) / 2,
     // O_k = (Z_k - conj Z_{L-k}) / (2i)
     cplx *out = a.spec + row * a.lds;
-    for (int k = threadIdx.x; k <= L; k += blockDim.x) {
-        const int kk = (k == L) ? 0 : k, kr = (k == 0) ? 0 : L - k;
-        const cplx zk = row_out<MODE>(a, x, kk, z0);
-        const cplx zr = fft::conj(row_out<MODE>(a, x, kr, z0));
-        const cplx e = fft::scale(fft::add(zk, zr), 0.5);
-        const cplx o = fft::rot90<-1>(fft::scale(fft::sub(zk, zr), 0.5));
-        out[k] = fft::add(e, fft::mul(a.pt[k], o));
+    const int bd = blockDim.x;
+    for (int k0 = threadIdx.x; k0 <= L; k0 += IO_BATCH * bd) {
+        OutAt ok[IO_BATCH], orr[IO_BATCH];
+        cplx pt[IO_BATCH];
+#pragma unroll
+        for (int j = 0; j < IO_BATCH; j++) {
+            const int k = min(k0 + j * bd, L);
+            const int kk = (k == L) ? 0 : k, kr = (k == 0) ? 0 : L - k;
+            ok[j] = row_out_at<MODE>(a, kk);
+            orr[j] = row_out_at<MODE>(a, kr);
+            pt[j] = a.pt[k];
+        }
+#pragma unroll
+        for (int j = 0; j < IO_BATCH; j++) {
+            const int k = k0 + j * bd;
+            if (k > L) continue;
+            const int kk = (k == L) ? 0 : k, kr = (k == 0) ? 0 : L - k;
+            const cplx zk = row_out<MODE>(a, x, kk, ok[j], z0);
+            const cplx zr = fft::conj(row_out<MODE>(a, x, kr, orr[j], z0));
+            const cplx e = fft::scale(fft::add(zk, zr), 0.5);
+            const cplx o = fft::rot90<-1>(fft::scale(fft::sub(zk, zr), 0.5));
+            out[k] = fft::add(e, fft::mul(pt[j], o));
+        }
     }
 }
 
@@ -282,14 +351,23 @@ __device__ __forceinline__ void c2r_row(const RowArgs &a, size_t row)
     cplx *x = lds_rows;
     const int L = a.L;
     const cplx *X = a.spec + row * a.lds;
+    const cplx v0 = (MODE == fft::DFT_RADER) ? c2r_input(a, X, 0) : fft::mk(0.0, 0.0);
     const fft::TwSplit tw = lds_twiddles(a.tw, a.plan.M);
     row_dft<MODE>(a, x, tw, [&](int n) { return c2r_input(a, X, n); });
-    const cplx v0 = (MODE == fft::DFT_RADER) ? c2r_input(a, X, 0) : fft::mk(0.0, 0.0);
     // z_n = conj(DFT_n); f_2n = 2 Re z_n, f_2n+1 = 2 Im z_n
     cplx *f = (cplx *)(a.f + row * a.ldf);
-    for (int n = threadIdx.x; n < L; n += blockDim.x) {
-        const cplx y = row_out<MODE>(a, x, n, v0);
-        f[n] = fft::mk(2.0 * y.x, -2.0 * y.y);
+    const int bd = blockDim.x;
+    for (int n0 = threadIdx.x; n0 < L; n0 += IO_BATCH * bd) {
+        OutAt o[IO_BATCH];
+#pragma unroll
+        for (int j = 0; j < IO_BATCH; j++) o[j] = row_out_at<MODE>(a, min(n0 + j * bd, L - 1));
+#pragma unroll
+        for (int j = 0; j < IO_BATCH; j++) {
+            const int n = n0 + j * bd;
+            if (n >= L) continue;
+            const cplx y = row_out<MODE>(a, x, n, o[j], v0);
+            f[n] = fft::mk(2.0 * y.x, -2.0 * y.y);
+        }
     }
 }
 
@@ -442,12 +520,24 @@ __global__ __launch_bounds__(1024) void split_a_kernel(SplitArgs a)
         const int n = n1 + L1 * n2;
         return C2R ? c2r_input_full(L, a.ptf, X, n) : z[n];
     };
-    row_dft<MODE>(a.sub, x, tw, get);
     const cplx v0 = (MODE == fft::DFT_RADER) ? get(0) : fft::mk(0.0, 0.0);
+    row_dft<MODE>(a.sub, x, tw, get);
     cplx *y = a.Y + (row * L1 + n1) * (size_t)L2;
-    for (int k2 = threadIdx.x; k2 < L2; k2 += blockDim.x) {
-        const int j = (int)(((long)n1 * k2) % L);
-        y[k2] = fft::mul(row_out<MODE>(a.sub, x, k2, v0), a.twf[j]);
+    const int bd = blockDim.x;
+    for (int k0 = threadIdx.x; k0 < L2; k0 += IO_BATCH * bd) {
+        OutAt o[IO_BATCH];
+        cplx w[IO_BATCH];
+#pragma unroll
+        for (int i = 0; i < IO_BATCH; i++) {
+            const int k2 = min(k0 + i * bd, L2 - 1);
+            o[i] = row_out_at<MODE>(a.sub, k2);
+            w[i] = a.twf[(int)(((long)n1 * k2) % L)];
+        }
+#pragma unroll
+        for (int i = 0; i < IO_BATCH; i++) {
+            const int k2 = k0 + i * bd;
+            if (k2 < L2) y[k2] = fft::mul(row_out<MODE>(a.sub, x, k2, o[i], v0), w[i]);
+        }
     }
 }
 
