@@ -1308,6 +1308,26 @@ __global__ __launch_bounds__(256) void sha_batch_kernel(const ShaArgs *__restric
 // Sum the tile partials in tile order, apply s_nm and scale[n] (if given),
 // and write the coefficients of a model of degree nmax_out (zeros above nmax)
 // (src/sha/sha_point.c:381,1088-1108; src/shc/shc_rescale.c:24-90).
+// Sum of the partial slots of one (n, m) in slot order: slots are read four
+// at a time from clamped indices, all four loads issued before the adds.
+__device__ __forceinline__ double2 slot_sum(const double2 *__restrict__ part, size_t P, size_t q,
+                                            int nslot)
+{
+    double vc = 0.0, vs = 0.0;
+    for (int t0 = 0; t0 < nslot; t0 += 4) {
+        double2 p[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) p[j] = part[(size_t)min(t0 + j, nslot - 1) * P + q];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (t0 + j < nslot) {
+                vc += p[j].x;
+                vs += p[j].y;
+            }
+    }
+    return make_double2(vc, vs);
+}
+
 __global__ void sha_reduce_kernel(int nmax, int ntile, const double2 *__restrict__ part,
                                   const double *__restrict__ sn, int nt, int nmax_out,
                                   const double *__restrict__ scale, double *__restrict__ c,
@@ -1321,17 +1341,14 @@ __global__ void sha_reduce_kernel(int nmax, int ntile, const double2 *__restrict
         double vc = 0.0, vs = 0.0;
         if (m <= nmax && n <= nmax) {
             const size_t q = dcol_off(nmax, m) + (n - m);
-            for (int tl = 0; tl < ntile; tl++) {
-                const double2 p = part[(size_t)tl * P + q];
-                vc += p.x;
-                vs += p.y;
-            }
             const double f = sn[dcol_off(nt, m) + (n - m)];
-            vc *= f;
-            vs *= f;
+            const double g = scale ? scale[n] : 1.0;
+            const double2 v = slot_sum(part, P, q, ntile);
+            vc = v.x * f;
+            vs = v.y * f;
             if (scale) {
-                vc *= scale[n];
-                vs *= scale[n];
+                vc *= g;
+                vs *= g;
             }
             if (m == 0) vs = 0.0;
         }
@@ -1353,18 +1370,13 @@ __global__ void sha_reduce_group_kernel(const ShaReduceJob *__restrict__ jobs,
     const double2 *part = (const double2 *)q.part;
     for (int n = m + blockIdx.x * blockDim.x + threadIdx.x; n <= nmax;
          n += gridDim.x * blockDim.x) {
-        double vc = 0.0, vs = 0.0;
-        for (int tl = 0; tl < q.ntile; tl++) {
-            const double2 p = part[(size_t)tl * P + col + n];
-            vc += p.x;
-            vs += p.y;
-        }
         const double f = sn[dcol_off(nt, m) + (n - m)];
-        vc *= f;
-        vs *= f;
+        const double g = q.scale ? q.scale[n] : 1.0;
+        const double2 v = slot_sum(part, P, col + n, q.ntile);
+        double vc = v.x * f, vs = v.y * f;
         if (q.scale) {
-            vc *= q.scale[n];
-            vs *= q.scale[n];
+            vc *= g;
+            vs *= g;
         }
         if (m == 0) vs = 0.0;
         q.cs[2 * (col + n)] = vc;
