@@ -49,11 +49,13 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6   # MI355X vector FP64 (also its FP64 matrix peak)
 UNIT = "(nmax+1)^2*nlat/s"
 # HBM bytes per launch at nmax = 2160 for the ring pairs per lane K the
-# built-in rule picks (synthesis 3, analysis 5), measured with rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE in separate runs and scaled by a 256 MiB
-# device copy measured in the same runs (HBM bytes = 2.0 FETCH_SIZE + 1.0
-# WRITE_SIZE).  Analysis writes one partial triangle per ring-tile slot.
-TRAFFIC_GL2160 = {("sha_kernel", 5): 367.3e6, ("shs_grid_kernel", 3): 189.4e6}
+# built-in rule picks (synthesis 3, analysis 6; analysis at K = 5 from an
+# earlier run), measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in
+# separate runs and scaled by a 256 MiB device copy measured in the same runs
+# (HBM bytes = 2.0 FETCH_SIZE + 1.0 WRITE_SIZE).  Analysis writes one partial
+# triangle per ring-tile slot.
+TRAFFIC_GL2160 = {("sha_kernel", 6): 321.3e6, ("sha_kernel", 5): 367.3e6,
+                  ("shs_grid_kernel", 3): 190.0e6}
 
 
 def last_ring_k(lib, name):
