@@ -873,7 +873,6 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
     // order m, the polar optimisation's first dropped order) and its
     // all-zero flag, requested first and from indices clamped into the
     // arrays, so that these loads and the staging's overlap
-    const bool col = m <= nmax;  // wave-uniform
     int ring[K];
     double t[K], sxs[K];
     int sxe[K], pm[K];

@@ -158,8 +158,11 @@ void keep_pool_memory()
 // allocations queued behind a deep stream).  A cached buffer carries an event
 // recorded after its last use; the next user waits for it on its own stream,
 // so reuse is ordered even across streams (or a recycled stream handle).
-// Calls under stream capture bypass the cache.  Cached bytes are capped; the
-// oldest buffers go first.
+// Cached bytes are capped; the oldest buffers go first.  No buffer is made
+// under stream capture: it would become a graph memory-allocation node, and
+// the library records no such nodes (the device-resident entry points refuse
+// a capturing stream before they enqueue anything; graphs come from the
+// batch API, whose buffers are allocated when the batch is created).
 struct CachedBuf {
     void *p;
     size_t bytes;
@@ -176,6 +179,18 @@ bool capturing(hipStream_t st)
 {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     return hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone;
+}
+
+// Entry points whose work needs per-call scratch: false (and err set) when
+// the stream is capturing, before anything is enqueued.
+bool capture_refused(hipStream_t st, charm_err *err, const char *func)
+{
+    if (!capturing(st)) return false;
+    err_set(err, __FILE__, (size_t)__LINE__, func, CHARM_EFUNCARG,
+            "This device entry point allocates per-call scratch and cannot be recorded under "
+            "stream capture: use charm_amd_batch_* (graphs recorded by the library) or "
+            "charm_amd_shs_sctr_dev_noalloc.");
+    return true;
 }
 
 // with g_sc_mu held
@@ -209,8 +224,10 @@ struct DevBuf {
     {
         s = st;
         if (!bytes) return;
-        const bool cache = !capturing(st);
-        if (cache) {
+        if (capturing(st))
+            throw HipFail{hipErrorStreamCaptureUnsupported, "scratch allocation under stream capture",
+                          __FILE__, __LINE__};
+        {
             std::lock_guard<std::mutex> lk(g_sc_mu);
             auto sit = g_sc.find(st);
             if (sit != g_sc.end()) {
@@ -232,7 +249,7 @@ struct DevBuf {
         keep_pool_memory();
         cap = bytes;
         HIPCHK(hipMallocAsync(&p, bytes, st));
-        if (cache && hipEventCreateWithFlags(&done, hipEventDisableTiming) != hipSuccess) done = nullptr;
+        if (hipEventCreateWithFlags(&done, hipEventDisableTiming) != hipSuccess) done = nullptr;
     }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -2048,6 +2065,7 @@ void charm_amd_shs_grid_dev(const charm_amd_grid *grd, unsigned long nmax, doubl
         CHARM_AMD_ERR(err, CHARM_EFUNCARG, "\"nmax\" exceeds the degree of the grid.");
         return;
     }
+    if (capture_refused((hipStream_t)stream, err, __func__)) return;
     try {
         hipStream_t s = (hipStream_t)stream;
         Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
@@ -2081,6 +2099,7 @@ void charm_amd_sha_grid_dev(const charm_amd_grid *grd, unsigned long nmax, doubl
         CHARM_AMD_ERR(err, CHARM_EFUNCARG, "\"nmax\" exceeds the degree of the grid.");
         return;
     }
+    if (capture_refused((hipStream_t)stream, err, __func__)) return;
     try {
         hipStream_t s = (hipStream_t)stream;
         Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
@@ -2113,6 +2132,7 @@ void charm_amd_shs_grid_multi_dev(const charm_amd_grid *grd, unsigned long nmax,
         return;
     }
     if (nset == 0) return;
+    if (capture_refused((hipStream_t)stream, err, __func__)) return;
     try {
         hipStream_t s = (hipStream_t)stream;
         Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
@@ -2175,6 +2195,7 @@ void charm_amd_shs_sctr_dev(size_t npoint, const double *lat_dev, const double *
                             const double *cs_dev, double *f_dev, void *stream, charm_err *err)
 {
     if (npoint == 0) return;
+    if (capture_refused((hipStream_t)stream, err, __func__)) return;
     try {
         hipStream_t s = (hipStream_t)stream;
         Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
@@ -2206,6 +2227,7 @@ void charm_amd_shs_grid_deriv_dev(const charm_amd_grid *grd, unsigned long nmax,
         CHARM_AMD_ERR(err, CHARM_EFUNCARG, "\"nmax\" exceeds the degree of the grid.");
         return;
     }
+    if (capture_refused((hipStream_t)stream, err, __func__)) return;
     try {
         hipStream_t s = (hipStream_t)stream;
         Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
@@ -2250,6 +2272,7 @@ void charm_amd_shs_sctr_deriv_dev(size_t npoint, const double *lat_dev, const do
         return;
     }
     if (npoint == 0) return;
+    if (capture_refused((hipStream_t)stream, err, __func__)) return;
     try {
         hipStream_t s = (hipStream_t)stream;
         Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);

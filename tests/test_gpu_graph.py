@@ -1,15 +1,16 @@
-"""The device-resident entry points under HIP stream capture: a grid
-synthesis and analysis (at the grid radius and at another radius, so that
-the radius powers and the rescale table are made inside the graph), a
-scattered synthesis and a gradient are captured once and replayed; every
-replay, with unrelated eager work and synchronisations in between, must
-reproduce the eager results bit for bit.  Nothing a captured call records
-may point at host memory that is gone by the time of the replay."""
+"""The device-resident entry points under HIP stream capture.
+
+They take per-call scratch from the stream-ordered allocator, which under
+capture would become graph memory-allocation nodes.  A replay of such a graph
+left the GPU faulted on MI355X, with no mistake found in the library that
+explains it, so the entry points refuse a capturing stream: the refusal is
+tested here, and the replay test is kept but skipped."""
 import ctypes as ct
 
 import numpy as np
 import pytest
 
+from charm_amd._err import CHARM_EFUNCARG
 from charm_amd.device import (DERIV_GRAD1, DevArray, DeviceGrid, Stream, hip, shs_sctr)
 from charm_amd.workloads import GM, R_EARTH, kaula_coeffs, scattered_points
 
@@ -29,6 +30,57 @@ def _graph_api():
     return h
 
 
+def _device_calls(grid, cs, pts, outs, N, st):
+    f1, f2, c1, fs, gr = outs
+    return [
+        lambda: grid.shs(cs, N, GM, R_EARTH + 2000.0, f1, st),
+        lambda: grid.shs(cs, N, GM, R_EARTH, f2, st),
+        lambda: grid.sha(f2, N, GM, R_EARTH, c1, st),
+        lambda: shs_sctr(*pts, cs, N, GM, R_EARTH, fs, st),
+        lambda: grid.shs_deriv(cs, N, GM, R_EARTH, DERIV_GRAD1, gr, st),
+    ]
+
+
+def test_device_calls_refuse_stream_capture():
+    """Each device-resident call on a capturing stream returns CHarm's
+    CHARM_EFUNCARG before it enqueues anything: the captured graph is empty."""
+    from charm_amd._err import CHarmError
+
+    N = 40
+    C, S = kaula_coeffs(N, 61)
+    grid = DeviceGrid(N, R_EARTH + 2000.0)
+    st = Stream()
+    cs = DevArray.from_numpy(np.stack([C, S], axis=1))
+    lat, lon, r = scattered_points(500, 62)
+    pts = [DevArray.from_numpy(a) for a in (lat, lon, r)]
+    outs = (DevArray((grid.nlat, grid.nlon)), DevArray((grid.nlat, grid.nlon)), DevArray(cs.shape),
+            DevArray(lat.size), DevArray((3, grid.nlat, grid.nlon)))
+    h = _graph_api()
+    h.hipGraphGetNodes.argtypes = [_vp, _vp, ct.POINTER(ct.c_size_t)]
+    assert h.hipStreamBeginCapture(st.handle, 2) == 0
+    errors = []
+    try:
+        for call in _device_calls(grid, cs, pts, outs, N, st):
+            try:
+                call()
+            except CHarmError as e:
+                errors.append((e.code, str(e)))
+    finally:
+        graph = _vp()
+        rc = h.hipStreamEndCapture(st.handle, ct.byref(graph))
+    assert rc == 0
+    try:
+        n = ct.c_size_t(0)
+        assert h.hipGraphGetNodes(graph, None, ct.byref(n)) == 0
+        assert n.value == 0
+    finally:
+        h.hipGraphDestroy(graph)
+    assert len(errors) == 5
+    assert all(code == CHARM_EFUNCARG and "stream capture" in msg for code, msg in errors)
+
+
+@pytest.mark.skip(reason="a replay of captured device calls (graph memory-allocation nodes) "
+                         "left the GPU faulted on MI355X; the entry points now refuse capture")
 def test_device_calls_replay_from_a_graph():
     N = 300
     C, S = kaula_coeffs(N, 61)
