@@ -94,6 +94,9 @@ __device__ __forceinline__ bool wave_all(bool p)
 //    for the next 16 degrees, so every term of the tile is exactly zero.
 constexpr int XS_PLAIN = -800;
 constexpr int XS_DEEP = -1250;
+// Deep threshold of the analysis, whose checks are 32 degrees apart (a
+// column grows by less than 2^288 over them): values stay below 2^-1100.
+constexpr int XA_DEEP = -1400;
 
 __device__ __forceinline__ int xs_exp(double x, double y)
 {
@@ -846,6 +849,27 @@ struct ShaSmem {
     double buf[ShaLds<K>::SIZE];
 };
 
+// The folded amplitudes of ring ic at order m, as the staging of sha_item
+// forms them (same products and sums, so the same doubles): weighted spectrum
+// of the northern row (cw Re F, -cw Im F) plus and minus that of its mirror
+// (zero without one).
+__device__ __forceinline__ void sha_amps(const ShaArgs &A, int ic, int m, double &pC, double &mC,
+                                         double &pS, double &mS)
+{
+    const double2 *spec = (const double2 *)A.spec;
+    const int rn = ic, rs = min(A.nlat - 1 - ic, A.nrow - 1);
+    const double2 FN = spec[(size_t)rn * A.lds + m];
+    const double2 FS = spec[(size_t)rs * A.lds + m];
+    const double cn = A.cws * A.cw[rn], cs = A.cws * A.cw[rs];
+    const bool mir = A.mirror[ic] != 0;
+    const double nx = cn * FN.x, ny = -(cn * FN.y);
+    const double sx = mir ? cs * FS.x : 0.0, sy = mir ? -(cs * FS.y) : 0.0;
+    pC = nx + sx;
+    mC = nx - sx;
+    pS = ny + sy;
+    mS = ny - sy;
+}
+
 // The ring tile starting at ring `rbase`, order quad `quad`; its sums go to
 // partial slot `slot`,
 // written (acc false: the slot's first tile) or added to (acc true).
@@ -987,27 +1011,42 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
     int e[K];
     double v0 = 0.0, v1 = 0.0;
 
-    // degree m: P_00 = 1 or the sectorial seed P_mm (X-number)
+    // degree m: P_00 = 1 or the sectorial seed P_mm (X-number).  A ring whose
+    // column is in the scaled form (e != 0) has values below 2^XS_PLAIN =
+    // 2^-800 at the last check: its terms are left out (its amplitudes are
+    // zero until it leaves the scaled form, when they are made again from the
+    // spectrum), so that scaled and plain rings run the same arithmetic and
+    // no term needs an ldexp.  What is left out is below 2^-500 times the
+    // amplitudes (checks are 32 degrees apart, which grow a column by less
+    // than 2^288), far below the rounding of any coefficient.
 #pragma unroll
     for (int k = 0; k < K; k++) {
         // P_00 = 1 at m = 0
         x[k] = skip[k] ? 0.0 : (m == 0 ? 1.0 : sxs[k]);
         e[k] = (skip[k] || m == 0) ? 0 : 960 * sxe[k];
         y[k] = 0.0;
-        const double p0 = ldexp(x[k], e[k]);
-        v0 = fma(p0, ampC[k], v0);
-        v1 = fma(p0, ampS[k], v1);
+        if (e[k] != 0) ampC[k] = ammC[k] = ampS[k] = ammS[k] = 0.0;
+        v0 = fma(x[k], ampC[k], v0);
+        v1 = fma(x[k], ampS[k], v1);
     }
-    bool dk[K], dead = true;
+    // The wave's state, made at the seed and every 32 degrees while a ring is
+    // scaled: xph (some ring scaled), alldeep (every ring deep: its values
+    // stay below 2^-1100 for the next 32 degrees, so every term is zero and
+    // the tiles run the recurrence only), dead (deep from the seed on: the
+    // whole column is zero).
+    bool xph, alldeep, dead;
+    {
+        bool anyx = false, deep = true;
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        const int ex = xs_exp(x[k], y[k]);
-        dk[k] = !valid[k] || (e[k] != 0 && e[k] + ex < XS_DEEP);
-        dead = dead && dk[k];
+        for (int k = 0; k < K; k++) {
+            const int ex = xs_exp(x[k], y[k]);
+            deep = deep && (!valid[k] || (e[k] != 0 && e[k] + ex < XA_DEEP));
+            anyx = anyx || e[k] != 0;
+        }
+        xph = wave_any(anyx);
+        alldeep = wave_all(deep);
+        dead = alldeep;
     }
-    // every term of the column is exactly zero while the wave stays deep from
-    // the seed to the last degree
-    dead = wave_all(dead);
     rd.put(0, v0, v1);
     rd.commit(1, dst + 2 * m);
     if (!PIPE) {
@@ -1031,128 +1070,100 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
     // top of the tile would also make the reducer's reads complete there.
     double a[8];
     if (n + 7 <= nmax) load_alpha8_const(calc + n, a);
-    // phase 1: 16-degree tiles (two reducer tiles each) while a lane of the
-    // wave still carries an exponent; an 8-degree tile at the end of the
-    // column, where 16 degrees do not fit (the plain phase has no ldexp)
-    int hlen = 16;
-    for (; n + 7 <= nmax; n += hlen) {
-        hlen = (n + 15 <= nmax) ? 16 : 8;
-        bool anyx = false, deep = true;
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            anyx = anyx || (e[k] != 0);
-            deep = deep && dk[k];
-        }
-        if (!wave_any(anyx)) {
-            dead = false;
-            break;
-        }
-        const bool alldeep = wave_all(deep);
-        dead = dead && alldeep;
-#pragma unroll 1
-        for (int h = 0; h < hlen; h += 8) {
-            const int nx = n + h + 8;
-            double an[8];
-            load_alpha8_const(calc + (nx + 7 <= nmax ? nx : n + h), an);
-            if (PIPE) rd.begin();
-            if (alldeep) {
-                // every term of the tile is exactly zero
-#pragma unroll
-                for (int j = 0; j < 8; j++) {
-#pragma unroll
-                    for (int k = 0; k < K; k++) {
-                        const double z = fma(a[j] * x[k], t[k], -y[k]);
-                        y[k] = x[k];
-                        x[k] = z;
-                    }
-                }
-                if (PIPE) rd.finish();
-                if (lane < 16 && !acc) dst[2 * (n + h) + lane] = 0.0;
-            } else {
-#pragma unroll
-                for (int j = 0; j < 8; j++) {
-                    const double al = a[j];
-                    double c = 0.0, sv = 0.0;
-#pragma unroll
-                    for (int k = 0; k < K; k++) {
-                        const double z = fma(al * x[k], t[k], -y[k]);
-                        y[k] = x[k];
-                        x[k] = z;
-                        const double p = ldexp(z, e[k]);
-                        c = fma(p, (j & 1) ? ampC[k] : ammC[k], c);
-                        sv = fma(p, (j & 1) ? ampS[k] : ammS[k], sv);
-                    }
-                    rd.put(j, c, sv);
-                }
-                if (PIPE) rd.finish();
-                rd.commit(8, dst + 2 * (n + h));
-                if (!PIPE) {
-                    rd.begin();
-                    rd.finish();
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < 8; j++) a[j] = an[j];
-        }
-        int ex[K], d[K];
-        bool need = false;
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            ex[k] = xs_exp(x[k], y[k]);
-            d[k] = e[k] + ex[k];
-            need = need || xs_need(e[k], ex[k]);
-        }
-        if (wave_any(need)) {
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                const int sh = xs_shift(e[k], ex[k]);
-                x[k] = ldexp(x[k], sh);
-                y[k] = ldexp(y[k], sh);
-                e[k] -= sh;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < K; k++) dk[k] = !valid[k] || (e[k] != 0 && d[k] < XS_DEEP);
-    }
-    // the last (fewer than 8) degrees: zero terms if still deep
-    {
-        bool deep = true;
-#pragma unroll
-        for (int k = 0; k < K; k++) deep = deep && dk[k];
-        dead = dead && wave_all(deep);
-    }
-    if (A.dead_mode == 1 && lane == 0) A.dead[(size_t)m * A.ntile + tile] = dead ? 1 : 0;
-    // phase 2: every lane in plain doubles
+    int win = 0;  // tiles since the last check
     for (; n + 7 <= nmax; n += 8) {
         double an[8];
         load_alpha8_const(calc + (n + 15 <= nmax ? n + 8 : n), an);
         if (PIPE) rd.begin();
+        if (alldeep) {
+            // every term of the tile is exactly zero
 #pragma unroll
-        for (int j = 0; j < 8; j += 2) {
-            double c0 = 0.0, s0 = 0.0, c1 = 0.0, s1 = 0.0;
+            for (int j = 0; j < 8; j++) {
 #pragma unroll
-            for (int k = 0; k < K; k++) {
-                const double z = fma(a[j] * x[k], t[k], -y[k]);
-                c0 = fma(z, ammC[k], c0);
-                s0 = fma(z, ammS[k], s0);
-                const double w = fma(a[j + 1] * z, t[k], -x[k]);
-                c1 = fma(w, ampC[k], c1);
-                s1 = fma(w, ampS[k], s1);
-                y[k] = z;
-                x[k] = w;
+                for (int k = 0; k < K; k++) {
+                    const double z = fma(a[j] * x[k], t[k], -y[k]);
+                    y[k] = x[k];
+                    x[k] = z;
+                }
             }
-            rd.put(j, c0, s0);
-            rd.put(j + 1, c1, s1);
-        }
-        if (PIPE) rd.finish();
-        rd.commit(8, dst + 2 * n);
-        if (!PIPE) {
-            rd.begin();
-            rd.finish();
+            if (PIPE) rd.finish();
+            if (lane < 16 && !acc) dst[2 * n + lane] = 0.0;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; j += 2) {
+                double c0 = 0.0, s0 = 0.0, c1 = 0.0, s1 = 0.0;
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const double z = fma(a[j] * x[k], t[k], -y[k]);
+                    c0 = fma(z, ammC[k], c0);
+                    s0 = fma(z, ammS[k], s0);
+                    const double w = fma(a[j + 1] * z, t[k], -x[k]);
+                    c1 = fma(w, ampC[k], c1);
+                    s1 = fma(w, ampS[k], s1);
+                    y[k] = z;
+                    x[k] = w;
+                }
+                rd.put(j, c0, s0);
+                rd.put(j + 1, c1, s1);
+            }
+            if (PIPE) rd.finish();
+            rd.commit(8, dst + 2 * n);
+            if (!PIPE) {
+                rd.begin();
+                rd.finish();
+            }
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) a[j] = an[j];
+        if (xph && ++win == 4) {
+            // every 32 degrees while a ring is scaled: rescale mantissas that
+            // left [2^-256, 2^256], return rings above 2^XS_PLAIN to plain
+            // doubles (and their amplitudes to them), find the deep rings
+            win = 0;
+            int d[K];
+            bool need = false;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int ex = xs_exp(x[k], y[k]);
+                d[k] = e[k] + ex;
+                need = need || xs_need(e[k], ex);
+            }
+            bool back[K], anyback = false;
+#pragma unroll
+            for (int k = 0; k < K; k++) back[k] = false;
+            if (wave_any(need)) {
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const int sh = xs_shift(e[k], d[k] - e[k]);
+                    x[k] = ldexp(x[k], sh);
+                    y[k] = ldexp(y[k], sh);
+                    back[k] = e[k] != 0 && sh == e[k];
+                    anyback = anyback || back[k];
+                    e[k] -= sh;
+                }
+            }
+            bool anyx = false, deep = true;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                deep = deep && (!valid[k] || (e[k] != 0 && d[k] < XA_DEEP));
+                anyx = anyx || e[k] != 0;
+            }
+            xph = wave_any(anyx);
+            alldeep = wave_all(deep);
+            dead = dead && alldeep;
+            if (wave_any(anyback)) {
+                // (the order and ring base through readfirstlane, so that the
+                // addresses are formed in this rare branch)
+                const int mr = uniform(m), rb = uniform(rbase);
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                    if (back[k])
+                        sha_amps(A, min(rb + k * 64 + lane, A.nring - 1), mr, ampC[k], ammC[k],
+                                 ampS[k], ammS[k]);
+            }
+        }
     }
+    if (A.dead_mode == 1 && lane == 0) A.dead[(size_t)m * A.ntile + tile] = dead ? 1 : 0;
     // last, partial tile
     if (n <= nmax) {
         const int cnt = nmax - n + 1;
@@ -1167,9 +1178,8 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
                     const double z = fma(al * x[k], t[k], -y[k]);
                     y[k] = x[k];
                     x[k] = z;
-                    const double p = ldexp(z, e[k]);
-                    c = fma(p, (j & 1) ? ampC[k] : ammC[k], c);
-                    sv = fma(p, (j & 1) ? ampS[k] : ammS[k], sv);
+                    c = fma(z, (j & 1) ? ampC[k] : ammC[k], c);
+                    sv = fma(z, (j & 1) ? ampS[k] : ammS[k], sv);
                 }
             }
             rd.put(j, c, sv);
@@ -1210,6 +1220,12 @@ __device__ __forceinline__ void sha_tile(const ShaArgs &A, int tile, int quad, i
     if constexpr (K >= 5) {
         if (kt == 4) {
             sha_item<4>(A, tile, rbase, quad, slot, acc, reinterpret_cast<ShaSmem<4> &>(sm));
+            return;
+        }
+    }
+    if constexpr (K == 6) {
+        if (kt == 5) {
+            sha_item<5>(A, tile, rbase, quad, slot, acc, reinterpret_cast<ShaSmem<5> &>(sm));
             return;
         }
     }
