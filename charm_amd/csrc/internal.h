@@ -132,7 +132,8 @@ struct MultiShsArgs {
     int nset;
     int mcount;           // orders written per row (nfc; orders above nmax get zeros)
     int nrt;              // ring tiles of 64 ring pairs
-    int npair;            // set tiles (16 sets) x ring tiles
+    int nst;              // set tiles of 16 sets
+    int nunit;            // (order group of 8, set tile) units
     const double *t;
     const double *sx;
     const int *six;
@@ -140,16 +141,20 @@ struct MultiShsArgs {
     const unsigned char *mirror;
     const double *al;     // alpha_nm, degree al_nt
     int al_nt;
-    const void *cs;       // (C s, S s) pairs, set-major, cs_stride pairs per set
-    size_t cs_stride;
+    const void *cs;       // (C s, S s) pairs degree-major: entry (m, n) of the triangle holds
+                          // nsetp consecutive sets (launch_scale_cs_multi's layout)
+    int nsetp;            // sets padded to a multiple of 16 (the padding holds zeros)
     void *spec;           // [set][row][lds] double2
     size_t lds;
     double mur;
     const int *pm0;       // as in ShsGridArgs; nullptr: off
 };
 int launch_shs_multi(const MultiShsArgs &a, void *stream);
+// (C_nm s_nm, S_nm s_nm) of nset models of degree nmax (set-major input, one
+// m-major triangle per set) into the layout of MultiShsArgs::cs (nsetp
+// sets per triangle entry, the padding zeroed)
 int launch_scale_cs_multi(int nmax, size_t nset, const void *cs_dev, const double *sn_dev, int nt,
-                          void *out_dev, void *stream);
+                          void *out_dev, int nsetp, void *stream);
 
 // Derivatives of the synthesis (src/shs/shs.h.in:176-436).  Every derivative
 // is a sum of ordinary syntheses F[T] over transformed coefficient tables T,

@@ -2137,8 +2137,14 @@ void charm_amd_shs_grid_multi_dev(const charm_amd_grid *grd, unsigned long nmax,
         hipStream_t s = (hipStream_t)stream;
         Tables tb = tables((int)std::max<unsigned long>(nmax, 1), s);
         const size_t P = tri_len(nmax), nlat = grd->g.nlat, nlon = grd->g.nlon, nfc = nlon / 2 + 1;
-        DevBuf cs2(nset * P * 2 * sizeof(double), s);
-        HIPCHK((hipError_t)launch_scale_cs_multi((int)nmax, nset, cs_dev, tb.sn, tb.nt, cs2.p, s));
+        if (nset > (size_t)(INT32_MAX / 16) * 16 - 16) {
+            CHARM_AMD_ERR(err, CHARM_EFUNCARG, "Too many coefficient sets.");
+            return;
+        }
+        const int nsetp = (int)((nset + 15) / 16 * 16);
+        DevBuf cs2((size_t)nsetp * P * 2 * sizeof(double), s);
+        HIPCHK((hipError_t)launch_scale_cs_multi((int)nmax, nset, cs_dev, tb.sn, tb.nt, cs2.p, nsetp,
+                                                 s));
         DevBuf spec(nset * nlat * nfc * 2 * sizeof(double), s);
         MultiShsArgs a{};
         a.nmax = (int)nmax;
@@ -2147,7 +2153,9 @@ void charm_amd_shs_grid_multi_dev(const charm_amd_grid *grd, unsigned long nmax,
         a.nset = (int)nset;
         a.mcount = (int)nfc;
         a.nrt = (int)((grd->g.nring + 63) / 64);
-        a.npair = (int)((nset + 15) / 16) * a.nrt;
+        a.nst = nsetp / 16;
+        a.nunit = (int)((nfc + 7) / 8) * a.nst;
+        a.nsetp = nsetp;
         a.t = grd->t;
         a.sx = grd->sx;
         a.six = grd->six;
@@ -2156,7 +2164,6 @@ void charm_amd_shs_grid_multi_dev(const charm_amd_grid *grd, unsigned long nmax,
         a.al = tb.al;
         a.al_nt = tb.nt;
         a.cs = cs2.p;
-        a.cs_stride = P;
         a.spec = spec.p;
         a.lds = nfc;
         a.mur = mu / R;
