@@ -215,11 +215,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void sh
             }
             if (!deep) mma(v, a2);
         };
-        // a whole block: degrees n0..n0+7 <= nmax, A-operands requested earlier
+        // a whole block: degrees n0..n0+7 <= nmax, alphas and A-operands
+        // requested earlier; requests the next whole block's alphas
+        double alc[8];
         auto block = [&](int n0, const double2 *a2) {
             double al[8];
 #pragma unroll
-            for (int j = 0; j < 8; j++) al[j] = calc[n0 + j];
+            for (int j = 0; j < 8; j++) al[j] = alc[j];
+            if (n0 + 15 <= nmax) {
+#pragma unroll
+                for (int j = 0; j < 8; j++) alc[j] = calc[n0 + 8 + j];
+            }
             const bool deep = deep_now();
             double v[8];
             if (xmode) {
@@ -249,7 +255,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void sh
         int n0 = m + 8;
         // whole blocks in pairs, alternating A-operand buffers; a buffer is
         // refilled right after its block, two blocks before it is used
-        if (n0 + 7 <= nmax) aload(n0, aA);
+        if (n0 + 7 <= nmax) {
+            aload(n0, aA);
+#pragma unroll
+            for (int j = 0; j < 8; j++) alc[j] = calc[n0 + j];
+        }
         if (n0 + 15 <= nmax) aload(n0 + 8, aB);
         for (; n0 + 15 <= nmax; n0 += 16) {
             block(n0, aA);
