@@ -281,42 +281,177 @@ __device__ __forceinline__ cplx row_out(const RowArgs &a, const cplx *x, int k, 
     return fft::axpy(1.0 / (double)M, x[o.s], v0);
 }
 
+// The whole-row kernels below read and write a row in batches that cover it
+// in one step at the common sizes (L = 2161 over 256 threads): a thread
+// issues all global loads of a batch (values, index and twiddle tables)
+// before it waits for any, and the LDS twiddle tables are fetched together
+// with the row's first batch.  Outputs k and L - k (r2c) and inputs n and
+// L - n (c2r) come from the same two values of the row and are made by one
+// thread; pt_{L-n} = -conj pt_n holds exactly (chirp_table_kernel).
+constexpr int POINT_BATCH = 9;
+
+// Pairs per batch (Bluestein pairs also load two chirp factors)
+template <int MODE>
+constexpr int pair_batch()
+{
+    return MODE == fft::DFT_BLUESTEIN ? 3 : 5;
+}
+
+__device__ __forceinline__ int tw_entries(int M)
+{
+    return fft::tw_hi_count(M) + min(M, fft::TW_LO);
+}
+
+// Entry i of the LDS twiddle tables (hi: every TW_LO-th power, then lo: the
+// first TW_LO powers), which sit one after the other behind the row's slots
+__device__ __forceinline__ cplx tw_entry(const cplx *tw, int nh, int i)
+{
+    return tw[i < nh ? i * fft::TW_LO : i - nh];
+}
+
+// This thread's first table entry, loaded before the row's first batch
+__device__ __forceinline__ cplx tw_fetch(const cplx *tw, int M)
+{
+    const int i = threadIdx.x;
+    return i < tw_entries(M) ? tw_entry(tw, fft::tw_hi_count(M), i) : fft::mk(0.0, 0.0);
+}
+
+// Writes the first entry and loads and writes the rest (threads beyond the
+// block size: short rows); visible after the caller's next barrier
+__device__ __forceinline__ fft::TwSplit tw_store(const cplx *tw, int M, cplx first)
+{
+    const int nh = fft::tw_hi_count(M), ne = tw_entries(M);
+    cplx *t = lds_rows + fft::slots(M);
+    if ((int)threadIdx.x < ne) t[threadIdx.x] = first;
+    for (int i = threadIdx.x + blockDim.x; i < ne; i += blockDim.x) t[i] = tw_entry(tw, nh, i);
+    fft::TwSplit r;
+    r.hi = t;
+    r.lo = t + nh;
+    return r;
+}
+
+// Where input n of the row's DFT goes (Rader: n >= 1 to slot(log_g n)) and
+// its factor (Bluestein: the chirp w_n)
+struct InAt {
+    int s;
+    cplx w;
+};
+
+template <int MODE>
+__device__ __forceinline__ InAt row_in_at(const RowArgs &a, int n)
+{
+    InAt o;
+    o.w = fft::mk(0.0, 0.0);
+    if (MODE == fft::DFT_RADER) {
+        o.s = fft::slot(a.lg[n]);
+    } else {
+        o.s = fft::slot(n);
+        if (MODE == fft::DFT_BLUESTEIN) o.w = a.w[n];
+    }
+    return o;
+}
+
+template <int MODE>
+__device__ __forceinline__ cplx row_in(const InAt &o, cplx v)
+{
+    return MODE == fft::DFT_BLUESTEIN ? fft::mul(v, o.w) : v;
+}
+
+// Zero padding of the convolution (Bluestein) and the sum of the inputs
+// (Rader, per wave beside the tables); then the transform
+template <int MODE>
+__device__ __forceinline__ void row_transform(const RowArgs &a, cplx *x, const fft::TwSplit &tw,
+                                              cplx acc)
+{
+    const int L = a.L, M = a.plan.M;
+    if (MODE == fft::DFT_BLUESTEIN)
+        for (int n = L + threadIdx.x; n < M; n += blockDim.x) x[fft::slot(n)] = fft::mk(0.0, 0.0);
+    if (MODE == fft::DFT_RADER) {
+        acc = wave_reduce(acc);
+        if ((threadIdx.x & 63) == 0) wave_sums(M)[threadIdx.x >> 6] = acc;
+    }
+    __syncthreads();
+    if (MODE == fft::DFT_DIRECT)
+        forward_dft(x, tw, a.plan);
+    else
+        convolve(x, a.B, tw, a.plan);
+}
+
 // f (2L reals per row) -> spec (L+1 complex per row), F_k = sum_j f_j e^{-2 pi i jk/2L}
 template <int MODE>
 __device__ __forceinline__ void r2c_row(const RowArgs &a, size_t row)
 {
     cplx *x = lds_rows;
-    const int L = a.L;
+    const int L = a.L, M = a.plan.M, bd = blockDim.x;
     const cplx *z = (const cplx *)(a.f + row * a.ldf);
-    const cplx z0 = (MODE == fft::DFT_RADER) ? z[0] : fft::mk(0.0, 0.0);
-    const fft::TwSplit tw = lds_twiddles(a.tw, a.plan.M);
-    row_dft<MODE>(a, x, tw, [&](int n) { return z[n]; });
-    // Z_k; F_k = E_k + e^{-i pi k/L} O_k with E_k = (Z_k + conj Z_{L-k}) / 2,
-    // O_k = (Z_k - conj Z_{L-k}) / (2i)
-    cplx *out = a.spec + row * a.lds;
-    const int bd = blockDim.x;
-    for (int k0 = threadIdx.x; k0 <= L; k0 += IO_BATCH * bd) {
-        OutAt ok[IO_BATCH], orr[IO_BATCH];
-        cplx pt[IO_BATCH];
+    const cplx tw0 = tw_fetch(a.tw, M);
+    cplx acc = fft::mk(0.0, 0.0);
+    // the first batch outside the loop, so that its loads are issued while the
+    // twiddle load is still in flight (a loop header waits for all of them)
+    auto gather = [&](int n0) {
+        cplx v[POINT_BATCH];
+        InAt o[POINT_BATCH];
 #pragma unroll
-        for (int j = 0; j < IO_BATCH; j++) {
-            const int k = min(k0 + j * bd, L);
-            const int kk = (k == L) ? 0 : k, kr = (k == 0) ? 0 : L - k;
-            ok[j] = row_out_at<MODE>(a, kk);
-            orr[j] = row_out_at<MODE>(a, kr);
-            pt[j] = a.pt[k];
+        for (int j = 0; j < POINT_BATCH; j++) {
+            const int n = min(n0 + j * bd, L - 1);
+            v[j] = z[n];
+            o[j] = row_in_at<MODE>(a, n);
         }
 #pragma unroll
-        for (int j = 0; j < IO_BATCH; j++) {
+        for (int j = 0; j < POINT_BATCH; j++) {
+            const int n = n0 + j * bd;
+            if (n >= L) continue;
+            if (MODE == fft::DFT_RADER) {
+                acc = fft::add(acc, v[j]);
+                if (n >= 1) x[o[j].s] = v[j];
+            } else {
+                x[o[j].s] = row_in<MODE>(o[j], v[j]);
+            }
+        }
+    };
+    gather(threadIdx.x);
+    for (int n0 = threadIdx.x + POINT_BATCH * bd; n0 < L; n0 += POINT_BATCH * bd) gather(n0);
+    // input 0 (Rader outputs), read once the row's loads are on their way: a
+    // scalar load before them makes every thread wait for it first
+    const cplx z0 = (MODE == fft::DFT_RADER) ? z[0] : fft::mk(0.0, 0.0);
+    row_transform<MODE>(a, x, tw_store(a.tw, M, tw0), acc);
+    // Z_k; F_k = E_k + e^{-i pi k/L} O_k with E_k = (Z_k + conj Z_{L-k}) / 2,
+    // O_k = (Z_k - conj Z_{L-k}) / (2i); F_{L-k} = conj(E_k - e^{-i pi k/L} O_k)
+    // (Z_L = Z_0)
+    cplx *out = a.spec + row * a.lds;
+    const int nk = (L + 1) / 2;  // pairs (k, L - k), 0 <= k < nk
+    constexpr int PAIR_BATCH = pair_batch<MODE>();
+    for (int k0 = threadIdx.x; k0 < nk; k0 += PAIR_BATCH * bd) {
+        OutAt ok[PAIR_BATCH], orr[PAIR_BATCH];
+        cplx pw[PAIR_BATCH];
+#pragma unroll
+        for (int j = 0; j < PAIR_BATCH; j++) {
+            const int k = min(k0 + j * bd, nk - 1), kr = (k == 0) ? 0 : L - k;
+            ok[j] = row_out_at<MODE>(a, k);
+            orr[j] = row_out_at<MODE>(a, kr);
+            pw[j] = a.pt[k];
+        }
+#pragma unroll
+        for (int j = 0; j < PAIR_BATCH; j++) {
             const int k = k0 + j * bd;
-            if (k > L) continue;
-            const int kk = (k == L) ? 0 : k, kr = (k == 0) ? 0 : L - k;
-            const cplx zk = row_out<MODE>(a, x, kk, ok[j], z0);
+            if (k >= nk) continue;
+            const int kr = (k == 0) ? 0 : L - k;
+            const cplx zk = row_out<MODE>(a, x, k, ok[j], z0);
             const cplx zr = fft::conj(row_out<MODE>(a, x, kr, orr[j], z0));
             const cplx e = fft::scale(fft::add(zk, zr), 0.5);
             const cplx o = fft::rot90<-1>(fft::scale(fft::sub(zk, zr), 0.5));
-            out[k] = fft::add(e, fft::mul(pt[j], o));
+            const cplx m = fft::mul(pw[j], o);
+            out[k] = fft::add(e, m);
+            out[L - k] = fft::conj(fft::sub(e, m));
         }
+    }
+    if (!(L & 1) && threadIdx.x == 0) {
+        const int k = L / 2;
+        const cplx zk = row_out<MODE>(a, x, k, row_out_at<MODE>(a, k), z0);
+        const cplx zr = fft::conj(zk);
+        const cplx e = fft::scale(fft::add(zk, zr), 0.5);
+        const cplx o = fft::rot90<-1>(fft::scale(fft::sub(zk, zr), 0.5));
+        out[k] = fft::add(e, fft::mul(a.pt[k], o));
     }
 }
 
@@ -343,26 +478,81 @@ __device__ __forceinline__ cplx c2r_input(const RowArgs &a, const cplx *X, int n
     return fft::conj(fft::add(e, fft::rot90<+1>(o)));
 }
 
+// c2r_input(n) and c2r_input(L - n) for 1 <= n, 2n != L, from X_n, X_{L-n}
+// and pt_n: with E and O of input n, Z_{L-n} = conj E + i conj O
+__device__ __forceinline__ void c2r_pair(cplx xk, cplx xr, cplx p, cplx &in_n, cplx &in_r)
+{
+    const cplx e = fft::scale(fft::add(xk, fft::conj(xr)), 0.5);
+    const cplx o = fft::mul(fft::scale(fft::sub(xk, fft::conj(xr)), 0.5), fft::conj(p));
+    in_n = fft::conj(fft::add(e, fft::rot90<+1>(o)));
+    in_r = fft::add(e, fft::rot90<-1>(o));
+}
+
 // spec (L+1 complex per row) -> f (2L reals per row),
 // f_j = sum_{m=0}^{2L-1} X_m e^{+2 pi i jm/2L} over the Hermitian extension
 template <int MODE>
 __device__ __forceinline__ void c2r_row(const RowArgs &a, size_t row)
 {
     cplx *x = lds_rows;
-    const int L = a.L;
+    const int L = a.L, M = a.plan.M, bd = blockDim.x;
     const cplx *X = a.spec + row * a.lds;
-    const cplx v0 = (MODE == fft::DFT_RADER) ? c2r_input(a, X, 0) : fft::mk(0.0, 0.0);
-    const fft::TwSplit tw = lds_twiddles(a.tw, a.plan.M);
-    row_dft<MODE>(a, x, tw, [&](int n) { return c2r_input(a, X, n); });
+    const cplx tw0 = tw_fetch(a.tw, M);
+    const int np = (L - 1) / 2;  // pairs (n, L - n), 1 <= n <= np
+    cplx acc = fft::mk(0.0, 0.0);
+    constexpr int PAIR_BATCH = pair_batch<MODE>();
+    // the first batch outside the loop (as in r2c_row)
+    auto gather = [&](int j0) {
+        cplx xk[PAIR_BATCH], xr[PAIR_BATCH], pw[PAIR_BATCH];
+        InAt ia[PAIR_BATCH], ib[PAIR_BATCH];
+#pragma unroll
+        for (int j = 0; j < PAIR_BATCH; j++) {
+            const int n = 1 + min(j0 + j * bd, np - 1);
+            xk[j] = X[n];
+            xr[j] = X[L - n];
+            pw[j] = a.pt[n];
+            ia[j] = row_in_at<MODE>(a, n);
+            ib[j] = row_in_at<MODE>(a, L - n);
+        }
+#pragma unroll
+        for (int j = 0; j < PAIR_BATCH; j++) {
+            if (j0 + j * bd >= np) continue;
+            cplx u, v;
+            c2r_pair(xk[j], xr[j], pw[j], u, v);
+            if (MODE == fft::DFT_RADER) acc = fft::add(acc, fft::add(u, v));
+            x[ia[j].s] = row_in<MODE>(ia[j], u);
+            x[ib[j].s] = row_in<MODE>(ib[j], v);
+        }
+    };
+    if (np > 0) {
+        gather(threadIdx.x);
+        for (int j0 = threadIdx.x + PAIR_BATCH * bd; j0 < np; j0 += PAIR_BATCH * bd) gather(j0);
+    }
+    // input 0, read once the row's loads are on their way (as in r2c_row)
+    const cplx v0 = c2r_input(a, X, 0);
+    if (threadIdx.x == 0) {
+        // input 0 (Rader: only in the sum) and, for even L, input L / 2
+        if (MODE == fft::DFT_RADER) {
+            acc = fft::add(acc, v0);
+        } else {
+            const InAt o = row_in_at<MODE>(a, 0);
+            x[o.s] = row_in<MODE>(o, v0);
+        }
+        if (!(L & 1)) {
+            const InAt o = row_in_at<MODE>(a, L / 2);
+            const cplx vh = c2r_input(a, X, L / 2);
+            if (MODE == fft::DFT_RADER) acc = fft::add(acc, vh);
+            x[o.s] = row_in<MODE>(o, vh);
+        }
+    }
+    row_transform<MODE>(a, x, tw_store(a.tw, M, tw0), acc);
     // z_n = conj(DFT_n); f_2n = 2 Re z_n, f_2n+1 = 2 Im z_n
     cplx *f = (cplx *)(a.f + row * a.ldf);
-    const int bd = blockDim.x;
-    for (int n0 = threadIdx.x; n0 < L; n0 += IO_BATCH * bd) {
-        OutAt o[IO_BATCH];
+    for (int n0 = threadIdx.x; n0 < L; n0 += POINT_BATCH * bd) {
+        OutAt o[POINT_BATCH];
 #pragma unroll
-        for (int j = 0; j < IO_BATCH; j++) o[j] = row_out_at<MODE>(a, min(n0 + j * bd, L - 1));
+        for (int j = 0; j < POINT_BATCH; j++) o[j] = row_out_at<MODE>(a, min(n0 + j * bd, L - 1));
 #pragma unroll
-        for (int j = 0; j < IO_BATCH; j++) {
+        for (int j = 0; j < POINT_BATCH; j++) {
             const int n = n0 + j * bd;
             if (n >= L) continue;
             const cplx y = row_out<MODE>(a, x, n, o[j], v0);
@@ -431,14 +621,17 @@ __global__ void chirp_table_kernel(int L, cplx *w, cplx *pt)
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n < L) w[n] = fft::chirp(n, L, -1);
     if (n <= L) {
+        // pt_n from the angle pi min(n, L - n) / L, pt_{L-n} = -conj pt_n
+        // exactly (the row kernels make outputs and inputs n, L - n in pairs)
+        const bool upper = 2 * n > L;
         double s, c;
-        sincos(3.14159265358979323846 * (double)n / (double)L, &s, &c);
+        sincos(3.14159265358979323846 * (double)(upper ? L - n : n) / (double)L, &s, &c);
         // exact at the quarter points
         if (2 * n == L) {
             c = 0.0;
             s = 1.0;
         }
-        pt[n] = fft::mk(c, -s);
+        pt[n] = upper ? fft::mk(-c, -s) : fft::mk(c, -s);
     }
 }
 

@@ -450,16 +450,29 @@ FFT_HD void last_block(cplx *x, int u)
 }
 
 // Last forward pass, product with the chirp spectrum, first inverse pass, in
-// registers (block u of R contiguous points).
+// registers (block u of R contiguous points).  The first half of the block's
+// spectrum values is requested before the LDS reads and the forward DFT, the
+// second half right after the first half's products: two overlapped global
+// round trips, where loads placed at their use (the register pressure of the
+// 16-point DFT sinks them there) cost four in a row.
 template <int R>
 FFT_HD void middle_block(cplx *x, const cplx *B, int u)
 {
+    constexpr int H = (R + 1) / 2;
+    const cplx *b = B + u * R;
+    cplx w[H];
+#pragma unroll
+    for (int r = 0; r < H; r++) w[r] = b[r];
     cplx v[R];
 #pragma unroll
     for (int r = 0; r < R; r++) v[r] = x[block_slot<R>(u, r)];
     dft<R, -1>(v);
 #pragma unroll
-    for (int r = 0; r < R; r++) v[r] = mul(v[r], B[u * R + r]);
+    for (int r = 0; r < H; r++) v[r] = mul(v[r], w[r]);
+#pragma unroll
+    for (int r = H; r < R; r++) w[r - H] = b[r];
+#pragma unroll
+    for (int r = H; r < R; r++) v[r] = mul(v[r], w[r - H]);
     dft<R, +1>(v);
 #pragma unroll
     for (int r = 0; r < R; r++) x[block_slot<R>(u, r)] = v[r];
