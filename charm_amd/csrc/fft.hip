@@ -177,67 +177,7 @@ __device__ __forceinline__ cplx wave_reduce(cplx v)
     return v;
 }
 
-// Rows are read and written by each thread in batches of IO_BATCH points:
-// the batch's global loads (inputs, index and twiddle tables) are all issued,
-// from indices clamped into the row, before any of them is waited for, so a
-// thread waits once per batch rather than once per point.
-constexpr int IO_BATCH = 4;
-
-// Loads the L inputs v(n) of a row's DFT into LDS in the order MODE needs
-// (Rader: point n >= 1 at slot(log_g n), the sum of all points per wave
-// beside the tables) and runs the transform.  get(n) gives input n, for any
-// 0 <= n < L.
-template <int MODE, class Get>
-__device__ __forceinline__ void row_dft(const RowArgs &a, cplx *x, const fft::TwSplit &tw, Get get)
-{
-    const fft::MixPlan &p = a.plan;
-    const int L = a.L, M = p.M;
-    const int bd = blockDim.x;
-    if (MODE == fft::DFT_RADER) {
-        cplx acc = fft::mk(0.0, 0.0);
-        for (int n0 = threadIdx.x; n0 < L; n0 += IO_BATCH * bd) {
-            cplx v[IO_BATCH];
-            int q[IO_BATCH];
-#pragma unroll
-            for (int j = 0; j < IO_BATCH; j++) {
-                const int n = min(n0 + j * bd, L - 1);
-                v[j] = get(n);
-                q[j] = a.lg[n];
-            }
-#pragma unroll
-            for (int j = 0; j < IO_BATCH; j++) {
-                const int n = n0 + j * bd;
-                if (n < L) {
-                    acc = fft::add(acc, v[j]);
-                    if (n >= 1) x[fft::slot(q[j])] = v[j];
-                }
-            }
-        }
-        acc = wave_reduce(acc);
-        if ((threadIdx.x & 63) == 0) wave_sums(M)[threadIdx.x >> 6] = acc;
-    } else {
-        for (int n0 = threadIdx.x; n0 < M; n0 += IO_BATCH * bd) {
-            cplx v[IO_BATCH];
-#pragma unroll
-            for (int j = 0; j < IO_BATCH; j++) {
-                const int n = min(n0 + j * bd, L - 1);
-                v[j] = (MODE == fft::DFT_BLUESTEIN) ? fft::mul(get(n), a.w[n]) : get(n);
-            }
-#pragma unroll
-            for (int j = 0; j < IO_BATCH; j++) {
-                const int n = n0 + j * bd;
-                if (n < M) x[fft::slot(n)] = n < L ? v[j] : fft::mk(0.0, 0.0);
-            }
-        }
-    }
-    __syncthreads();
-    if (MODE == fft::DFT_DIRECT)
-        forward_dft(x, tw, p);
-    else
-        convolve(x, a.B, tw, p);
-}
-
-// Output k of the row's DFT after row_dft, in two steps: row_out_at reads
+// Output k of the row's DFT after the transform, in two steps: row_out_at reads
 // what output k needs from the global tables (its LDS slot; Bluestein's
 // chirp factor), row_out the LDS (v0 = input 0, used by Rader).
 struct OutAt {
@@ -693,6 +633,51 @@ __device__ __forceinline__ cplx c2r_input_full(int L, const cplx *pt, const cplx
     return fft::conj(fft::add(e, fft::rot90<+1>(o)));
 }
 
+// Points per thread and batch of the column blocks (a column of L2 = 1543
+// points over 320 threads in one batch; 4 where a point of the inverse
+// transform also carries the chirp factor, which would spill at 5)
+constexpr int COL_BATCH = 5;
+
+template <int MODE, bool C2R>
+constexpr int col_batch()
+{
+    return (C2R && MODE == fft::DFT_BLUESTEIN) ? 4 : COL_BATCH;
+}
+
+// Loads the L inputs get(n) of a column's DFT into LDS in the order MODE
+// needs, the first batch's loads issued while the twiddle load (tw0) is in
+// flight, and runs the transform
+template <int MODE, int COL_BATCH, class Get>
+__device__ __forceinline__ void column_dft(const RowArgs &a, cplx *x, cplx tw0, Get get)
+{
+    const int L = a.L, M = a.plan.M, bd = blockDim.x;
+    cplx acc = fft::mk(0.0, 0.0);
+    auto gather = [&](int n0) {
+        cplx v[COL_BATCH];
+        InAt o[COL_BATCH];
+#pragma unroll
+        for (int j = 0; j < COL_BATCH; j++) {
+            const int n = min(n0 + j * bd, L - 1);
+            v[j] = get(n);
+            o[j] = row_in_at<MODE>(a, n);
+        }
+#pragma unroll
+        for (int j = 0; j < COL_BATCH; j++) {
+            const int n = n0 + j * bd;
+            if (n >= L) continue;
+            if (MODE == fft::DFT_RADER) {
+                acc = fft::add(acc, v[j]);
+                if (n >= 1) x[o[j].s] = v[j];
+            } else {
+                x[o[j].s] = row_in<MODE>(o[j], v[j]);
+            }
+        }
+    };
+    gather(threadIdx.x);
+    for (int n0 = threadIdx.x + COL_BATCH * bd; n0 < L; n0 += COL_BATCH * bd) gather(n0);
+    row_transform<MODE>(a, x, tw_store(a.tw, M, tw0), acc);
+}
+
 template <int MODE, bool C2R>
 __global__ __launch_bounds__(1024) void split_a_kernel(SplitArgs a)
 {
@@ -706,28 +691,29 @@ __global__ __launch_bounds__(1024) void split_a_kernel(SplitArgs a)
     const size_t row = (size_t)(b / span) * 8 + (b % span) % 8;
     const int n1 = (int)((b % span) / 8);
     if (row >= a.nrow) return;
-    const fft::TwSplit tw = lds_twiddles(a.sub.tw, a.sub.plan.M);
+    const cplx tw0 = tw_fetch(a.sub.tw, a.sub.plan.M);
     const cplx *X = a.spec + row * a.lds;
     const cplx *z = (const cplx *)(a.f + row * a.ldf);
     auto get = [&](int n2) {
         const int n = n1 + L1 * n2;
         return C2R ? c2r_input_full(L, a.ptf, X, n) : z[n];
     };
+    column_dft<MODE, col_batch<MODE, C2R>()>(a.sub, x, tw0, get);
+    // input 0 (Rader outputs; read after the column's loads)
     const cplx v0 = (MODE == fft::DFT_RADER) ? get(0) : fft::mk(0.0, 0.0);
-    row_dft<MODE>(a.sub, x, tw, get);
     cplx *y = a.Y + (row * L1 + n1) * (size_t)L2;
     const int bd = blockDim.x;
-    for (int k0 = threadIdx.x; k0 < L2; k0 += IO_BATCH * bd) {
-        OutAt o[IO_BATCH];
-        cplx w[IO_BATCH];
+    for (int k0 = threadIdx.x; k0 < L2; k0 += COL_BATCH * bd) {
+        OutAt o[COL_BATCH];
+        cplx w[COL_BATCH];
 #pragma unroll
-        for (int i = 0; i < IO_BATCH; i++) {
+        for (int i = 0; i < COL_BATCH; i++) {
             const int k2 = min(k0 + i * bd, L2 - 1);
             o[i] = row_out_at<MODE>(a.sub, k2);
             w[i] = a.twf[(int)(((long)n1 * k2) % L)];
         }
 #pragma unroll
-        for (int i = 0; i < IO_BATCH; i++) {
+        for (int i = 0; i < COL_BATCH; i++) {
             const int k2 = k0 + i * bd;
             if (k2 < L2) y[k2] = fft::mul(row_out<MODE>(a.sub, x, k2, o[i], v0), w[i]);
         }

@@ -94,9 +94,12 @@ __device__ __forceinline__ bool wave_all(bool p)
 //    for the next 16 degrees, so every term of the tile is exactly zero.
 constexpr int XS_PLAIN = -800;
 constexpr int XS_DEEP = -1250;
-// Deep threshold of the analysis, whose checks are 32 degrees apart (a
-// column grows by less than 2^288 over them): values stay below 2^-1100.
-constexpr int XA_DEEP = -1400;
+// The analysis makes a returning ring's amplitudes again from the spectrum
+// (a global round trip for its wave): once one ring of a wave returns, every
+// scaled ring of the wave above 2^XS_BACK returns with it.  Its values and
+// their previous ones (at least 2^-1009: a column grows by less than 2^9 per
+// degree) are normal doubles, so the recurrence rounds as in the scaled form.
+constexpr int XS_BACK = -1000;
 
 __device__ __forceinline__ int xs_exp(double x, double y)
 {
@@ -1030,22 +1033,20 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
         v1 = fma(x[k], ampS[k], v1);
     }
     // The wave's state, made at the seed and every 32 degrees while a ring is
-    // scaled: xph (some ring scaled), alldeep (every ring deep: its values
-    // stay below 2^-1100 for the next 32 degrees, so every term is zero and
-    // the tiles run the recurrence only), dead (deep from the seed on: the
-    // whole column is zero).
-    bool xph, alldeep, dead;
+    // scaled: xph (some ring scaled), allx (every ring scaled: every term is
+    // left out, so the tiles run the recurrence only and write zero sums),
+    // dead (every ring scaled from the seed on: the whole column is zero).
+    bool xph, allx, dead;
     {
-        bool anyx = false, deep = true;
+        bool anyx = false, all = true;
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const int ex = xs_exp(x[k], y[k]);
-            deep = deep && (!valid[k] || (e[k] != 0 && e[k] + ex < XA_DEEP));
             anyx = anyx || e[k] != 0;
+            all = all && (!valid[k] || e[k] != 0);
         }
         xph = wave_any(anyx);
-        alldeep = wave_all(deep);
-        dead = alldeep;
+        allx = wave_all(all);
+        dead = allx;
     }
     rd.put(0, v0, v1);
     rd.commit(1, dst + 2 * m);
@@ -1071,12 +1072,15 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
     double a[8];
     if (n + 7 <= nmax) load_alpha8_const(calc + n, a);
     int win = 0;  // tiles since the last check
-    for (; n + 7 <= nmax; n += 8) {
+    // While a ring of the wave is scaled: one degree at a time (the sums of
+    // two degrees at once, as below, need registers that the rings'
+    // exponents and the checks take here), checks every 32 degrees.
+    for (; xph && n + 7 <= nmax; n += 8) {
         double an[8];
         load_alpha8_const(calc + (n + 15 <= nmax ? n + 8 : n), an);
         if (PIPE) rd.begin();
-        if (alldeep) {
-            // every term of the tile is exactly zero
+        if (allx) {
+            // every term of the tile is left out
 #pragma unroll
             for (int j = 0; j < 8; j++) {
 #pragma unroll
@@ -1090,21 +1094,17 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
             if (lane < 16 && !acc) dst[2 * n + lane] = 0.0;
         } else {
 #pragma unroll
-            for (int j = 0; j < 8; j += 2) {
-                double c0 = 0.0, s0 = 0.0, c1 = 0.0, s1 = 0.0;
+            for (int j = 0; j < 8; j++) {
+                double c = 0.0, sv = 0.0;
 #pragma unroll
                 for (int k = 0; k < K; k++) {
                     const double z = fma(a[j] * x[k], t[k], -y[k]);
-                    c0 = fma(z, ammC[k], c0);
-                    s0 = fma(z, ammS[k], s0);
-                    const double w = fma(a[j + 1] * z, t[k], -x[k]);
-                    c1 = fma(w, ampC[k], c1);
-                    s1 = fma(w, ampS[k], s1);
-                    y[k] = z;
-                    x[k] = w;
+                    y[k] = x[k];
+                    x[k] = z;
+                    c = fma(z, (j & 1) ? ampC[k] : ammC[k], c);
+                    sv = fma(z, (j & 1) ? ampS[k] : ammS[k], sv);
                 }
-                rd.put(j, c0, s0);
-                rd.put(j + 1, c1, s1);
+                rd.put(j, c, sv);
             }
             if (PIPE) rd.finish();
             rd.commit(8, dst + 2 * n);
@@ -1115,26 +1115,28 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) a[j] = an[j];
-        if (xph && ++win == 4) {
+        if (++win == 4) {
             // every 32 degrees while a ring is scaled: rescale mantissas that
             // left [2^-256, 2^256], return rings above 2^XS_PLAIN to plain
-            // doubles (and their amplitudes to them), find the deep rings
+            // doubles (and their amplitudes to them)
             win = 0;
             int d[K];
-            bool need = false;
+            bool need = false, must = false;
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const int ex = xs_exp(x[k], y[k]);
                 d[k] = e[k] + ex;
                 need = need || xs_need(e[k], ex);
+                must = must || (e[k] != 0 && d[k] > XS_PLAIN);
             }
             bool back[K], anyback = false;
 #pragma unroll
             for (int k = 0; k < K; k++) back[k] = false;
             if (wave_any(need)) {
+                const int plain = wave_any(must) ? XS_BACK : XS_PLAIN;
 #pragma unroll
                 for (int k = 0; k < K; k++) {
-                    const int sh = xs_shift(e[k], d[k] - e[k]);
+                    const int sh = (e[k] == 0 || d[k] > plain) ? e[k] : -(d[k] - e[k]);
                     x[k] = ldexp(x[k], sh);
                     y[k] = ldexp(y[k], sh);
                     back[k] = e[k] != 0 && sh == e[k];
@@ -1142,15 +1144,15 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
                     e[k] -= sh;
                 }
             }
-            bool anyx = false, deep = true;
+            bool anyx = false, all = true;
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                deep = deep && (!valid[k] || (e[k] != 0 && d[k] < XA_DEEP));
                 anyx = anyx || e[k] != 0;
+                all = all && (!valid[k] || e[k] != 0);
             }
             xph = wave_any(anyx);
-            alldeep = wave_all(deep);
-            dead = dead && alldeep;
+            allx = wave_all(all);
+            dead = dead && allx;
             if (wave_any(anyback)) {
                 // (the order and ring base through readfirstlane, so that the
                 // addresses are formed in this rare branch)
@@ -1162,6 +1164,37 @@ __device__ __forceinline__ void sha_item(const ShaArgs &A, int tile, int rbase, 
                                  ampS[k], ammS[k]);
             }
         }
+    }
+    // Every ring plain: two degrees at a time
+    for (; n + 7 <= nmax; n += 8) {
+        double an[8];
+        load_alpha8_const(calc + (n + 15 <= nmax ? n + 8 : n), an);
+        if (PIPE) rd.begin();
+#pragma unroll
+        for (int j = 0; j < 8; j += 2) {
+            double c0 = 0.0, s0 = 0.0, c1 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const double z = fma(a[j] * x[k], t[k], -y[k]);
+                c0 = fma(z, ammC[k], c0);
+                s0 = fma(z, ammS[k], s0);
+                const double w = fma(a[j + 1] * z, t[k], -x[k]);
+                c1 = fma(w, ampC[k], c1);
+                s1 = fma(w, ampS[k], s1);
+                y[k] = z;
+                x[k] = w;
+            }
+            rd.put(j, c0, s0);
+            rd.put(j + 1, c1, s1);
+        }
+        if (PIPE) rd.finish();
+        rd.commit(8, dst + 2 * n);
+        if (!PIPE) {
+            rd.begin();
+            rd.finish();
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) a[j] = an[j];
     }
     if (A.dead_mode == 1 && lane == 0) A.dead[(size_t)m * A.ntile + tile] = dead ? 1 : 0;
     // last, partial tile
