@@ -54,8 +54,8 @@ UNIT = "(nmax+1)^2*nlat/s"
 # separate runs and scaled by a 256 MiB device copy measured in the same runs
 # (HBM bytes = 2.0 FETCH_SIZE + 1.0 WRITE_SIZE).  Analysis writes one partial
 # triangle per ring-tile slot.
-TRAFFIC_GL2160 = {("sha_kernel", 6): 349.7e6, ("sha_kernel", 5): 367.3e6,
-                  ("shs_grid_kernel", 3): 189.8e6}
+TRAFFIC_GL2160 = {("sha_kernel", 6): 345.7e6, ("sha_kernel", 5): 367.3e6,
+                  ("shs_grid_kernel", 3): 189.2e6}
 
 
 def last_ring_k(lib, name):
