@@ -388,7 +388,8 @@ def test_device_grid_one_slot_rescale_roundtrip():
     to n = 8200, so the round trip is well conditioned, while a rescale left
     out would be off by 4e-10 of max|C| already at degree 2.  (At R + 5 km the
     factor reaches 1.6e-3 and both analysis paths, one slot or several, lose
-    1e-11 of max|C| in the high zonal degrees.)"""
+    1e-11 of max|C| in the high zonal degrees.)  Then the per-degree relative
+    error of a second round trip, with U[-1, 1) coefficients."""
     from charm_amd.device import DevArray, DeviceGrid, Stream
 
     N = 8200
@@ -406,6 +407,30 @@ def test_device_grid_one_slot_rescale_roundtrip():
     err = max(np.abs(o[:, 0] - C).max(), np.abs(o[:, 1] - S).max()) / np.abs(C).max()
     print("nmax 8200, r0 = R + 500 m: round trip %.2e" % err)
     assert err <= TOL
+    # Per degree (as CHarm's dda, src/shc/shc_dda.c:16-36), on coefficients
+    # drawn from U[-1, 1) as CHarm's accuracy protocol draws them, so that
+    # every degree carries signal (Kaula's fall as 1e-5 / n^2, below the
+    # rounding of C00 = 1 at high degree): the error over a degree's orders
+    # against that degree's coefficients.  A wrong s_nm offset, a missing
+    # rescale or a missed write at any degree would show as an O(1e-4 n) or
+    # O(1) relative error there.
+    rng = np.random.default_rng(8201)
+    U = rng.uniform(-1.0, 1.0, (C.size, 2))
+    m_of = np.repeat(np.arange(N + 1), np.arange(N + 1, 0, -1))
+    U[m_of == 0, 1] = 0.0
+    cu = DevArray.from_numpy(U)
+    dg.shs(cu, N, GM, R_EARTH, f, st)
+    dg.sha(f, N, GM, R_EARTH, out, st)
+    st.synchronize()
+    o = out.numpy()
+    first = np.concatenate(([0], np.cumsum(np.arange(N + 1, 0, -1))[:-1]))
+    n_of = m_of + (np.arange(C.size) - np.repeat(first, np.arange(N + 1, 0, -1)))
+    d2 = np.bincount(n_of, ((o - U) ** 2).sum(axis=1), N + 1)
+    c2 = np.bincount(n_of, (U ** 2).sum(axis=1), N + 1)
+    rel = np.sqrt(d2[1:] / c2[1:])
+    print("U[-1,1): per-degree relative error max %.2e (degree %d), degree 0 %.2e"
+          % (rel.max(), 1 + int(rel.argmax()), abs(o[0, 0] - U[0, 0])))
+    assert rel.max() <= 1e-10 and abs(o[0, 0] - U[0, 0]) <= 1e-10
 
 
 @pytest.mark.parametrize("nlon", (20000, 21602, 24570, 10802, 19682))
